@@ -1,0 +1,185 @@
+/* libsvimx -- MI355X-native narrow-band level-set erosion (C ABI).
+ *
+ * Every entry point takes the HIP stream as `void*` (a hipStream_t), raw
+ * device pointers, int64 sizes and double scalars, and returns 0 or a
+ * negative SVIMX_E* code; svimx_last_error() describes the last failure of
+ * the calling thread.  No C++ exception crosses this boundary.
+ *
+ * Layout: dense fields are float32 C-order (x, y, z) grids of nx*ny*nz nodes
+ * (flat strides ny*nz, nz, 1); a band is a sorted int32 list of flat node
+ * indices; points are float32 index (cell) coordinates, 3 per point.
+ *
+ * Ownership: the caller allocates every buffer (the Python host uses the
+ * PyTorch caching allocator) including one workspace per job of
+ * svimx_workspace_bytes() bytes whose first SVIMX_R_COUNT doubles hold the
+ * SVIMX_R_* results.  Plans (svimx_nr_plan_*) own their small device tables.
+ *
+ * Asynchrony: functions enqueue on `stream` and return, except those whose
+ * comment says "synchronises" (they report a count the host needs).
+ *
+ * The reference interface each group replaces is cited as file:line of
+ * jwcalder/SVIErosion.
+ */
+#ifndef SVIMX_H_
+#define SVIMX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SVIMX_OK = 0,
+  SVIMX_EBADARG = -1,   /* invalid argument (maps to ValueError) */
+  SVIMX_EMARGIN = -2,   /* band reaches the grid boundary (maps to RuntimeError) */
+  SVIMX_ECAPACITY = -3, /* an output buffer is too small; required size reported */
+  SVIMX_EDEVICE = -4    /* HIP runtime error */
+};
+
+/* Result slots (doubles) at the start of the workspace. */
+enum {
+  SVIMX_R_VOLUME_SUM = 0, /* sum over the band of cell fractions (FP64) */
+  SVIMX_R_HAS_NEG = 1,    /* band holds phi < 0 */
+  SVIMX_R_HAS_POS = 2,    /* band holds phi > 0 */
+  SVIMX_R_VMAX = 3,       /* max of V after the cutoff, over the band */
+  SVIMX_R_QMIN = 4,       /* min / max of N_r (SVI) or H (curvature) over |phi| < h */
+  SVIMX_R_QMAX = 5,
+  SVIMX_R_DVMAX = 6,      /* max dV/dN over |phi| < h (SVI) */
+  SVIMX_R_CAPPED = 7,     /* count of N > n_cap over |phi| < h */
+  SVIMX_R_VSUM = 8,       /* sum of V before the cutoff over |phi| < h (FP64) */
+  SVIMX_R_NNEAR = 9,      /* count of |phi| < h */
+  SVIMX_R_DMAX = 10,      /* max dV/dk1 + dV/dk2 over |phi| < h (curvature) */
+  SVIMX_R_QMIN_ALL = 11,  /* the same statistics over every evaluated band node, */
+  SVIMX_R_QMAX_ALL = 12,  /* used when no node has |phi| < h                      */
+  SVIMX_R_DVMAX_ALL = 13,
+  SVIMX_R_CAPPED_ALL = 14,
+  SVIMX_R_VSUM_ALL = 15,
+  SVIMX_R_NALL = 16,
+  SVIMX_R_DMAX_ALL = 17,
+  SVIMX_R_TILES = 18,     /* N_r tiles evaluated by the last svimx_nr_exec */
+  SVIMX_R_COUNT = 32
+};
+
+/* Speed law parameters (svierosion/flows.py:29-213). */
+typedef struct svimx_law {
+  int32_t kind; /* 0 SVI power law, 1 mean curvature, 2 Gauss curvature */
+  int32_t k;    /* SVI exponent */
+  int32_t s;    /* SVI geometric switch (0/1) */
+  int32_t reserved;
+  double t;     /* SVI singularity */
+  double eps;   /* additive constant */
+  double n_cap; /* SVI cap on N (1 when s = 0) */
+  double alpha; /* curvature exponent */
+  double delta; /* curvature regularisation */
+} svimx_law_t;
+
+const char* svimx_last_error(void);
+int svimx_version(void);
+int64_t svimx_workspace_bytes(int nx, int ny, int nz);
+
+/* ---------------------------------------------------------------- band
+ * Clamp phi to +-width*h, band = nodes with |phi| < width*h*(1-1e-5) (float32
+ * compare), write the sorted band, copy the clamped field into phi_copy1/2
+ * (may be NULL) and reset `occupancy` to (phi < 0).  Synchronises.
+ * info (host, 9 int64): band count, inside nodes outside the band, margin ok,
+ * min i/j/k and max i/j/k over the band.
+ * Replaces LevelSet.rebuild_band / _check_margin (svierosion/levelset.py:70-86);
+ * the margin failure is reported through info[2] and SVIMX_EMARGIN. */
+int svimx_band_rebuild(void* stream, float* phi, float* phi_copy1, float* phi_copy2, float* occupancy,
+                       int nx, int ny, int nz, double h, double width, int margin, int32_t* band,
+                       int64_t band_cap, void* ws, int64_t ws_bytes, int64_t* info);
+
+/* ---------------------------------------------------------------- per-stage kernels
+ * Cell fractions of the band written into `occupancy`, their FP64 sum
+ * (SVIMX_R_VOLUME_SUM) and the interface flags.  When `nr_plan` is not NULL the
+ * plan's tiles are reset and the tiles holding the tricubic stencils of the
+ * closest points of band nodes with |phi| < cut_hi*h are marked for svimx_nr_exec.
+ * Replaces LevelSet.cell_fractions / dense_fractions / volume / has_interface
+ * (svierosion/levelset.py:93-126, svierosion/kernels.py:173-181). */
+int svimx_fractions(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
+                    int nz, double h, float* occupancy, void* nr_plan, double cut_hi, void* ws,
+                    int64_t ws_bytes);
+
+/* Principal curvatures of the band scattered into dense k1/k2 fields that are
+ * zero elsewhere (svierosion/evolve.py:136-143, svierosion/kernels.py:274-299). */
+int svimx_curvatures(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
+                     int nz, double h, float* k1_dense, float* k2_dense);
+
+/* Velocity of every band node: closest point, interpolated N_r (tricubic) or
+ * k1/k2 (trilinear), speed law, smoothstep cutoff cut_lo*h -> cut_hi*h; with
+ * stats != 0 the SVIMX_R_VMAX..SVIMX_R_DMAX_ALL results.  `field` is the dense
+ * N_r for SVI laws; k1/k2 the dense curvature fields otherwise.
+ * Replaces ErosionSimulation._velocity (svierosion/evolve.py:108-155). */
+int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
+                   int nz, double h, const svimx_law_t* law, const float* field, const float* k1,
+                   const float* k2, double cut_lo, double cut_hi, int stats, float* V, void* ws,
+                   int64_t ws_bytes);
+
+/* One TVD-RK3 stage over the band (svierosion/evolve.py:157-160, :172-179):
+ *   L = V * |grad phi_in|_Godunov-WENO5
+ *   stage 0: out = u0 + dt L            (L saved into L0 when not NULL)
+ *   stage 1: out = 3/4 u0 + 1/4 (in + dt L)
+ *   stage 2: out = 1/3 u0 + 2/3 (in + dt L)
+ *   stage 3: out = u0 + dt L0           (replays stage 0 with a new dt; no stencil)
+ * u0 is the dense field at the start of the step; out may not alias in. */
+int svimx_rk_stage(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                   const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, double dt,
+                   const float* V, float* L0);
+
+/* ---------------------------------------------------------------- N_r (ball convolution)
+ * A plan holds the FP32 ball weights (svierosion/svi.py:24-55, computed on the
+ * host in FP64) as a per-column table, the tile marks and the tile list.
+ * svimx_nr_exec evaluates N_r = 1 - h^3 (w * f)(x) / (4/3 pi r^3) by direct
+ * summation at every node of every marked tile and writes it into nr_out
+ * (other nodes untouched).  Replaces BallConvolution (svierosion/svi.py:58-93). */
+int svimx_nr_plan_create(const double* weights, int m, double h, double radius, int nx, int ny, int nz,
+                         void** plan);
+int svimx_nr_plan_destroy(void* plan);
+int svimx_nr_reset(void* stream, void* plan);
+int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t n);
+int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws,
+                  int64_t ws_bytes);
+
+/* ---------------------------------------------------------------- reinitialisation
+ * Min (2010) PDE reinitialisation on the band dilated by `dilation` cells minus
+ * `margin` boundary layers: `iterations` TVD-RK2 pseudo-time iterations with
+ * ENO2 + subcell fix, in place on `phi`; `scratch` is a second dense field
+ * (overwritten).  work_idx/geo hold work_cap nodes (geo: 8 floats per node);
+ * mask_a/mask_b are dense byte fields.  Synchronises (work-set size in
+ * *n_work).  Replaces LevelSet.reinitialize (svierosion/levelset.py:154-183)
+ * minus the final rebuild_band, which the caller runs next. */
+int svimx_reinit(void* stream, float* phi, float* scratch, const int32_t* band, int64_t nb, int nx,
+                 int ny, int nz, double h, int iterations, double cfl, int dilation, int margin,
+                 uint8_t* mask_a, uint8_t* mask_b, int32_t* work_idx, float* geo, int64_t work_cap,
+                 void* ws, int64_t ws_bytes, int64_t* n_work);
+
+/* ---------------------------------------------------------------- operators
+ * Node-list forms of the reference's @kernel functions (svierosion/kernels.py),
+ * sharing the device code of the fused stage kernels; used by LevelSet methods
+ * and parity tests. */
+int svimx_op_godunov(void* stream, const float* phi, const int32_t* idx, int64_t n, int nx, int ny,
+                     int nz, double inv_h, float* out);                       /* :146-169 */
+int svimx_op_cell_fractions(void* stream, const float* phi, const int32_t* idx, int64_t n, int nx,
+                            int ny, int nz, float* out);                      /* :173-181 */
+int svimx_op_closest_points(void* stream, const float* phi, const int32_t* idx, int64_t n, int nx,
+                            int ny, int nz, double h, float* out_p);          /* :185-208 */
+int svimx_op_tricubic(void* stream, const float* field, int nx, int ny, int nz, const float* p,
+                      int64_t n, float* out);                                 /* :212-228 */
+int svimx_op_trilinear(void* stream, const float* field, int nx, int ny, int nz, const float* p,
+                       int64_t n, float* out);                                /* :257-270 */
+int svimx_op_principal_curvatures(void* stream, const float* phi, const int32_t* idx, int64_t n,
+                                  int nx, int ny, int nz, double h, float* k1,
+                                  float* k2);                                 /* :274-299 */
+int svimx_op_interface_mask(void* stream, const float* phi, const int32_t* idx, int64_t n, int nx,
+                            int ny, int nz, uint8_t* out);                    /* :303-309 */
+int svimx_op_reinit_geometry(void* stream, const float* phi, const int32_t* idx, int64_t n, int nx,
+                             int ny, int nz, double h, double cfl, float* geo); /* :335-359 */
+int svimx_op_point_triangle_distance(void* stream, const float* p, const float* tri, int64_t n,
+                                     float* out);                             /* :382-427 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVIMX_H_ */

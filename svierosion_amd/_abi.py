@@ -1,0 +1,130 @@
+"""ctypes binding of libsvimx (include/svimx.h).
+
+The library is built in-tree by ``svierosion_amd.build``; importing this
+module loads it and fails loudly if it is missing -- there is no fallback.
+Device buffers are torch tensors on the ROCm device; their ``data_ptr()`` and
+the current torch stream are passed through the C ABI.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(os.environ.get("SVIMX_LIB", Path(__file__).resolve().parent / "libsvimx.so"))
+
+OK, EBADARG, EMARGIN, ECAPACITY, EDEVICE = 0, -1, -2, -3, -4
+
+# result slots (SVIMX_R_*)
+R_VOLUME_SUM, R_HAS_NEG, R_HAS_POS, R_VMAX = 0, 1, 2, 3
+R_QMIN, R_QMAX, R_DVMAX, R_CAPPED, R_VSUM, R_NNEAR, R_DMAX = 4, 5, 6, 7, 8, 9, 10
+R_QMIN_ALL, R_QMAX_ALL, R_DVMAX_ALL, R_CAPPED_ALL, R_VSUM_ALL, R_NALL, R_DMAX_ALL = 11, 12, 13, 14, 15, 16, 17
+R_TILES = 18
+R_COUNT = 32
+
+
+class Law(C.Structure):
+    """svimx_law_t."""
+    _fields_ = [("kind", C.c_int32), ("k", C.c_int32), ("s", C.c_int32), ("reserved", C.c_int32),
+                ("t", C.c_double), ("eps", C.c_double), ("n_cap", C.c_double),
+                ("alpha", C.c_double), ("delta", C.c_double)]
+
+
+_P, _I, _I64, _D, _U8P = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_void_p
+
+_SIGNATURES = {
+    "svimx_last_error": (C.c_char_p, []),
+    "svimx_version": (_I, []),
+    "svimx_workspace_bytes": (_I64, [_I, _I, _I]),
+    "svimx_band_rebuild": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _I, _P, _I64, _P, _I64, _P]),
+    "svimx_fractions": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _P, _D, _P, _I64]),
+    "svimx_curvatures": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _P]),
+    "svimx_velocity": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _P, _D, _D, _I, _P,
+                            _P, _I64]),
+    "svimx_rk_stage": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P]),
+    "svimx_nr_plan_create": (_I, [_P, _I, _D, _D, _I, _I, _I, C.POINTER(C.c_void_p)]),
+    "svimx_nr_plan_destroy": (_I, [_P]),
+    "svimx_nr_reset": (_I, [_P, _P]),
+    "svimx_nr_mark_nodes": (_I, [_P, _P, _P, _I64]),
+    "svimx_nr_exec": (_I, [_P, _P, _P, _P, _P, _I64]),
+    "svimx_reinit": (_I, [_P, _P, _P, _P, _I64, _I, _I, _I, _D, _I, _D, _I, _I, _P, _P, _P, _P, _I64, _P, _I64,
+                          _P]),
+    "svimx_op_godunov": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
+    "svimx_op_cell_fractions": (_I, [_P, _P, _P, _I64, _I, _I, _I, _P]),
+    "svimx_op_closest_points": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
+    "svimx_op_tricubic": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P]),
+    "svimx_op_trilinear": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P]),
+    "svimx_op_principal_curvatures": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _P]),
+    "svimx_op_interface_mask": (_I, [_P, _P, _P, _I64, _I, _I, _I, _P]),
+    "svimx_op_reinit_geometry": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _D, _P]),
+    "svimx_op_point_triangle_distance": (_I, [_P, _P, _P, _I64, _P]),
+}
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m svierosion_amd.build` "
+                          "(svierosion_amd has no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class SvimxError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = "") -> None:
+    """Map a libsvimx return code onto the reference's exception types."""
+    if rc == OK:
+        return
+    msg = lib.svimx_last_error().decode(errors="replace")
+    if what:
+        msg = f"{what}: {msg}"
+    if rc == EBADARG:
+        raise ValueError(msg)
+    if rc == EMARGIN:
+        raise RuntimeError(msg)
+    raise SvimxError(f"libsvimx error {rc}: {msg}")
+
+
+def ptr(t):
+    """Raw pointer of a tensor (None -> NULL)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def stream():
+    """hipStream_t of torch's current stream on the current device."""
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def require_device(device) -> torch.device:
+    dev = torch.device(device)
+    if dev.type != "cuda" or not torch.cuda.is_available():
+        raise RuntimeError("svierosion_amd computes on a ROCm GPU; no GPU device is available "
+                           f"(requested {dev}) and there is no CPU fallback")
+    return dev
+
+
+def workspace(shape, device) -> torch.Tensor:
+    nbytes = int(lib.svimx_workspace_bytes(*map(int, shape)))
+    return torch.zeros(nbytes, dtype=torch.uint8, device=device)
+
+
+def results_view(ws: torch.Tensor) -> torch.Tensor:
+    return ws[: 8 * R_COUNT].view(torch.float64)
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)

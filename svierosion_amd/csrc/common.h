@@ -1,0 +1,125 @@
+// Host-side plumbing shared by the libsvimx translation units: error state,
+// launch checks, reduction helpers and the workspace layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/svimx.h"
+#include "ops.h"
+
+namespace svimx {
+
+void set_error(const char* fmt, ...);
+
+#define SVIMX_CHECK_HIP(expr)                                                       \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      ::svimx::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),     \
+                         __FILE__, __LINE__);                                       \
+      return SVIMX_EDEVICE;                                                         \
+    }                                                                               \
+  } while (0)
+
+#define SVIMX_CHECK_LAUNCH() SVIMX_CHECK_HIP(hipGetLastError())
+
+#define SVIMX_REQUIRE(cond, code, ...)         \
+  do {                                         \
+    if (!(cond)) {                             \
+      ::svimx::set_error(__VA_ARGS__);         \
+      return (code);                           \
+    }                                          \
+  } while (0)
+
+#define SVIMX_TRY_BEGIN try {
+#define SVIMX_TRY_END                                          \
+  }                                                            \
+  catch (const std::exception& ex) {                           \
+    ::svimx::set_error("exception: %s", ex.what());            \
+    return SVIMX_EDEVICE;                                      \
+  }                                                            \
+  catch (...) {                                                \
+    ::svimx::set_error("unknown exception");                   \
+    return SVIMX_EDEVICE;                                      \
+  }
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
+
+// ---------------------------------------------------------------- workspace
+// One per job, caller-allocated (svimx_workspace_bytes).  Fixed carve-up:
+//   [0, 4 KiB)            results (doubles, SVIMX_R_* slots)
+//   partials              kPartialSlots doubles per block of kBandThreads
+//   compaction counts     one int64 per compaction chunk (+ scan)
+struct Workspace {
+  double* results;
+  double* partials;
+  int64_t max_partial_blocks;
+  int64_t* chunk_counts;
+  int64_t max_chunks;
+};
+
+constexpr int kBandThreads = 256;       // threads per block of band-list kernels
+constexpr int kPartialSlots = 16;       // doubles written per block by reduction kernels
+constexpr int kCompactThreads = 256;
+constexpr int kCompactPerThread = 16;
+constexpr int64_t kCompactChunk = (int64_t)kCompactThreads * kCompactPerThread;
+
+int64_t workspace_bytes_for(int64_t n_nodes);
+bool carve_workspace(void* ws, int64_t ws_bytes, int64_t n_nodes, Workspace& out);
+
+// ---------------------------------------------------------------- device reductions
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int64_t wave_sum_i(int64_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block reduction through LDS for blockDim.x == kBandThreads (4 waves).
+// op: 0 sum, 1 max, 2 min.  Result valid in thread 0.
+template <int OP>
+__device__ __forceinline__ double block_reduce(double v, double* lds4) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double u = __shfl_xor(v, o, 64);
+    v = OP == 0 ? v + u : (OP == 1 ? fmax(v, u) : fmin(v, u));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds4[wave] = v;
+  __syncthreads();
+  double r = lds4[0];
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int w = 1; w < nw; ++w) {
+    double u = lds4[w];
+    r = OP == 0 ? r + u : (OP == 1 ? fmax(r, u) : fmin(r, u));
+  }
+  return r;
+}
+
+// Launch-wide finalisation: reduce nblocks x kPartialSlots partials in a fixed
+// order (deterministic) and store into results[dst + slot] with per-slot op.
+// ops[slot]: 0 sum, 1 max, 2 min, -1 skip.
+struct ReduceSpec {
+  int8_t op[kPartialSlots];
+  int8_t dst[kPartialSlots];
+};
+int launch_finalize(hipStream_t st, const double* partials, int64_t nblocks, double* results,
+                    const ReduceSpec& spec);
+
+}  // namespace svimx

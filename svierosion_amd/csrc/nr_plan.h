@@ -1,0 +1,44 @@
+// N_r plan: ball weights laid out per (dx, dy) column for the direct
+// convolution, plus the tile marks that select where N_r is evaluated.
+#pragma once
+
+#include "common.h"
+
+namespace svimx {
+
+constexpr int kNrTile = 4;  // N_r is evaluated on 4x4x4 node tiles
+
+struct NrColumn {
+  int dx, dy;     // column offset
+  int dz0, nz;    // first dz and number of nonzero taps
+  int woff;       // offset into the weight array
+};
+
+struct NrPlan {
+  int m;
+  double h, radius;
+  Dims d;
+  int ntx, nty, ntz;
+  int64_t ntiles;
+  int ncols;
+  int nweights;
+  float scale;   // float32(h^3)
+  float negbv;   // float32(-1 / (4/3 pi r^3))
+  NrColumn* cols;    // device
+  float* weights;    // device
+  uint8_t* marks;    // device, ntiles
+  int32_t* list;     // device, ntiles
+  int32_t* count;    // device, 1
+};
+
+// Mark the tiles covering the tricubic stencil [base, base+3]^3 of a point.
+__device__ __forceinline__ void mark_stencil_tiles(uint8_t* marks, int nty, int ntz, int bx, int by, int bz) {
+  int tx0 = bx / kNrTile, tx1 = (bx + 3) / kNrTile;
+  int ty0 = by / kNrTile, ty1 = (by + 3) / kNrTile;
+  int tz0 = bz / kNrTile, tz1 = (bz + 3) / kNrTile;
+  for (int tx = tx0; tx <= tx1; ++tx)
+    for (int ty = ty0; ty <= ty1; ++ty)
+      for (int tz = tz0; tz <= tz1; ++tz) marks[((int64_t)tx * nty + ty) * ntz + tz] = 1;
+}
+
+}  // namespace svimx

@@ -1,0 +1,366 @@
+// Per-node device math shared by every kernel of libsvimx.
+//
+// Layout conventions: phi and every dense field are C-order (x, y, z) float32
+// grids with flat strides (ny*nz, nz, 1); a band is a sorted int32 list of flat
+// node indices; points are in index (cell) coordinates.
+//
+// The formulas restate the reference's stencils (file:line cited per
+// function) and keep their operation order, so float32 results agree with the
+// reference's single-precision path to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace svimx {
+
+struct Dims {
+  int nx, ny, nz;
+  __host__ __device__ int64_t sx() const { return (int64_t)ny * nz; }
+  __host__ __device__ int64_t sy() const { return nz; }
+  __host__ __device__ int64_t size() const { return (int64_t)nx * ny * nz; }
+};
+
+__device__ __forceinline__ void unflat(const Dims& d, int64_t n, int& i, int& j, int& k) {
+  int64_t syz = (int64_t)d.ny * d.nz;
+  i = (int)(n / syz);
+  int64_t r = n - (int64_t)i * syz;
+  j = (int)(r / d.nz);
+  k = (int)(r - (int64_t)j * d.nz);
+}
+
+__device__ __forceinline__ float sq(float x) { return x * x; }
+
+// Jiang-Peng WENO5 derivative from five one-sided differences
+// (svierosion/kernels.py:83-97).
+__device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5) {
+  float p1 = v1 / 3.0f - 7.0f * v2 / 6.0f + 11.0f * v3 / 6.0f;
+  float p2 = -v2 / 6.0f + 5.0f * v3 / 6.0f + v4 / 3.0f;
+  float p3 = v3 / 3.0f + 5.0f * v4 / 6.0f - v5 / 6.0f;
+  float s1 = 13.0f / 12.0f * sq(v1 - 2.0f * v2 + v3) + 0.25f * sq(v1 - 4.0f * v2 + 3.0f * v3);
+  float s2 = 13.0f / 12.0f * sq(v2 - 2.0f * v3 + v4) + 0.25f * sq(v2 - v4);
+  float s3 = 13.0f / 12.0f * sq(v3 - 2.0f * v4 + v5) + 0.25f * sq(3.0f * v3 - 4.0f * v4 + v5);
+  float big = fmaxf(fmaxf(fmaxf(v1 * v1, v2 * v2), fmaxf(v3 * v3, v4 * v4)), v5 * v5);
+  float eps = 1e-6f * big + 1e-10f;  // (s + eps)^2 stays a normal float
+  float a1 = 0.1f / sq(s1 + eps);
+  float a2 = 0.6f / sq(s2 + eps);
+  float a3 = 0.3f / sq(s3 + eps);
+  return (a1 * p1 + a2 * p2 + a3 * p3) / (a1 + a2 + a3);
+}
+
+// Godunov |grad phi| for phi_t = V |grad phi|, V >= 0, WENO5 one-sided
+// derivatives (svierosion/kernels.py:146-169).
+__device__ __forceinline__ float godunov_inward(const float* __restrict__ phi, int64_t n,
+                                                const Dims& d, float inv_h) {
+  const float c = phi[n];
+  const int64_t st[3] = {d.sx(), d.sy(), 1};
+  float g2 = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t s = st[a];
+    float m3 = phi[n - 3 * s], m2 = phi[n - 2 * s], m1 = phi[n - s];
+    float p1 = phi[n + s], p2 = phi[n + 2 * s], p3 = phi[n + 3 * s];
+    float d0 = (m2 - m3) * inv_h, d1 = (m1 - m2) * inv_h, d2 = (c - m1) * inv_h;
+    float d3 = (p1 - c) * inv_h, d4 = (p2 - p1) * inv_h, d5 = (p3 - p2) * inv_h;
+    float dm = weno5(d0, d1, d2, d3, d4);
+    float dp = weno5(d5, d4, d3, d2, d1);
+    g2 = g2 + fmaxf(sq(fminf(dm, 0.0f)), sq(fmaxf(dp, 0.0f)));
+  }
+  return sqrtf(g2);
+}
+
+// Fraction of the unit cube on the side n.xi < -s of a plane; a >= b >= c >= 0
+// are the sorted |n_i| (svierosion/kernels.py:104-122, report/report.tex:337-365).
+__device__ __forceinline__ float cube_plane_fraction(float s, float a, float b, float c) {
+  float M = 0.5f * (a + b + c);
+  float y = M - s;
+  bool flip = y > M;
+  y = flip ? 2.0f * M - y : y;
+  y = fmaxf(y, 0.0f);
+  const float tiny = 1e-12f;
+  float bs = fmaxf(b, tiny), cs = fmaxf(c, tiny);
+  float q = 3.0f * y * y - 3.0f * y * c + c * c;
+  float H;
+  if (y <= c) {
+    H = y * y * y / (bs * cs);
+  } else if (y <= b) {
+    H = q / bs;
+  } else if (y <= b + c) {
+    float u = fmaxf(y - b, 0.0f);
+    H = (q - u * u * u / cs) / bs;
+  } else {
+    H = 3.0f * (2.0f * y - b - c);
+  }
+  float ua = fmaxf(y - a, 0.0f);
+  float F = H / (6.0f * a) - ua * ua * ua / (6.0f * a * bs * cs);
+  F = flip ? 1.0f - F : F;
+  return fminf(fmaxf(F, 0.0f), 1.0f);
+}
+
+__device__ __forceinline__ void sort3_abs(float x, float y, float z, float& hi, float& mid, float& lo) {
+  x = fabsf(x); y = fabsf(y); z = fabsf(z);
+  hi = fmaxf(fmaxf(x, y), z);
+  lo = fminf(fminf(x, y), z);
+  mid = x + y + z - hi - lo;
+}
+
+struct Grad {
+  float c, gx, gy, gz;
+};
+
+__device__ __forceinline__ Grad central(const float* __restrict__ phi, int64_t n, const Dims& d) {
+  Grad g;
+  g.c = phi[n];
+  g.gx = 0.5f * (phi[n + d.sx()] - phi[n - d.sx()]);
+  g.gy = 0.5f * (phi[n + d.sy()] - phi[n - d.sy()]);
+  g.gz = 0.5f * (phi[n + 1] - phi[n - 1]);
+  return g;
+}
+
+// Tangent-plane cell fraction (svierosion/kernels.py:173-181).
+__device__ __forceinline__ float cell_fraction(const Grad& g) {
+  float gn = fmaxf(sqrtf(g.gx * g.gx + g.gy * g.gy + g.gz * g.gz), 1e-30f);
+  float a, b, c;
+  sort3_abs(g.gx / gn, g.gy / gn, g.gz / gn, a, b, c);
+  return cube_plane_fraction(g.c / gn, a, b, c);
+}
+
+// First capped Newton projection onto the zero set, clamped to [1, n-3.001]
+// (svierosion/kernels.py:185-208).  hi[] = n - 3.001 per axis.
+__device__ __forceinline__ void closest_point(const Grad& g, int i, int j, int k, float h,
+                                              const float hi[3], float p[3]) {
+  float g2 = fmaxf(g.gx * g.gx + g.gy * g.gy + g.gz * g.gz, 1e-30f);
+  float ac = fabsf(g.c);
+  float length = ac / sqrtf(g2);
+  float cap = 1.25f * ac / h + 0.25f;
+  float scale = (g.c / g2) * fminf(cap / fmaxf(length, 1e-30f), 1.0f);
+  p[0] = fminf(fmaxf((float)i - scale * g.gx, 1.0f), hi[0]);
+  p[1] = fminf(fmaxf((float)j - scale * g.gy, 1.0f), hi[1]);
+  p[2] = fminf(fmaxf((float)k - scale * g.gz, 1.0f), hi[2]);
+}
+
+// Lagrange cubic weights on nodes -1, 0, 1, 2 (svierosion/kernels.py:134-137).
+__device__ __forceinline__ void cubic_w(float t, float w[4]) {
+  w[0] = -t * (t - 1.0f) * (t - 2.0f) / 6.0f;
+  w[1] = (t + 1.0f) * (t - 1.0f) * (t - 2.0f) / 2.0f;
+  w[2] = -(t + 1.0f) * t * (t - 2.0f) / 2.0f;
+  w[3] = (t + 1.0f) * t * (t - 1.0f) / 6.0f;
+}
+
+// Tricubic interpolation at index point p (svierosion/kernels.py:212-228).
+__device__ __forceinline__ float tricubic(const float* __restrict__ f, const Dims& d, const float p[3]) {
+  float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+  float wx[4], wy[4], wz[4];
+  cubic_w(p[0] - fx, wx);
+  cubic_w(p[1] - fy, wy);
+  cubic_w(p[2] - fz, wz);
+  int64_t b0 = ((int64_t)fx - 1) * d.sx() + ((int64_t)fy - 1) * d.sy() + ((int64_t)fz - 1);
+  float out = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      float wab = wx[a] * wy[b];
+      const float* row = f + b0 + a * d.sx() + b * d.sy();
+#pragma unroll
+      for (int c = 0; c < 4; ++c) out = out + (wab * wz[c]) * row[c];
+    }
+  }
+  return out;
+}
+
+// Trilinear interpolation at index point p (svierosion/kernels.py:257-270).
+__device__ __forceinline__ float trilinear(const float* __restrict__ f, const Dims& d, const float p[3]) {
+  float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+  float t0 = p[0] - fx, t1 = p[1] - fy, t2 = p[2] - fz;
+  int64_t b0 = (int64_t)fx * d.sx() + (int64_t)fy * d.sy() + (int64_t)fz;
+  float out = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    float wa = a ? t0 : 1.0f - t0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      float wb = wa * (b ? t1 : 1.0f - t1);
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        out = out + wb * (c ? t2 : 1.0f - t2) * f[b0 + a * d.sx() + b * d.sy() + c];
+    }
+  }
+  return out;
+}
+
+// Principal curvatures k1 >= k2 of the level set through node n from second
+// central differences (svierosion/kernels.py:274-299).
+__device__ __forceinline__ void principal_curvatures(const float* __restrict__ f, int64_t n,
+                                                     const Dims& d, float h, float& k1, float& k2) {
+  const int64_t sx = d.sx(), sy = d.sy(), sz = 1;
+  float c = f[n];
+  float xp = f[n + sx], xm = f[n - sx], yp = f[n + sy], ym = f[n - sy], zp = f[n + sz], zm = f[n - sz];
+  float px = (xp - xm) / (2.0f * h), py = (yp - ym) / (2.0f * h), pz = (zp - zm) / (2.0f * h);
+  float hh = h * h;
+  float pxx = (xp - 2.0f * c + xm) / hh;
+  float pyy = (yp - 2.0f * c + ym) / hh;
+  float pzz = (zp - 2.0f * c + zm) / hh;
+  float q4 = 4.0f * h * h;
+  float pxy = (f[n + sx + sy] - f[n + sx - sy] - f[n - sx + sy] + f[n - sx - sy]) / q4;
+  float pxz = (f[n + sx + sz] - f[n + sx - sz] - f[n - sx + sz] + f[n - sx - sz]) / q4;
+  float pyz = (f[n + sy + sz] - f[n + sy - sz] - f[n - sy + sz] + f[n - sy - sz]) / q4;
+  float g2 = fmaxf(px * px + py * py + pz * pz, 1e-30f);
+  float gn = sqrtf(g2);
+  float ksum = ((pyy + pzz) * px * px + (pxx + pzz) * py * py + (pxx + pyy) * pz * pz -
+                2.0f * (px * py * pxy + px * pz * pxz + py * pz * pyz)) / (g2 * gn);
+  float K = (px * px * (pyy * pzz - pyz * pyz) + py * py * (pxx * pzz - pxz * pxz) +
+             pz * pz * (pxx * pyy - pxy * pxy) + 2.0f * px * py * (pxz * pyz - pxy * pzz) +
+             2.0f * py * pz * (pxy * pxz - pyz * pxx) + 2.0f * px * pz * (pxy * pyz - pxz * pyy)) /
+            (g2 * g2);
+  float H = 0.5f * ksum;
+  float r = sqrtf(fmaxf(H * H - K, 0.0f));
+  k1 = H + r;
+  k2 = H - r;
+}
+
+// ---------------------------------------------------------------- reinitialisation
+__device__ __forceinline__ float minmod(float a, float b) {
+  return (a * b > 0.0f) ? (fabsf(a) < fabsf(b) ? a : b) : 0.0f;
+}
+
+// Quadratic-ENO distance from a node (p0) to the zero towards a neighbour (p1)
+// (Min 2010; svierosion/kernels.py:312-331).
+__device__ __forceinline__ float interface_distance(float p0, float p1, float dd, float h) {
+  float den = p0 - p1;
+  float lin = h * p0 / (den != 0.0f ? den : 1e-30f);
+  float c2 = 0.5f * dd / (h * h);
+  float c1 = (p1 - p0) / h;
+  float c0 = 0.5f * (p0 + p1) - c2 * h * h / 4.0f;
+  float disc = c1 * c1 - 4.0f * c2 * c0;
+  float sqd = sqrtf(fmaxf(disc, 0.0f));
+  float q = -0.5f * (c1 + (c1 >= 0.0f ? sqd : -sqd));
+  float xi = c0 / (q != 0.0f ? q : 1e-30f);
+  float quad = h / 2.0f + xi;
+  bool ok = (disc >= 0.0f) && (fabsf(dd) > 1e-10f * h * h) && (quad >= 0.0f) && (quad <= h);
+  return fminf(fmaxf(ok ? quad : lin, 0.0f), h);
+}
+
+// Reinit geometry of one node: flags bit (2a) = crossing towards +axis a,
+// bit (2a+1) = towards -axis a; bits 6-7 encode sgn (1 -> +1, 2 -> -1).
+// x[2a] / x[2a+1] = subcell distances (svierosion/kernels.py:335-359).
+struct ReinitGeo {
+  float dtau;
+  float x[6];
+  uint32_t flags;
+};
+
+__device__ __forceinline__ ReinitGeo reinit_geometry(const float* __restrict__ f, int64_t n,
+                                                     const Dims& d, float h, float cfl) {
+  ReinitGeo g;
+  const int64_t st[3] = {d.sx(), d.sy(), 1};
+  float c = f[n];
+  float hmin = h;
+  uint32_t flags = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int64_t s = st[a];
+    float m2 = f[n - 2 * s], m1 = f[n - s], p1 = f[n + s], p2 = f[n + 2 * s];
+    float d_m1 = c - 2.0f * m1 + m2;
+    float d_0 = p1 - 2.0f * c + m1;
+    float d_p1 = p2 - 2.0f * p1 + c;
+    bool cp = (c * p1) < 0.0f;
+    bool cm = (c * m1) < 0.0f;
+    float xp = interface_distance(c, p1, minmod(d_0, d_p1), h);
+    float xm = interface_distance(c, m1, minmod(d_0, d_m1), h);
+    if (cp) hmin = fminf(hmin, xp);
+    if (cm) hmin = fminf(hmin, xm);
+    flags |= (cp ? 1u : 0u) << (2 * a);
+    flags |= (cm ? 1u : 0u) << (2 * a + 1);
+    g.x[2 * a] = fmaxf(xp, 1e-6f * h);
+    g.x[2 * a + 1] = fmaxf(xm, 1e-6f * h);
+  }
+  flags |= (c > 0.0f ? 1u : (c < 0.0f ? 2u : 0u)) << 6;
+  g.flags = flags;
+  g.dtau = cfl * fmaxf(hmin, 1e-3f * h);
+  return g;
+}
+
+// -sgn(phi0)(|grad phi| - 1) with ENO2 differences and the subcell fix
+// (svierosion/kernels.py:363-379).
+__device__ __forceinline__ float reinit_rhs(const float* __restrict__ f, int64_t n, const Dims& d,
+                                            float h, uint32_t flags, const float x[6]) {
+  const int64_t st[3] = {d.sx(), d.sy(), 1};
+  float c = f[n];
+  uint32_t sb = (flags >> 6) & 3u;
+  float sgn = sb == 1u ? 1.0f : (sb == 2u ? -1.0f : 0.0f);
+  bool outside = sgn > 0.0f;
+  float g2 = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int64_t s = st[a];
+    float m2 = f[n - 2 * s], m1 = f[n - s], p1 = f[n + s], p2 = f[n + 2 * s];
+    float mm_p = minmod(p1 - 2.0f * c + m1, p2 - 2.0f * p1 + c);
+    float mm_m = minmod(p1 - 2.0f * c + m1, c - 2.0f * m1 + m2);
+    bool cp = (flags >> (2 * a)) & 1u;
+    bool cm = (flags >> (2 * a + 1)) & 1u;
+    float xp = x[2 * a], xm = x[2 * a + 1];
+    float Dp = cp ? (-c / xp - 0.5f * xp * mm_p / (h * h)) : ((p1 - c - 0.5f * mm_p) / h);
+    float Dm = cm ? (c / xm + 0.5f * xm * mm_m / (h * h)) : ((c - m1 + 0.5f * mm_m) / h);
+    float pos = fmaxf(sq(fmaxf(Dm, 0.0f)), sq(fminf(Dp, 0.0f)));
+    float neg = fmaxf(sq(fminf(Dm, 0.0f)), sq(fmaxf(Dp, 0.0f)));
+    g2 = g2 + (outside ? pos : neg);
+  }
+  return -sgn * (sqrtf(g2) - 1.0f);
+}
+
+// ---------------------------------------------------------------- point-triangle distance
+// Squared distance from p to triangle (a, b, c), Ericson 5.1.5 region tests with
+// the reference's priority order (svierosion/kernels.py:382-427).  Inputs are
+// in a coordinate frame local to the triangle so float32 keeps ~1e-7 cells.
+__device__ __forceinline__ float point_triangle_d2(const float p[3], const float a[3],
+                                                   const float b[3], const float c[3]) {
+  float ab[3], ac[3], ap[3], bp[3], cp[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    ab[q] = b[q] - a[q]; ac[q] = c[q] - a[q];
+    ap[q] = p[q] - a[q]; bp[q] = p[q] - b[q]; cp[q] = p[q] - c[q];
+  }
+  float d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
+  float d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
+  float d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
+  float d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
+  float d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
+  float d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
+  float va = d3 * d6 - d5 * d4;
+  float vb = d5 * d2 - d1 * d6;
+  float vc = d1 * d4 - d3 * d2;
+  const float tiny = 1e-30f;
+  float den = va + vb + vc;
+  den = fabsf(den) > tiny ? den : tiny;
+  float v, w;
+  // highest priority first: the first matching region wins
+  if (d1 <= 0.0f && d2 <= 0.0f) {                       // vertex a
+    v = 0.0f; w = 0.0f;
+  } else if (d3 >= 0.0f && d4 <= d3) {                  // vertex b
+    v = 1.0f; w = 0.0f;
+  } else if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {  // edge ab
+    v = d1 / fmaxf(d1 - d3, tiny); w = 0.0f;
+  } else if (d6 >= 0.0f && d5 <= d6) {                  // vertex c
+    v = 0.0f; w = 1.0f;
+  } else if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {  // edge ac
+    v = 0.0f; w = d2 / fmaxf(d2 - d6, tiny);
+  } else {
+    float e1 = d4 - d3, e2 = d5 - d6;
+    if (va <= 0.0f && e1 >= 0.0f && e2 >= 0.0f) {        // edge bc
+      float t = e1 / fmaxf(e1 + e2, tiny);
+      v = 1.0f - t; w = t;
+    } else {                                             // interior
+      v = vb / den; w = vc / den;
+    }
+  }
+  float r = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    float qq = a[q] + v * ab[q] + w * ac[q];
+    r += sq(p[q] - qq);
+  }
+  return r;
+}
+
+}  // namespace svimx

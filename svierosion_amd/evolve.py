@@ -1,0 +1,284 @@
+"""Time integration of the erosion flows on the device (API of svierosion/evolve.py).
+
+Each RHS evaluation (one TVD-RK3 stage) is four libsvimx launches over the
+band plus the N_r summation over the marked tiles:
+
+    fractions (+ volume partials, interface flags, N_r tile marks)
+    N_r direct summation      (SVI laws)   | curvature scatter (curvature laws)
+    velocity (+ dt statistics on stage 0)
+    RK stage (Godunov/WENO5, L = V |grad phi|, stage combination)
+
+Buffers: the step-start field u0 stays in one dense copy (A); stage outputs go
+to B, C, B, so a secant retry restarts from A without a copy and replays
+stage 0 from the saved L0 (svierosion/evolve.py:215-231).  The host reads the
+results twice per accepted step: the dt statistics after stage 0 and the
+volume after stage 2; the fractions of the final state are reused by the next
+step's first stage.
+"""
+
+from __future__ import annotations
+
+import math
+import time as _time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _abi
+from .flows import SpeedLaw
+from .levelset import LevelSet
+from .svi import BallConvolution
+
+
+@dataclass
+class StepRecord:
+    step: int
+    time: float
+    dt: float
+    volume: float
+    fraction: float
+    vmax: float
+    vmean: float
+    qmin: float = float("nan")
+    qmax: float = float("nan")
+    capped: int = 0
+    reinit: bool = False
+    wall: float = 0.0
+
+
+@dataclass
+class RunResult:
+    stop_reason: str
+    steps: int
+    time: float
+    fraction: float
+    history: list = field(default_factory=list)
+
+
+class ErosionSimulation:
+    """Evolve a :class:`LevelSet` under a speed law (svierosion/evolve.py:67-250)."""
+
+    def __init__(self, levelset: LevelSet, law: SpeedLaw, cfl: float = 0.5, feedback_cfl: float = 1.0,
+                 parabolic_cfl: float = 0.25, reinit_distance: float = 1.5, reinit_iterations: int = 20,
+                 cutoff=(4.0, 5.5), newton_steps: int = 1):
+        self.ls = levelset
+        self.law = law
+        self.cfl = float(cfl)
+        self.feedback_cfl = float(feedback_cfl)
+        self.parabolic_cfl = float(parabolic_cfl)
+        self.reinit_distance = float(reinit_distance)
+        self.reinit_iterations = int(reinit_iterations)
+        self.cutoff = tuple(float(c) for c in cutoff)
+        if self.cutoff[1] > levelset.width:
+            raise ValueError("cutoff must lie inside the band")
+        if int(newton_steps) != 1:
+            raise NotImplementedError("the fused velocity kernel uses one projection step")
+        self.newton_steps = 1
+        self._law_abi = law.to_abi()
+        dev = levelset.device
+        n = levelset.grid.size
+        self.conv = None
+        self.field = None
+        self.k1 = self.k2 = None
+        if law.requires_svi:
+            self.conv = BallConvolution(levelset.grid, law.radius, device=dev)
+            self._check_convolution_margin()
+            self.field = torch.zeros(n, dtype=torch.float32, device=dev)
+        else:
+            self.k1 = torch.zeros(n, dtype=torch.float32, device=dev)
+            self.k2 = torch.zeros(n, dtype=torch.float32, device=dev)
+        self._alloc_band_arrays()
+        self._frac_valid = None     # buffer index whose fractions/marks are current
+        self.initial_volume = self.ls.volume()
+        self.time = 0.0
+        self.steps = 0
+        self.history: list[StepRecord] = []
+        self._displacement = 0.0
+        self.timers = {"stats_sync": 0.0, "volume_sync": 0.0}
+
+    # ------------------------------------------------------------- checks
+    def _check_convolution_margin(self) -> None:
+        """Circular and direct sums agree if the solid stays m+1 cells from the boundary."""
+        m = self.conv.m + 1
+        inside = self.ls.phi < 0
+        for ax in range(3):
+            other = tuple(a for a in range(3) if a != ax)
+            hit = torch.nonzero(inside.any(dim=other[1]).any(dim=other[0])).squeeze(1)
+            if hit.numel() == 0:
+                continue
+            if int(hit.min()) < m or int(hit.max()) > self.ls.dims[ax] - 1 - m:
+                raise ValueError("grid padding is smaller than the SVI ball radius; use a larger padding")
+
+    def _alloc_band_arrays(self) -> None:
+        cap = max(self.ls.nb, 1)
+        dev = self.ls.device
+        self.V = torch.empty(cap, dtype=torch.float32, device=dev)
+        self.L0 = torch.empty(cap, dtype=torch.float32, device=dev)
+
+    # ---------------------------------------------------------- stages
+    def _plan(self):
+        return self.conv.plan if self.conv is not None else None
+
+    def _prepare(self, buf: int, stats: bool) -> None:
+        """Fractions, N_r / curvatures and velocity for the field in bufs[buf]."""
+        ls = self.ls
+        st = _abi.stream()
+        saved = ls.cur
+        ls.cur = buf
+        try:
+            if self._frac_valid != buf:
+                ls.fractions(self._plan(), self.cutoff[1])
+                self._frac_valid = buf
+            phi = ls.flat
+            if self.conv is not None:
+                self.conv.exec_marked(ls.occupancy, self.field, ls.ws)
+            else:
+                _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
+                                                     _abi.ptr(self.k1), _abi.ptr(self.k2)))
+            _abi.check(_abi.lib.svimx_velocity(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
+                                               self._law_abi, _abi.ptr(self.field), _abi.ptr(self.k1),
+                                               _abi.ptr(self.k2), self.cutoff[0], self.cutoff[1], int(stats),
+                                               _abi.ptr(self.V), _abi.ptr(ls.ws), ls.ws.numel()), "velocity")
+        finally:
+            ls.cur = saved
+
+    def _stage(self, stage: int, src: int, u0: int, dst: int, dt: float) -> None:
+        ls = self.ls
+        _abi.check(_abi.lib.svimx_rk_stage(_abi.stream(), stage, _abi.ptr(ls.bufs[src]), _abi.ptr(ls.bufs[u0]),
+                                           _abi.ptr(ls.bufs[dst]), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
+                                           float(dt), _abi.ptr(self.V), _abi.ptr(self.L0)), "rk stage")
+        if self._frac_valid == dst:
+            self._frac_valid = None
+
+    def _rk3_tail(self, a: int, b: int, c: int, dt: float) -> None:
+        """Stages 1 and 2 after stage 0 wrote bufs[b]; the result lands in bufs[b]."""
+        self._prepare(b, False)
+        self._stage(1, b, a, c, dt)
+        self._prepare(c, False)
+        self._stage(2, c, a, b, dt)
+
+    def _volume_of(self, buf: int) -> tuple[float, np.ndarray]:
+        ls = self.ls
+        saved = ls.cur
+        ls.cur = buf
+        try:
+            ls.fractions(self._plan(), self.cutoff[1])
+            self._frac_valid = buf
+            t0 = _time.perf_counter()
+            r = ls.results()
+            self.timers["volume_sync"] += _time.perf_counter() - t0
+            return ls.volume_from_results(r), r
+        finally:
+            ls.cur = saved
+
+    def _stats(self) -> tuple[float, dict]:
+        t0 = _time.perf_counter()
+        r = self.ls.results()
+        self.timers["stats_sync"] += _time.perf_counter() - t0
+        info = {}
+        # statistics over |phi| < h, or over every evaluated node when that set is
+        # empty (svierosion/evolve.py:131, :147)
+        near = r[_abi.R_NNEAR] > 0
+        if self.law.requires_svi:
+            if r[_abi.R_NALL] > 0 or near:
+                info["qmin"] = float(r[_abi.R_QMIN if near else _abi.R_QMIN_ALL])
+                info["qmax"] = float(r[_abi.R_QMAX if near else _abi.R_QMAX_ALL])
+                info["dvmax"] = float(r[_abi.R_DVMAX if near else _abi.R_DVMAX_ALL])
+                info["capped"] = int(r[_abi.R_CAPPED if near else _abi.R_CAPPED_ALL])
+        else:
+            info["qmin"] = float(r[_abi.R_QMIN if near else _abi.R_QMIN_ALL])
+            info["qmax"] = float(r[_abi.R_QMAX if near else _abi.R_QMAX_ALL])
+            info["dmax"] = float(r[_abi.R_DMAX if near else _abi.R_DMAX_ALL])
+        if near:
+            info["vmean"] = float(r[_abi.R_VSUM] / r[_abi.R_NNEAR])
+        elif r[_abi.R_NALL] > 0:
+            info["vmean"] = float(r[_abi.R_VSUM_ALL] / r[_abi.R_NALL])
+        return float(r[_abi.R_VMAX]), info
+
+    def stable_dt(self, vmax: float, info: dict) -> float:
+        """min(cfl h/max V, feedback_cfl (4r/3)/max dV/dN, parabolic_cfl h^2/max D) (svierosion/evolve.py:162-170)."""
+        h = self.ls.h
+        dt = self.cfl * h / vmax if vmax > 0 else math.inf
+        if self.law.requires_svi and info.get("dvmax", 0) > 0:
+            dt = min(dt, self.feedback_cfl * (4.0 * self.law.radius / 3.0) / info["dvmax"])
+        if self.law.requires_curvature and info.get("dmax", 0) > 0:
+            dt = min(dt, self.parabolic_cfl * h * h / info["dmax"])
+        return dt
+
+    def svi_field(self, nodes: torch.Tensor | None = None) -> torch.Tensor:
+        """Dense N_r of the current shape (NaN outside the evaluated tiles when ``nodes`` is given)."""
+        return self.conv.exterior_fraction(self.ls.dense_fractions(), nodes)
+
+    # ---------------------------------------------------------------- run
+    def run(self, target_fraction: float | None = 0.9, max_steps: int = 100000, max_time: float | None = None,
+            tol: float = 1e-6, callback=None) -> RunResult:
+        """Evolve until the volume equals ``target_fraction`` of the initial volume (or
+        ``max_time`` / ``max_steps``); re-entrant with decreasing targets."""
+        ls = self.ls
+        V0 = self.initial_volume
+        target = None if target_fraction is None else target_fraction * V0
+        vol, res = self._volume_of(ls.cur)
+        reason = "max_steps"
+        n_start = self.steps
+        while self.steps - n_start < max_steps:
+            if target is not None and vol <= target * (1 + tol):
+                reason = "target_volume"
+                break
+            if max_time is not None and self.time >= max_time * (1 - 1e-12):
+                reason = "max_time"
+                break
+            if not (res[_abi.R_HAS_NEG] > 0 and res[_abi.R_HAS_POS] > 0):
+                reason = "vanished"
+                break
+            t_wall = _time.time()
+            a = ls.cur
+            b, c = (a + 1) % 3, (a + 2) % 3
+            self._prepare(a, True)
+            vmax, info = self._stats()
+            dt = self.stable_dt(vmax, info)
+            if max_time is not None:
+                dt = min(dt, max_time - self.time)
+            if not math.isfinite(dt) or dt <= 0:
+                reason = "stalled"
+                break
+            self._stage(0, a, a, b, dt)
+            self._rk3_tail(a, b, c, dt)
+            vnew, res = self._volume_of(b)
+            if target is not None and vnew < target:
+                lo_t, lo_v, hi_t, hi_v = 0.0, vol, dt, vnew
+                for _ in range(8):
+                    if hi_v >= lo_v:
+                        break
+                    dts = lo_t + (target - lo_v) * (hi_t - lo_t) / (hi_v - lo_v)
+                    self._stage(3, a, a, b, dts)
+                    self._rk3_tail(a, b, c, dts)
+                    v, res = self._volume_of(b)
+                    if abs(v - target) <= tol * V0:
+                        break
+                    if v > target:
+                        lo_t, lo_v = dts, v
+                    else:
+                        hi_t, hi_v = dts, v
+                dt, vnew = dts, v
+            ls.cur = b
+            self.time += dt
+            self.steps += 1
+            self._displacement += dt * vmax
+            did_reinit = False
+            reached = target is not None and vnew <= target * (1 + tol)
+            if self._displacement >= self.reinit_distance * ls.h and not reached:
+                ls.reinitialize(self.reinit_iterations)
+                self._alloc_band_arrays()
+                self._frac_valid = None
+                self._displacement = 0.0
+                did_reinit = True
+                vnew, res = self._volume_of(ls.cur)
+            rec = StepRecord(self.steps, self.time, dt, vnew, vnew / V0, vmax, info.get("vmean", vmax),
+                             info.get("qmin", float("nan")), info.get("qmax", float("nan")),
+                             info.get("capped", 0), did_reinit, _time.time() - t_wall)
+            self.history.append(rec)
+            if callback is not None:
+                callback(self, rec)
+            vol = vnew
+        return RunResult(reason, self.steps, self.time, vol / V0, self.history)

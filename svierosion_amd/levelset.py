@@ -1,0 +1,210 @@
+"""Narrow-band level set on the device (API of svierosion/levelset.py).
+
+Device layout (per job, in HBM):
+
+* three dense float32 copies of phi (``bufs``); one holds the current field,
+  the other two are the TVD-RK3 stage outputs.  Outside the band every copy
+  holds the same clamped value +-width*h, so any copy can serve as a stencil
+  input; ``rebuild_band`` re-synchronises them.
+* ``occupancy`` -- the dense occupancy f (1 inside, 0 outside, the tangent-plane
+  cell fraction at band nodes), the input of the N_r summation.  Only band
+  entries change between rebuilds.
+* the band -- a sorted int32 list of flat indices (capacity = grid size).
+* a workspace with the reduction partials and the result slots.
+
+All numerics run in libsvimx; this class sequences the calls.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _abi
+from .grid import Grid
+
+MARGIN = 4  # cells between any band node and the grid boundary
+
+
+def _idx32(idx: torch.Tensor, device) -> torch.Tensor:
+    return idx.to(device=device, dtype=torch.int32).contiguous()
+
+
+class LevelSet:
+    """A level set function with an explicitly stored narrow band."""
+
+    def __init__(self, phi: torch.Tensor, grid: Grid, width: float = 6.0):
+        if tuple(phi.shape) != tuple(grid.shape):
+            raise ValueError("phi shape does not match grid")
+        self.device = _abi.require_device(phi.device if phi.is_cuda else "cuda")
+        self.grid = grid
+        self.h = float(grid.spacing)
+        self.width = float(width)
+        self.dims = tuple(int(s) for s in grid.shape)
+        self.strides = tuple(int(s) for s in grid.strides)
+        n = grid.size
+        if n >= 2 ** 31:
+            raise ValueError("grid has more than 2^31 nodes")
+        first = phi.to(self.device, torch.float32).contiguous().reshape(-1).clone()
+        self.bufs = [first, torch.empty_like(first), torch.empty_like(first)]
+        self.cur = 0
+        self.occupancy = torch.zeros(n, dtype=torch.float32, device=self.device)
+        self.ws = _abi.workspace(self.dims, self.device)
+        self._band = torch.empty(n, dtype=torch.int32, device=self.device)
+        self.nb = 0
+        self.inside_outside = 0
+        self._reinit_bufs = None
+        self.rebuild_band()
+
+    # ------------------------------------------------------------ state
+    @property
+    def flat(self) -> torch.Tensor:
+        return self.bufs[self.cur]
+
+    @property
+    def phi(self) -> torch.Tensor:
+        return self.flat.view(self.dims)
+
+    @property
+    def band(self) -> torch.Tensor:
+        return self._band[: self.nb]
+
+    def results(self) -> np.ndarray:
+        """Host copy of the workspace result slots (synchronises the stream)."""
+        return _abi.results_view(self.ws).cpu().numpy()
+
+    # ------------------------------------------------------------ band
+    def rebuild_band(self) -> None:
+        """Band = nodes with |phi| < width*h; clamp all other values (svierosion/levelset.py:70-86)."""
+        others = [b for i, b in enumerate(self.bufs) if i != self.cur]
+        info = (C.c_int64 * 9)()
+        rc = _abi.lib.svimx_band_rebuild(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(others[0]),
+                                         _abi.ptr(others[1]), _abi.ptr(self.occupancy), *self.dims, self.h,
+                                         self.width, MARGIN, _abi.ptr(self._band), self._band.numel(),
+                                         _abi.ptr(self.ws), self.ws.numel(), info)
+        self.nb = int(info[0])
+        self.inside_outside = int(info[1])
+        _abi.check(rc, "band rebuild")
+
+    def fractions(self, plan=None, cut_hi: float = 5.5) -> None:
+        """Cell fractions of the current band into ``occupancy`` (+ volume and interface results)."""
+        _abi.check(_abi.lib.svimx_fractions(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(self._band), self.nb,
+                                            *self.dims, self.h, _abi.ptr(self.occupancy), plan, float(cut_hi),
+                                            _abi.ptr(self.ws), self.ws.numel()), "fractions")
+
+    def volume_from_results(self, res) -> float:
+        return (self.inside_outside + float(res[_abi.R_VOLUME_SUM])) * self.h ** 3
+
+    def interface_nodes(self, idx: torch.Tensor | None = None) -> torch.Tensor:
+        """Band nodes having a 6-neighbour of the opposite (or zero) sign."""
+        idx = self.band if idx is None else _idx32(idx, self.device)
+        out = torch.empty(idx.numel(), dtype=torch.uint8, device=self.device)
+        _abi.check(_abi.lib.svimx_op_interface_mask(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx), idx.numel(),
+                                                    *self.dims, _abi.ptr(out)))
+        return idx[out.bool()]
+
+    def has_interface(self) -> bool:
+        self.fractions()
+        r = self.results()
+        return bool(r[_abi.R_HAS_NEG] > 0) and bool(r[_abi.R_HAS_POS] > 0)
+
+    # ---------------------------------------------------- derivatives
+    def central_gradient(self, idx: torch.Tensor) -> torch.Tensor:
+        f = self.flat
+        idx = idx.to(self.device, torch.int64)
+        return torch.stack([0.5 * (f[idx + s] - f[idx - s]) for s in self.strides], 1)
+
+    def godunov_norm_inward(self, idx: torch.Tensor) -> torch.Tensor:
+        idx = _idx32(idx, self.device)
+        out = torch.empty(idx.numel(), dtype=torch.float32, device=self.device)
+        _abi.check(_abi.lib.svimx_op_godunov(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx), idx.numel(),
+                                             *self.dims, 1.0 / self.h, _abi.ptr(out)))
+        return out
+
+    def principal_curvatures(self, idx: torch.Tensor):
+        idx = _idx32(idx, self.device)
+        k1 = torch.empty(idx.numel(), dtype=torch.float32, device=self.device)
+        k2 = torch.empty_like(k1)
+        _abi.check(_abi.lib.svimx_op_principal_curvatures(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx),
+                                                          idx.numel(), *self.dims, self.h, _abi.ptr(k1),
+                                                          _abi.ptr(k2)))
+        return k1, k2
+
+    # ------------------------------------------------- volume fractions
+    def cell_fractions(self, idx: torch.Tensor | None = None) -> torch.Tensor:
+        idx = self.band if idx is None else _idx32(idx, self.device)
+        out = torch.empty(idx.numel(), dtype=torch.float32, device=self.device)
+        _abi.check(_abi.lib.svimx_op_cell_fractions(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx),
+                                                    idx.numel(), *self.dims, _abi.ptr(out)))
+        return out
+
+    def dense_fractions(self) -> torch.Tensor:
+        """Dense occupancy (1 inside, 0 outside, cell fractions in the band)."""
+        self.fractions()
+        return self.occupancy.view(self.dims).clone()
+
+    def volume(self) -> float:
+        self.fractions()
+        return self.volume_from_results(self.results())
+
+    # ----------------------------------------------------- projection
+    def closest_points(self, idx: torch.Tensor, newton_steps: int = 1) -> torch.Tensor:
+        """First capped Newton projection onto the zero set, index coordinates (m, 3)."""
+        if newton_steps != 1:
+            raise NotImplementedError("only the single projection step of the erosion path is implemented")
+        idx = _idx32(idx, self.device)
+        out = torch.empty((idx.numel(), 3), dtype=torch.float32, device=self.device)
+        _abi.check(_abi.lib.svimx_op_closest_points(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx), idx.numel(),
+                                                    *self.dims, self.h, _abi.ptr(out)))
+        return out
+
+    def clamp_to_interior(self, p: torch.Tensor) -> torch.Tensor:
+        hi = torch.tensor([n - 3.001 for n in self.dims], device=p.device, dtype=p.dtype)
+        return torch.minimum(p.clamp_min(1.0), hi)
+
+    # --------------------------------------------------- reinitialisation
+    def reinitialize(self, iterations: int = 12, cfl: float = 0.45, dilation: int = 3) -> None:
+        """Min (2010) PDE reinitialisation with subcell fix, then band rebuild
+        (svierosion/levelset.py:154-183)."""
+        n = self.grid.size
+        if self._reinit_bufs is None:
+            self._reinit_bufs = (torch.empty(n, dtype=torch.uint8, device=self.device),
+                                 torch.empty(n, dtype=torch.uint8, device=self.device),
+                                 torch.empty(n, dtype=torch.int32, device=self.device))
+        ma, mb, widx = self._reinit_bufs
+        geo = torch.empty(8 * max(self.nb * 4, 1024), dtype=torch.float32, device=self.device)
+        scratch = self.bufs[(self.cur + 1) % 3]
+        nwork = C.c_int64()
+        rc = _abi.lib.svimx_reinit(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(scratch), _abi.ptr(self._band),
+                                   self.nb, *self.dims, self.h, int(iterations), float(cfl), int(dilation), MARGIN,
+                                   _abi.ptr(ma), _abi.ptr(mb), _abi.ptr(widx), _abi.ptr(geo), geo.numel() // 8,
+                                   _abi.ptr(self.ws), self.ws.numel(), C.byref(nwork))
+        if rc == _abi.ECAPACITY:
+            geo = torch.empty(8 * int(nwork.value), dtype=torch.float32, device=self.device)
+            rc = _abi.lib.svimx_reinit(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(scratch), _abi.ptr(self._band),
+                                       self.nb, *self.dims, self.h, int(iterations), float(cfl), int(dilation),
+                                       MARGIN, _abi.ptr(ma), _abi.ptr(mb), _abi.ptr(widx), _abi.ptr(geo),
+                                       geo.numel() // 8, _abi.ptr(self.ws), self.ws.numel(), C.byref(nwork))
+        _abi.check(rc, "reinitialisation")
+        self.rebuild_band()
+
+
+# ------------------------------------------------------------- interpolation
+def _interp(fn, flat: torch.Tensor, shape, p: torch.Tensor) -> torch.Tensor:
+    f = flat.contiguous().reshape(-1)
+    p = p.to(f.device, torch.float32).contiguous()
+    out = torch.empty(p.shape[0], dtype=torch.float32, device=f.device)
+    _abi.check(fn(_abi.stream(), _abi.ptr(f), *map(int, shape), _abi.ptr(p), p.shape[0], _abi.ptr(out)))
+    return out
+
+
+def tricubic(flat: torch.Tensor, shape, p: torch.Tensor) -> torch.Tensor:
+    """Tensor-product cubic Lagrange interpolation at index coordinates p (m, 3)."""
+    return _interp(_abi.lib.svimx_op_tricubic, flat, shape, p)
+
+
+def trilinear(flat: torch.Tensor, shape, p: torch.Tensor) -> torch.Tensor:
+    """Trilinear interpolation at index coordinates p (m, 3)."""
+    return _interp(_abi.lib.svimx_op_trilinear, flat, shape, p)

@@ -1,0 +1,82 @@
+"""Full evolution on the GPU: exact sphere solutions and per-step parity with the oracle."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+from scipy.integrate import solve_ivp
+
+from oracle import flow as F
+from tests.conftest import CTR, sphere_phi
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_sphere(R, h, pad):
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.levelset import LevelSet
+    phi, origin, h, shape = sphere_phi(R, h, pad)
+    g = Grid(origin, h, shape)
+    return LevelSet(torch.from_numpy(phi.astype(np.float32)).cuda(), g, width=6), g, phi
+
+
+def _interface_radius(ls, g):
+    idx = ls.band
+    x = g.node_coords(idx).cpu().numpy()
+    ph = ls.flat[idx.long()].cpu().numpy()
+    near = np.abs(ph) < g.spacing
+    return np.linalg.norm(x[near] - CTR, axis=1) - ph[near]
+
+
+@pytest.mark.parametrize("k,s,t,tol", [(1, 0, None, 3e-5), (2, 1, 0.8, 3e-4), (0, 0, None, 3e-5)])
+def test_svi_sphere_radius_ode(device, k, s, t, tol):
+    """Reference tests/test_flows.py:99-107: the sphere radius follows dR/dt = -F(N_r(R))."""
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.flows import SVIPowerLaw
+    r, T = 0.4, 0.15
+    law = SVIPowerLaw(r, k, s, t)
+    Fn = lambda R: float(law.speed(torch.tensor(0.5 + 3.0 * r / (16.0 * R), dtype=torch.float64)))
+    Rex = solve_ivp(lambda _t, y: [-Fn(y[0])], (0, T), [1.0], rtol=1e-12, atol=1e-14).y[0, -1]
+    ls, g, _ = _gpu_sphere(1.0, r / 8, r + 10 * r / 8)
+    ErosionSimulation(ls, law).run(target_fraction=None, max_time=T)
+    Rn = _interface_radius(ls, g)
+    assert abs(Rn.mean() - Rex) < tol
+    assert Rn.max() - Rn.min() < 10 * tol + 1e-4
+
+
+def test_curvature_sphere_radius(device):
+    """Reference tests/test_flows.py:109-116."""
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.flows import GaussCurvatureLaw, MeanCurvatureLaw
+    T, h = 0.1, 0.05
+    cases = ((MeanCurvatureLaw(1.0), math.sqrt(1 - 2 * T)),
+             (GaussCurvatureLaw(1.0), (1 - 3 * T) ** (1 / 3)),
+             (GaussCurvatureLaw(0.25, delta=1e-4), (1 - 1.5 * T) ** (2 / 3)))
+    for law, Rex in cases:
+        ls, g, _ = _gpu_sphere(1.0, h, 12 * h)
+        ErosionSimulation(ls, law).run(target_fraction=None, max_time=T)
+        Rn = _interface_radius(ls, g)
+        assert abs(Rn.mean() - Rex) < 5e-4, law.describe()
+
+
+def test_history_matches_oracle(device):
+    """Same seeded sphere, SVI k=1 s=1 t=0.93 run to a target fraction on GPU and oracle:
+    per-step volume fraction within 1e-4, step count and stop reason equal."""
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.flows import SVIPowerLaw
+    r, h = 0.3, 0.05
+    ls, g, phi = _gpu_sphere(1.0, h, r + 10 * h)
+    sim = ErosionSimulation(ls, SVIPowerLaw(r, 1, 1, 0.93))
+    res = sim.run(target_fraction=0.8)
+    ref_ls = F.LevelSet(phi.astype(np.float32), g.shape, h)
+    ref = F.Simulation(ref_ls, F.SVILaw(r, 1, 1, 0.93))
+    reason, steps, _, frac, hist = ref.run(target_fraction=0.8)
+    assert res.stop_reason == reason == "target_volume"
+    assert abs(sim.initial_volume - ref.initial_volume) < 1e-6 * ref.initial_volume
+    assert len(res.history) == len(hist)
+    for a, b in zip(res.history, hist):
+        assert abs(a.fraction - b.fraction) < 1e-4
+        assert abs(a.dt - b.dt) < 1e-3 * b.dt
+        assert a.reinit == b.reinit
+    assert abs(res.fraction - 0.8) < 1e-5

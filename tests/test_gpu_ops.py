@@ -1,0 +1,198 @@
+"""HIP operators and level-set passes against the CPU oracle on identical float32 inputs.
+
+Tolerances are stated per check; the oracle runs the reference's float32
+formulas, so differences are rounding (fused multiply-adds, division order).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import flow as F
+from oracle import stencils as S
+from tests.conftest import sphere_phi
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(distort=True, R=1.0, h=0.05, pad=0.6):
+    phi, origin, h, shape = sphere_phi(R, h, pad, distort=distort)
+    phi32 = phi.astype(np.float32)
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.levelset import LevelSet
+    g = Grid(origin, h, shape)
+    ls = LevelSet(torch.from_numpy(phi32).cuda(), g, width=6.0)
+    ref = F.LevelSet(phi32, shape, h, width=6.0)
+    return ls, ref, g
+
+
+def test_band_rebuild_matches(device):
+    ls, ref, g = _pair()
+    assert ls.nb == ref.band.size
+    assert np.array_equal(ls.band.cpu().numpy(), ref.band)
+    assert np.array_equal(ls.flat.cpu().numpy(), ref.flat)
+    n_in = int((ref.flat < 0).sum()) - int((ref.flat[ref.band] < 0).sum())
+    assert ls.inside_outside == n_in
+
+
+def test_clamped_band_edge_case(device):
+    """Reference tests/test_geometry.py:99-107: float32(6h) may round below the FP64 cap."""
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.levelset import LevelSet
+    h = 0.0075
+    g = Grid((0.0, 0.0, 0.0), h, (24, 24, 24))
+    phi = torch.full(g.shape, 6.0, dtype=torch.float32) * h
+    phi[12, 12, 12] = -h
+    ls = LevelSet(phi.cuda(), g, width=6.0)
+    assert ls.nb < 30
+
+
+def test_operators_match_oracle(device):
+    ls, ref, g = _pair()
+    idx = ref.band
+    flat = ref.flat
+    st = ref.strides
+    got = ls.godunov_norm_inward(torch.from_numpy(idx)).cpu().numpy()
+    want = S.godunov_inward(flat, idx, st, 1.0 / g.spacing)
+    assert np.abs(got - want).max() < 2e-5 * max(1.0, np.abs(want).max())
+    got = ls.cell_fractions(torch.from_numpy(idx)).cpu().numpy()
+    assert np.abs(got - S.cell_fractions(flat, idx, st)).max() < 2e-6
+    got = ls.closest_points(torch.from_numpy(idx)).cpu().numpy()
+    assert np.abs(got - S.closest_points(flat, idx, g.shape, g.spacing)).max() < 2e-5
+    k1, k2 = ls.principal_curvatures(torch.from_numpy(idx))
+    w1, w2 = S.principal_curvatures(flat, idx, st, g.spacing)
+    scale = np.abs(w1).max()
+    assert np.abs(k1.cpu().numpy() - w1).max() < 1e-3 * scale
+    assert np.abs(k2.cpu().numpy() - w2).max() < 1e-3 * scale
+    got = ls.interface_nodes().cpu().numpy()
+    assert np.array_equal(got, ref.interface_nodes())
+
+
+def test_interpolation_and_distance_ops(device):
+    from svierosion_amd import _abi
+    from svierosion_amd.levelset import tricubic, trilinear
+    rng = np.random.default_rng(4)
+    shape = (20, 22, 24)
+    f = rng.normal(size=shape).astype(np.float32)
+    p = np.stack([rng.uniform(1, n - 3.001, 4000) for n in shape], 1).astype(np.float32)
+    got = tricubic(torch.from_numpy(f).cuda(), shape, torch.from_numpy(p).cuda()).cpu().numpy()
+    assert np.abs(got - S.tricubic(f.reshape(-1), shape, p)).max() < 1e-5
+    got = trilinear(torch.from_numpy(f).cuda(), shape, torch.from_numpy(p).cuda()).cpu().numpy()
+    assert np.abs(got - S.trilinear(f.reshape(-1), shape, p)).max() < 1e-5
+    # point-triangle distance in local coordinates
+    pts = rng.normal(size=(5000, 3)).astype(np.float32) * 2
+    tri = rng.normal(size=(5000, 3, 3)).astype(np.float32)
+    out = torch.empty(5000, dtype=torch.float32, device="cuda")
+    pt, tt = torch.from_numpy(pts).cuda(), torch.from_numpy(tri.reshape(5000, 9)).cuda()
+    _abi.check(_abi.lib.svimx_op_point_triangle_distance(_abi.stream(), _abi.ptr(pt), _abi.ptr(tt), 5000,
+                                                         _abi.ptr(out)))
+    want = S.point_triangle_distance2(pts.astype(np.float64), tri[:, 0].astype(np.float64),
+                                      tri[:, 1].astype(np.float64), tri[:, 2].astype(np.float64))
+    assert np.abs(out.cpu().numpy() - want).max() < 2e-5 * max(1.0, want.max())
+
+
+def test_reinit_geometry_matches(device):
+    from svierosion_amd import _abi
+    ls, ref, g = _pair()
+    idx = ref.band
+    geo = torch.empty(8 * idx.size, dtype=torch.float32, device="cuda")
+    it = torch.from_numpy(idx.astype(np.int32)).cuda()
+    _abi.check(_abi.lib.svimx_op_reinit_geometry(_abi.stream(), _abi.ptr(ls.flat), _abi.ptr(it), idx.size,
+                                                 *g.shape, g.spacing, 0.45, _abi.ptr(geo)))
+    geo = geo.cpu().numpy().reshape(-1, 8)
+    want = S.reinit_geometry(ref.flat, idx, ref.strides, g.spacing, 0.45)
+    assert np.abs(geo[:, 0] - want[1]).max() < 1e-5 * g.spacing
+    flags = geo[:, 7].astype(np.int64)
+    for a in range(3):
+        cp, cm, xp, xm = want[2 + 4 * a: 6 + 4 * a]
+        assert np.array_equal(((flags >> (2 * a)) & 1).astype(bool), cp)
+        assert np.array_equal(((flags >> (2 * a + 1)) & 1).astype(bool), cm)
+        sel = cp
+        assert np.abs(geo[sel, 1 + 2 * a] - xp[sel]).max(initial=0) < 2e-5 * g.spacing
+        sel = cm
+        assert np.abs(geo[sel, 2 + 2 * a] - xm[sel]).max(initial=0) < 2e-5 * g.spacing
+
+
+def test_fractions_volume_and_interface(device):
+    ls, ref, g = _pair()
+    vol = ls.volume()
+    want = ref.volume()
+    assert abs(vol - want) < 1e-6 * abs(want)
+    assert ls.has_interface() and ref.has_interface()
+    occ = ls.dense_fractions().cpu().numpy()
+    assert np.abs(occ - ref.dense_fractions()).max() < 2e-6
+
+
+def test_nr_direct_matches_fft_and_brute_force(device):
+    """N_r at every node of a small grid: HIP direct sum vs the oracle's FFT (2e-5 abs) and
+    vs an FP64 brute-force sum at sampled nodes (2e-6 abs)."""
+    from svierosion_amd.svi import BallConvolution
+    ls, ref, g = _pair(distort=True, R=1.0, h=0.05, pad=0.6)
+    r = 0.3
+    conv = BallConvolution(g, r, device="cuda")
+    occ = ls.dense_fractions()
+    N = conv.exterior_fraction(occ).cpu().numpy()
+    f = ref.dense_fractions()
+    w = F.ball_cell_weights(r, g.spacing)
+    Nfft = F.exterior_fraction_fft(f, F.ball_kernel_hat(g.shape, w), g.spacing, r)
+    m = conv.m
+    inner = tuple(slice(m + 1, n - m - 1) for n in g.shape)
+    assert np.abs(N[inner] - Nfft[inner]).max() < 2e-5
+    rng = np.random.default_rng(0)
+    pts = np.stack([rng.integers(m + 1, n - m - 1, 200) for n in g.shape], 1)
+    direct = F.exterior_fraction_direct(f, w, g.spacing, r, pts)
+    assert np.abs(N[pts[:, 0], pts[:, 1], pts[:, 2]] - direct).max() < 2e-6
+
+
+@pytest.mark.parametrize("ratio,tol", [(8, 3e-4), (16, 6e-5)])
+def test_nr_sphere_closed_form(device, ratio, tol):
+    """Reference tests/test_flows.py:46-56: N_r at interface closest points vs 1/2 + 3r/(16R)."""
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.levelset import LevelSet, tricubic
+    from svierosion_amd.svi import BallConvolution
+    R, r = 1.0, 0.4
+    h = r / ratio
+    phi, origin, h, shape = sphere_phi(R, h, r + 10 * h)
+    g = Grid(origin, h, shape)
+    ls = LevelSet(torch.from_numpy(phi.astype(np.float32)).cuda(), g)
+    conv = BallConvolution(g, r, device="cuda")
+    iface = ls.interface_nodes()
+    pts = ls.closest_points(iface)
+    base = (torch.floor(pts).long() - 1)
+    stencil = []
+    for a in range(4):
+        for b in range(4):
+            for c in range(4):
+                q = base + torch.tensor([a, b, c], device=base.device)
+                stencil.append(q[:, 0] * shape[1] * shape[2] + q[:, 1] * shape[2] + q[:, 2])
+    nodes = torch.unique(torch.cat(stencil))
+    N = conv.exterior_fraction(ls.dense_fractions(), nodes)
+    vals = tricubic(N.view(-1), shape, pts)
+    err = (vals.double() - F.sphere_exterior_fraction(r, R)).abs().max().item()
+    assert err < tol
+
+
+def test_reinitialize_matches_oracle(device):
+    ls, ref, g = _pair(distort=True, R=1.0, h=0.08, pad=12 * 0.08)
+    ls.reinitialize(40)
+    ref.reinitialize(40)
+    assert ls.nb == ref.band.size or abs(ls.nb - ref.band.size) <= 2
+    got = ls.flat.cpu().numpy()
+    near = np.abs(ref.flat) < 3 * g.spacing
+    assert np.abs(got - ref.flat)[near].max() < 1e-4 * g.spacing
+
+
+def test_reinitialize_preserves_interface(device):
+    """Reference tests/test_flows.py:79-88."""
+    from tests.conftest import CTR
+    for h, tol in ((0.08, 0.01), (0.04, 0.005)):
+        ls, ref, g = _pair(distort=True, R=1.0, h=h, pad=12 * h)
+        ls.reinitialize(40)
+        idx = ls.band
+        x = g.node_coords(idx).cpu().numpy()
+        ph = ls.flat[idx.long()].cpu().numpy()
+        ex = np.linalg.norm(x - CTR, axis=1) - 1.0
+        near = np.abs(ex) < 1.5 * h
+        assert np.abs(ph - ex)[near].max() / h < tol

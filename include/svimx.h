@@ -154,6 +154,30 @@ int svimx_reinit(void* stream, float* phi, float* scratch, const int32_t* band, 
                  uint8_t* mask_a, uint8_t* mask_b, int32_t* work_idx, float* geo, int64_t work_cap,
                  void* ws, int64_t ws_bytes, int64_t* n_work);
 
+/* ---------------------------------------------------------------- mesh -> signed distance
+ * Dense phi = +-min(d, band)*h, negative inside, for a triangle mesh already in
+ * index coordinates.  Inputs (device):
+ *   tri_local  n_split x 9 float32: split triangles (edges <= 8 cells), vertices
+ *              relative to tri_org (an integer node per triangle)
+ *   tri_box    n_split x 6 int32: clipped node box (lo xyz, hi xyz) of the
+ *              triangle dilated by exact_band cells
+ *   tri_vid    n_split x 3 int32 vertex ids; star_first/star_deg (n_vertices)
+ *              and star_tris (3 n_split) the vertex -> triangle CSR
+ *   faces_idx  n_faces x 9 float64: original faces for the sign
+ * sign_axes_mask: bit a casts lines along axis a (majority vote over the set);
+ * signed_rule: winding number (1) or parity (0).  dist_out / tid_out (may be
+ * NULL) receive the unclamped distance (cells) and closest split triangle.
+ * Allocates its temporaries and synchronises.
+ * Replaces mesh_to_sdf (svierosion/sdf.py:365-415) and its stages
+ * _exact_near_field :100-145, _propagate :163-202, _refine_with_stars :211-265,
+ * inside_mask :343-361. */
+int svimx_sdf_build(void* stream, const float* tri_local, const int32_t* tri_org, const int32_t* tri_box,
+                    int64_t n_split, const int32_t* tri_vid, const int32_t* star_first, const int32_t* star_deg,
+                    const int32_t* star_tris, int64_t n_vertices, const double* faces_idx, int64_t n_faces,
+                    int sign_axes_mask, int signed_rule, int nx, int ny, int nz, double h, double band,
+                    double exact_band, float* phi_out, float* dist_out, int32_t* tid_out, void* ws,
+                    int64_t ws_bytes);
+
 /* ---------------------------------------------------------------- operators
  * Node-list forms of the reference's @kernel functions (svierosion/kernels.py),
  * sharing the device code of the fused stage kernels; used by LevelSet methods
@@ -166,6 +190,8 @@ int svimx_op_closest_points(void* stream, const float* phi, const int32_t* idx, 
                             int ny, int nz, double h, float* out_p);          /* :185-208 */
 int svimx_op_tricubic(void* stream, const float* field, int nx, int ny, int nz, const float* p,
                       int64_t n, float* out);                                 /* :212-228 */
+int svimx_op_tricubic_gradient(void* stream, const float* field, int nx, int ny, int nz, const float* p,
+                               int64_t n, float* out, float* grad);                   /* :232-253 */
 int svimx_op_trilinear(void* stream, const float* field, int nx, int ny, int nz, const float* p,
                        int64_t n, float* out);                                /* :257-270 */
 int svimx_op_principal_curvatures(void* stream, const float* phi, const int32_t* idx, int64_t n,

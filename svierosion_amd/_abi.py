@@ -52,10 +52,13 @@ _SIGNATURES = {
     "svimx_nr_exec": (_I, [_P, _P, _P, _P, _P, _I64]),
     "svimx_reinit": (_I, [_P, _P, _P, _P, _I64, _I, _I, _I, _D, _I, _D, _I, _I, _P, _P, _P, _P, _I64, _P, _I64,
                           _P]),
-    "svimx_op_godunov": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
+    "svimx_sdf_build": (_I, [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _P, _I64, _I, _I, _I, _I, _I, _D, _D, _D,
+                             _P, _P, _P, _P, _I64]),
+    "svimx_op_godunov":(_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
     "svimx_op_cell_fractions": (_I, [_P, _P, _P, _I64, _I, _I, _I, _P]),
     "svimx_op_closest_points": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
     "svimx_op_tricubic": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P]),
+    "svimx_op_tricubic_gradient": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P, _P]),
     "svimx_op_trilinear": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P]),
     "svimx_op_principal_curvatures": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _P]),
     "svimx_op_interface_mask": (_I, [_P, _P, _P, _I64, _I, _I, _I, _P]),

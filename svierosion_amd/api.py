@@ -1,0 +1,291 @@
+"""High-level interface: erode a triangle mesh (API of svierosion/api.py).
+
+``erode(mesh, ErosionParams)`` keeps the reference's signature, parameters,
+result fields, stop reasons and error types.  The pipeline is the same
+(svierosion/api.py:197-253): grid sizing, mesh SDF, level set, simulation,
+t > max N_r check, evolution to each snapshot target, surface extraction and
+per-vertex annotation -- with every numerical stage on the GPU through
+libsvimx.
+"""
+
+from __future__ import annotations
+
+import time
+import warnings
+from dataclasses import asdict, dataclass, field
+
+import numpy as np
+import torch
+
+from . import _abi
+from .evolve import ErosionSimulation, StepRecord
+from .flows import GaussCurvatureLaw, MeanCurvatureLaw, SpeedLaw, SVIPowerLaw
+from .grid import Grid, get_device
+from .levelset import MARGIN, LevelSet, tricubic
+from .meshio import TriMesh
+from .sdf import mesh_to_sdf, point_triangle_distance2
+from .surface import extract_surface, sample_field
+
+
+@dataclass
+class ErosionParams:
+    """All parameters of an erosion run (lengths in mesh units); ``spacing`` beats ``resolution``."""
+
+    law: str = "svi"
+    radius: float | None = None
+    k: int = 1
+    s: int = 0
+    t: float | None = None
+    eps: float = 0.0
+    max_speed_ratio: float = 100.0
+    alpha: float = 1.0
+    delta: float | None = None
+    target_fraction: float = 0.9
+    snapshots: tuple = ()
+    spacing: float | None = None
+    resolution: int = 200
+    band: float = 6.0
+    cfl: float = 0.5
+    feedback_cfl: float = 1.0
+    parabolic_cfl: float = 0.25
+    reinit_distance: float = 1.5
+    reinit_iterations: int = 20
+    project_vertices: bool = True
+    device: str | None = None
+    max_steps: int = 1_000_000
+
+    def make_law(self, length_scale: float = 1.0) -> SpeedLaw:
+        """Speed law; ``length_scale`` sets the automatic delta (svierosion/api.py:54-66)."""
+        if self.law == "svi":
+            if self.radius is None:
+                raise ValueError("the SVI law needs a radius")
+            return SVIPowerLaw(self.radius, self.k, self.s, self.t, self.max_speed_ratio, self.eps)
+        if self.law in ("mean", "mean_curvature"):
+            d = self.delta if self.delta is not None else (1.0 / length_scale if self.alpha < 1 else 0.0)
+            return MeanCurvatureLaw(self.alpha, self.eps, d)
+        if self.law in ("gauss", "gauss_curvature"):
+            d = self.delta if self.delta is not None else (length_scale ** -2 if self.alpha < 1 else 0.0)
+            return GaussCurvatureLaw(self.alpha, self.eps, d)
+        raise ValueError(f"unknown law {self.law!r}")
+
+
+@dataclass
+class ErosionResult:
+    mesh: TriMesh
+    snapshots: list = field(default_factory=list)
+    history: list = field(default_factory=list)
+    grid: Grid | None = None
+    stop_reason: str = ""
+    time: float = 0.0
+    fraction: float = 1.0
+    initial_volume_grid: float = 0.0
+    initial_volume_mesh: float = 0.0
+    timings: dict = field(default_factory=dict)
+    params: dict = field(default_factory=dict)
+    initial_mesh: TriMesh | None = None
+    level_set: LevelSet | None = None
+    simulation: ErosionSimulation | None = None
+
+    def summary(self) -> str:
+        t = self.timings
+        return (f"stop: {self.stop_reason}, steps {len(self.history)}, time {self.time:.6g}, "
+                f"volume fraction {self.fraction:.5f} | {self.grid.describe()} | "
+                f"sdf {t.get('sdf', 0):.1f}s, evolution {t.get('evolve', 0):.1f}s, "
+                f"meshing {t.get('mesh', 0):.1f}s")
+
+
+def make_grid(mesh: TriMesh, params: ErosionParams) -> Grid:
+    """h = L/resolution (or spacing); padding max((band+MARGIN+2)h, r+3h) (svierosion/api.py:94-101)."""
+    lo, hi = mesh.bounds
+    ext = float((hi - lo).max())
+    h = params.spacing if params.spacing else ext / params.resolution
+    pad = (params.band + MARGIN + 2) * h
+    if params.law == "svi":
+        pad = max(pad, params.radius + 3 * h)
+    return Grid.around(lo, hi, h, pad)
+
+
+def point_mesh_distance(points: np.ndarray, mesh: TriMesh, max_edge: float | None = None, k: int = 16,
+                        device=None, exact: bool = False) -> np.ndarray:
+    """Unsigned distance from points to a mesh: k-d tree candidates over triangle centroids
+    (bisected to <= max_edge), exact point-triangle distances on the device, and the
+    certified re-query of svierosion/api.py:104-168."""
+    from scipy.spatial import cKDTree
+    dev = _abi.require_device(get_device(device))
+    points = np.asarray(points, dtype=np.float64)
+    tri = mesh.triangles()
+    e = np.linalg.norm(tri[:, 1] - tri[:, 0], axis=1)
+    max_edge = max(max_edge or 0.0, 2.0 * float(np.median(e)) if len(e) else 0.0)
+    tri = _bisect_long(tri, max_edge)
+    cen = tri.mean(1)
+    rho = float(np.linalg.norm(tri - cen[:, None, :], axis=2).max()) if len(tri) else 0.0
+    tree = cKDTree(cen)
+    ref = points.mean(0) if len(points) else np.zeros(3)
+    T = torch.as_tensor(tri - ref, dtype=torch.float32, device=dev)
+
+    def evaluate(pidx, cand):
+        res = np.empty(len(pidx))
+        step = max(1, (1 << 22) // max(cand.shape[1], 1))
+        for s0 in range(0, len(pidx), step):
+            P = torch.as_tensor(points[pidx[s0:s0 + step]] - ref, dtype=torch.float32, device=dev)
+            Cc = torch.as_tensor(cand[s0:s0 + step], device=dev)
+            t = T[Cc.clamp_min(0)]
+            d2 = point_triangle_distance2(P[:, None, :].expand(-1, Cc.shape[1], -1), t[:, :, 0], t[:, :, 1],
+                                          t[:, :, 2])
+            d2 = torch.where(Cc >= 0, d2, torch.full_like(d2, float("inf")))
+            res[s0:s0 + step] = d2.min(1).values.sqrt().double().cpu().numpy()
+        return res
+
+    out = np.full(len(points), np.inf)
+    todo = np.arange(len(points))
+    kk = min(k, len(tri))
+    while len(todo):
+        dk, cand = tree.query(points[todo], k=kk, workers=-1)
+        dk, cand = dk.reshape(len(todo), kk), cand.reshape(len(todo), kk)
+        out[todo] = np.minimum(out[todo], evaluate(todo, cand))
+        complete = (dk[:, -1] >= out[todo] + rho) | (kk >= len(tri))
+        todo = todo[~complete]
+        if kk >= 256 and len(todo):
+            if not exact:
+                break
+            lists = tree.query_ball_point(points[todo], out[todo] + rho, workers=-1)
+            width = max(len(l) for l in lists)
+            cand = np.full((len(todo), width), -1, dtype=np.int64)
+            for i, l in enumerate(lists):
+                cand[i, :len(l)] = l
+            out[todo] = np.minimum(out[todo], evaluate(todo, cand))
+            break
+        kk = min(4 * kk, len(tri))
+    return out
+
+
+def _bisect_long(tri: np.ndarray, max_edge: float) -> np.ndarray:
+    """Longest-edge bisection until every edge <= max_edge (svierosion/sdf.py:73-96)."""
+    done = []
+    while len(tri):
+        e = np.stack([np.linalg.norm(tri[:, (q + 1) % 3] - tri[:, (q + 2) % 3], axis=1) for q in range(3)], 1)
+        ok = e.max(1) <= max_edge
+        done.append(tri[ok])
+        big = tri[~ok]
+        if not len(big):
+            break
+        j = e[~ok].argmax(1)
+        r = np.arange(len(big))
+        a, b, c = big[r, j], big[r, (j + 1) % 3], big[r, (j + 2) % 3]
+        mid = 0.5 * (b + c)
+        tri = np.concatenate([np.stack([a, b, mid], 1), np.stack([a, mid, c], 1)])
+    return np.concatenate(done) if done else np.zeros((0, 3, 3))
+
+
+def _stencil_nodes(p: torch.Tensor, shape) -> torch.Tensor:
+    """Flat indices of the 4^3 tricubic stencils of index points p."""
+    base = torch.floor(p).long() - 1
+    offs = torch.stack(torch.meshgrid(*(torch.arange(4, device=p.device),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    q = (base[:, None, :] + offs[None]).reshape(-1, 3)
+    for a in range(3):
+        q[:, a].clamp_(0, shape[a] - 1)
+    return torch.unique(q[:, 0] * (shape[1] * shape[2]) + q[:, 1] * shape[2] + q[:, 2])
+
+
+def _annotate(sim: ErosionSimulation, mesh: TriMesh, initial: TriMesh, law: SpeedLaw) -> None:
+    """Per-vertex depth, N_r or curvatures, and speed (svierosion/api.py:171-194)."""
+    if len(mesh.vertices) == 0:
+        return
+    grid = sim.ls.grid
+    mesh.vertex_data["depth"] = point_mesh_distance(mesh.vertices, initial, max_edge=4 * grid.spacing,
+                                                    device=sim.ls.device).astype(np.float32)
+    if law.requires_svi:
+        p = torch.as_tensor(grid.to_index(mesh.vertices), dtype=torch.float32, device=sim.ls.device)
+        N = sim.svi_field(_stencil_nodes(p, grid.shape))
+        Nv = tricubic(N.reshape(-1), grid.shape, p).double().cpu().numpy()
+        mesh.vertex_data["N"] = Nv.astype(np.float32)
+        mesh.vertex_data["speed"] = law.speed(torch.as_tensor(Nv)).numpy().astype(np.float32)
+    else:
+        ls = sim.ls
+        k1, k2 = ls.principal_curvatures(ls.band)
+        dense = torch.zeros(grid.size, dtype=torch.float32, device=ls.device)
+        dense[ls.band.long()] = k1
+        K1 = sample_field(dense, grid, mesh.vertices)
+        dense[ls.band.long()] = k2
+        K2 = sample_field(dense, grid, mesh.vertices)
+        mesh.vertex_data["H"] = (0.5 * (K1 + K2)).astype(np.float32)
+        mesh.vertex_data["K"] = (K1 * K2).astype(np.float32)
+        V, _ = law.speed_and_diffusivity(torch.as_tensor(K1), torch.as_tensor(K2))
+        mesh.vertex_data["speed"] = V.numpy().astype(np.float32)
+
+
+def initial_max_nr(sim: ErosionSimulation) -> float:
+    """max N_r at the closest points of the initial interface nodes (svierosion/api.py:220-223)."""
+    ls = sim.ls
+    cp = ls.closest_points(ls.interface_nodes())
+    if cp.shape[0] == 0:
+        return 0.0
+    N = sim.svi_field(_stencil_nodes(cp, ls.dims))
+    return float(tricubic(N.reshape(-1), ls.dims, cp).max())
+
+
+def erode(mesh: TriMesh, params: ErosionParams, callback=None, verbose: bool = False,
+          keep_state: bool = False) -> ErosionResult:
+    """Erode ``mesh`` under ``params.law`` until ``params.target_fraction`` of its volume remains."""
+    dev = _abi.require_device(get_device(params.device))
+    topo = mesh.edge_topology()
+    if topo["boundary_edges"] or topo["nonmanifold_edges"]:
+        warnings.warn(f"input mesh is not closed ({topo['boundary_edges']} boundary and "
+                      f"{topo['nonmanifold_edges']} non-manifold edges); inside/outside is decided by a "
+                      "majority vote of ray casts along the three axes, check the result", RuntimeWarning)
+    lo, hi = mesh.bounds
+    law = params.make_law(length_scale=float((hi - lo).max()))
+    timings = {}
+    t0 = time.time()
+    grid = make_grid(mesh, params)
+    phi = mesh_to_sdf(mesh, grid, band=params.band, device=dev, verbose=verbose)
+    ls = LevelSet(phi, grid, width=params.band)
+    torch.cuda.synchronize(dev)
+    timings["sdf"] = time.time() - t0
+    if verbose:
+        print(f"[erode] {grid.describe()}, device {dev}; law: {law.describe()}")
+    sim = ErosionSimulation(ls, law, cfl=params.cfl, feedback_cfl=params.feedback_cfl,
+                            parabolic_cfl=params.parabolic_cfl, reinit_distance=params.reinit_distance,
+                            reinit_iterations=params.reinit_iterations)
+    if law.requires_svi and law.s == 1:
+        nmax = initial_max_nr(sim)
+        if nmax >= law.t:
+            raise ValueError(f"max N_r on the initial surface is {nmax:.4f} >= t = {law.t}; "
+                             "choose t closer to 1 (the flow needs t > N_r everywhere)")
+        if verbose:
+            print(f"[erode] max N_r on initial surface = {nmax:.4f} (t = {law.t})")
+    targets = sorted({float(f) for f in params.snapshots if f > params.target_fraction},
+                     reverse=True) + [params.target_fraction]
+    snaps = []
+    t1 = time.time()
+    tmesh = 0.0
+    res = None
+    for frac in targets:
+        res = sim.run(target_fraction=frac, max_steps=params.max_steps, callback=callback)
+        tm = time.time()
+        m = extract_surface(ls, project=params.project_vertices)
+        _annotate(sim, m, mesh, law)
+        tmesh += time.time() - tm
+        snaps.append((res.fraction, sim.time, m))
+        if res.stop_reason != "target_volume":
+            break
+    timings["evolve"] = time.time() - t1 - tmesh
+    timings["mesh"] = tmesh
+    final = snaps[-1][2]
+    out = ErosionResult(final, snaps[:-1], sim.history, grid, res.stop_reason, sim.time, res.fraction,
+                        sim.initial_volume, mesh.volume(), timings,
+                        {**asdict(params), "law_description": law.describe(), **law.params()}, mesh)
+    if keep_state:
+        out.level_set, out.simulation = ls, sim
+    return out
+
+
+def volume_fraction_report(res: ErosionResult) -> str:
+    v_in = res.initial_volume_mesh
+    v_out = res.mesh.volume() if len(res.mesh.faces) else 0.0
+    return (f"mesh volume {v_in:.6g} -> {v_out:.6g} (ratio {v_out / v_in:.5f}); "
+            f"level-set volume ratio {res.fraction:.5f}")
+
+
+__all__ = ["ErosionParams", "ErosionResult", "erode", "make_grid", "point_mesh_distance", "volume_fraction_report",
+           "StepRecord"]

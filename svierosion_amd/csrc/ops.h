@@ -169,6 +169,38 @@ __device__ __forceinline__ float tricubic(const float* __restrict__ f, const Dim
   return out;
 }
 
+// Derivatives of the Lagrange cubic weights (svierosion/kernels.py:139-141).
+__device__ __forceinline__ void cubic_dw(float t, float w[4]) {
+  w[0] = -(3.0f * t * t - 6.0f * t + 2.0f) / 6.0f;
+  w[1] = (3.0f * t * t - 4.0f * t - 1.0f) / 2.0f;
+  w[2] = -(3.0f * t * t - 2.0f * t - 2.0f) / 2.0f;
+  w[3] = (3.0f * t * t - 1.0f) / 6.0f;
+}
+
+// Tricubic value and gradient (per cell) at p (svierosion/kernels.py:232-253).
+__device__ __forceinline__ float tricubic_grad(const float* __restrict__ f, const Dims& d, const float p[3],
+                                               float g[3]) {
+  float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+  float wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+  cubic_w(p[0] - fx, wx); cubic_w(p[1] - fy, wy); cubic_w(p[2] - fz, wz);
+  cubic_dw(p[0] - fx, dx); cubic_dw(p[1] - fy, dy); cubic_dw(p[2] - fz, dz);
+  int64_t b0 = ((int64_t)fx - 1) * d.sx() + ((int64_t)fy - 1) * d.sy() + ((int64_t)fz - 1);
+  float val = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      const float* row = f + b0 + a * d.sx() + b * d.sy();
+      for (int c = 0; c < 4; ++c) {
+        float v = row[c];
+        val = val + wx[a] * wy[b] * wz[c] * v;
+        gx = gx + dx[a] * wy[b] * wz[c] * v;
+        gy = gy + wx[a] * dy[b] * wz[c] * v;
+        gz = gz + wx[a] * wy[b] * dz[c] * v;
+      }
+    }
+  g[0] = gx; g[1] = gy; g[2] = gz;
+  return val;
+}
+
 // Trilinear interpolation at index point p (svierosion/kernels.py:257-270).
 __device__ __forceinline__ float trilinear(const float* __restrict__ f, const Dims& d, const float p[3]) {
   float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);

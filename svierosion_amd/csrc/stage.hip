@@ -292,6 +292,16 @@ __global__ void k_op_tricubic(const float* f, Dims d, const float* p, int64_t n,
   float pp[3] = {p[3 * q], p[3 * q + 1], p[3 * q + 2]};
   out[q] = tricubic(f, d, pp);
 }
+__global__ void k_op_tricubic_grad(const float* f, Dims d, const float* p, int64_t n, float* out, float* grad) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  float pp[3] = {p[3 * q], p[3 * q + 1], p[3 * q + 2]};
+  float g[3];
+  out[q] = tricubic_grad(f, d, pp, g);
+  grad[3 * q] = g[0];
+  grad[3 * q + 1] = g[1];
+  grad[3 * q + 2] = g[2];
+}
 __global__ void k_op_trilinear(const float* f, Dims d, const float* p, int64_t n, float* out) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
@@ -451,6 +461,13 @@ int svimx_op_tricubic(void* stream, const float* field, int nx, int ny, int nz, 
                       float* out) {
   if (n <= 0) return SVIMX_OK;
   hipLaunchKernelGGL(k_op_tricubic, GRID1(n), field, Dims{nx, ny, nz}, p, n, out);
+  SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+}
+int svimx_op_tricubic_gradient(void* stream, const float* field, int nx, int ny, int nz, const float* p, int64_t n,
+                               float* out, float* grad) {
+  if (n <= 0) return SVIMX_OK;
+  hipLaunchKernelGGL(k_op_tricubic_grad, GRID1(n), field, Dims{nx, ny, nz}, p, n, out, grad);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
 }

@@ -1,0 +1,96 @@
+"""Host-side pieces of the drop-in API that need no GPU: grid sizing, meshes,
+PLY round trips (reference tests/test_geometry.py:20-49), generators, law parameters,
+surface extraction topology."""
+
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import stencils as S
+
+
+def test_grid_matches_oracle_sizing():
+    from svierosion_amd.api import ErosionParams, make_grid
+    from svierosion_amd import shapes
+    m = shapes.cube(1.0)
+    g = make_grid(m, ErosionParams(law="svi", radius=0.1, resolution=64))
+    assert g.shape == (90, 90, 90) and g.spacing == 1 / 64
+    assert np.allclose(g.origin, -0.6953125)
+    o, h, shp = S.make_grid(*m.bounds, resolution=64, radius=0.1)
+    assert shp == g.shape and np.allclose(o, g.origin)
+
+
+def test_ply_roundtrip_binary_and_ascii():
+    from svierosion_amd import shapes
+    from svierosion_amd.plyio import read_ply, write_ply
+    m = shapes.torus(1.0, 0.3, 32, 16)
+    q = m.vertices[:, 2].copy()
+    with tempfile.TemporaryDirectory() as d:
+        for binary, prec in ((True, "float"), (True, "double"), (False, "double")):
+            p = os.path.join(d, f"t_{binary}_{prec}.ply")
+            write_ply(p, m, binary=binary, precision=prec, vertex_data={"quality": q})
+            r = read_ply(p)
+            tol = 1e-6 if prec == "float" else 1e-12
+            assert np.allclose(r.vertices, m.vertices, atol=tol)
+            assert np.array_equal(r.faces, m.faces)
+            assert np.allclose(r.vertex_data["quality"], q, atol=1e-6)
+
+
+def test_ply_quads_are_triangulated():
+    from svierosion_amd.plyio import read_ply
+    txt = ("ply\nformat ascii 1.0\nelement vertex 4\nproperty float x\nproperty float y\n"
+           "property float z\nelement face 1\nproperty list uchar int vertex_indices\nend_header\n"
+           "0 0 0\n1 0 0\n1 1 0\n0 1 0\n4 0 1 2 3\n")
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "q.ply")
+        open(p, "w").write(txt)
+        assert read_ply(p).faces.shape == (2, 3)
+
+
+@pytest.mark.parametrize("name", ["cube", "icosphere", "torus", "sh_fragment", "noisy_ellipsoid"])
+def test_generators_closed_and_outward(name):
+    from svierosion_amd import shapes
+    make = {"cube": lambda: shapes.cube(1.0), "icosphere": lambda: shapes.icosphere(3),
+            "torus": lambda: shapes.torus(1.0, 0.35, 32, 16), "sh_fragment": lambda: shapes.sh_fragment(3, seed=1),
+            "noisy_ellipsoid": lambda: shapes.noisy_ellipsoid(3, seed=2)}[name]
+    m = make()
+    assert m.is_watertight() and m.is_consistently_oriented() and m.signed_volume() > 0
+
+
+def test_icosphere_counts():
+    from svierosion_amd import shapes
+    for s in range(4):
+        m = shapes.icosphere(s)
+        assert len(m.faces) == 20 * 4 ** s
+        assert np.allclose(np.linalg.norm(m.vertices, axis=1), 1.0)
+
+
+def test_law_parameters_and_errors():
+    from svierosion_amd.api import ErosionParams
+    from svierosion_amd.flows import SVIPowerLaw, flat_speed
+    law = SVIPowerLaw(0.3, k=2, s=1, t=0.93)
+    assert abs(law._raw(law.n_cap) / flat_speed(law) - 100) < 1e-6
+    for bad in (dict(radius=-1), dict(radius=1, k=-1), dict(radius=1, s=2), dict(radius=1, s=1, t=0.4)):
+        with pytest.raises(ValueError):
+            SVIPowerLaw(**bad)
+    with pytest.raises(ValueError):
+        ErosionParams(law="svi").make_law()
+    with pytest.raises(ValueError):
+        ErosionParams(law="nope").make_law()
+    p = ErosionParams(law="gauss", alpha=0.25)
+    assert abs(p.make_law(2.0).delta - 0.25) < 1e-15
+
+
+def test_marching_tetrahedra_closed_and_accurate():
+    from svierosion_amd.meshio import TriMesh
+    from svierosion_amd.surface import _marching_tetrahedra
+    h = 0.04
+    ax = np.arange(-1.3, 1.3 + 1e-9, h)
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    phi = (np.sqrt(X ** 2 + (1.2 * Y) ** 2 + Z ** 2) - 1.0).astype(np.float32)
+    v, f = _marching_tetrahedra(phi)
+    m = TriMesh(-1.3 + h * v, f)
+    assert m.is_watertight() and m.is_consistently_oriented()
+    assert abs(m.signed_volume() / (4 / 3 * np.pi / 1.2) - 1) < 2e-3

@@ -140,6 +140,10 @@ int svimx_nr_reset(void* stream, void* plan);
 int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t n);
 int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws,
                   int64_t ws_bytes);
+/* The two halves of svimx_nr_exec: compact the marked tiles into the plan's
+ * tile list, then run the summation kernel over that list. */
+int svimx_nr_select(void* stream, void* plan);
+int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws, int64_t ws_bytes);
 
 /* ---------------------------------------------------------------- reinitialisation
  * Min (2010) PDE reinitialisation on the band dilated by `dilation` cells minus

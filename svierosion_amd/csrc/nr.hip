@@ -211,25 +211,39 @@ int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t 
   return SVIMX_OK;
 }
 
-int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws, int64_t ws_bytes) {
+int svimx_nr_select(void* stream, void* plan) {
   SVIMX_TRY_BEGIN
   NrPlan* p = static_cast<NrPlan*>(plan);
-  SVIMX_REQUIRE(p && occupancy && nr_out, SVIMX_EBADARG, "null argument");
-  Workspace w;
-  if (!carve_workspace(ws, ws_bytes, p->d.size(), w)) return SVIMX_EBADARG;
+  SVIMX_REQUIRE(p, SVIMX_EBADARG, "null plan");
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(k_nr_zero_count, dim3(1), dim3(1), 0, st, p->count);
   SVIMX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_nr_compact_tiles, dim3(blocks_for(p->ntiles, 256)), dim3(256), 0, st, p->marks, p->ntiles,
                      p->list, p->count);
   SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+  SVIMX_TRY_END
+}
+
+int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws, int64_t ws_bytes) {
+  SVIMX_TRY_BEGIN
+  NrPlan* p = static_cast<NrPlan*>(plan);
+  SVIMX_REQUIRE(p && occupancy && nr_out, SVIMX_EBADARG, "null argument");
+  Workspace w;
+  if (!carve_workspace(ws, ws_bytes, p->d.size(), w)) return SVIMX_EBADARG;
   const size_t lds = sizeof(float) * 2 * (size_t)(kTY + 2 * p->m) * nr_pitch(p->m);
   int grid = (int)std::min<int64_t>(p->ntiles, 256 * 8);
-  hipLaunchKernelGGL(k_nr_planes, dim3(grid), dim3(256), lds, st, occupancy, p->d, p->list, p->count, p->nty,
-                     p->ntz, p->m, p->cols, p->col_start, p->weights, p->scale, p->negbv, nr_out, w.results);
+  hipLaunchKernelGGL(k_nr_planes, dim3(grid), dim3(256), lds, as_stream(stream), occupancy, p->d, p->list, p->count,
+                     p->nty, p->ntz, p->m, p->cols, p->col_start, p->weights, p->scale, p->negbv, nr_out, w.results);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
   SVIMX_TRY_END
+}
+
+int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws, int64_t ws_bytes) {
+  int rc = svimx_nr_select(stream, plan);
+  if (rc) return rc;
+  return svimx_nr_sum(stream, plan, occupancy, nr_out, ws, ws_bytes);
 }
 
 }  // extern "C"

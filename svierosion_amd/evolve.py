@@ -18,6 +18,7 @@ step's first stage.
 
 from __future__ import annotations
 
+import contextlib
 import math
 import time as _time
 from dataclasses import dataclass, field
@@ -96,6 +97,26 @@ class ErosionSimulation:
         self.history: list[StepRecord] = []
         self._displacement = 0.0
         self.timers = {"stats_sync": 0.0, "volume_sync": 0.0}
+        # name -> [(start, end) torch.cuda.Event pairs]; None disables event recording
+        self.kernel_events = None
+
+    @contextlib.contextmanager
+    def _span(self, name: str):
+        """Bracket the launches of one stage with HIP events on the current stream."""
+        if self.kernel_events is None:
+            yield
+            return
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        yield
+        b.record()
+        self.kernel_events.setdefault(name, []).append((a, b))
+
+    def kernel_times_ms(self) -> dict:
+        """Per-stage list of event durations (synchronises)."""
+        torch.cuda.synchronize()
+        return {k: [a.elapsed_time(b) for a, b in v] for k, v in (self.kernel_events or {}).items()}
 
     # ------------------------------------------------------------- checks
     def _check_convolution_margin(self) -> None:
@@ -128,28 +149,43 @@ class ErosionSimulation:
         ls.cur = buf
         try:
             if self._frac_valid != buf:
-                ls.fractions(self._plan(), self.cutoff[1])
+                with self._span("fractions"):
+                    ls.fractions(self._plan(), self.cutoff[1])
                 self._frac_valid = buf
             phi = ls.flat
             if self.conv is not None:
-                self.conv.exec_marked(ls.occupancy, self.field, ls.ws)
+                _abi.check(_abi.lib.svimx_nr_select(st, self.conv.plan), "N_r select")
+                with self._span("nr_sum"):
+                    _abi.check(_abi.lib.svimx_nr_sum(st, self.conv.plan, _abi.ptr(ls.occupancy), _abi.ptr(self.field),
+                                                     _abi.ptr(ls.ws), ls.ws.numel()), "N_r")
             else:
-                _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
-                                                     _abi.ptr(self.k1), _abi.ptr(self.k2)))
-            _abi.check(_abi.lib.svimx_velocity(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
-                                               self._law_abi, _abi.ptr(self.field), _abi.ptr(self.k1),
-                                               _abi.ptr(self.k2), self.cutoff[0], self.cutoff[1], int(stats),
-                                               _abi.ptr(self.V), _abi.ptr(ls.ws), ls.ws.numel()), "velocity")
+                with self._span("curvatures"):
+                    _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims,
+                                                         ls.h, _abi.ptr(self.k1), _abi.ptr(self.k2)))
+            with self._span("velocity"):
+                self._velocity_call(st, phi, stats)
         finally:
             ls.cur = saved
 
+    def _velocity_call(self, st, phi, stats: bool) -> None:
+        ls = self.ls
+        _abi.check(_abi.lib.svimx_velocity(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
+                                               self._law_abi, _abi.ptr(self.field), _abi.ptr(self.k1),
+                                               _abi.ptr(self.k2), self.cutoff[0], self.cutoff[1], int(stats),
+                                               _abi.ptr(self.V), _abi.ptr(ls.ws), ls.ws.numel()), "velocity")
+
     def _stage(self, stage: int, src: int, u0: int, dst: int, dt: float) -> None:
+        ls = self.ls
+        with self._span("rk_stage"):
+            self._stage_call(stage, src, u0, dst, dt)
+        if self._frac_valid == dst:
+            self._frac_valid = None
+
+    def _stage_call(self, stage: int, src: int, u0: int, dst: int, dt: float) -> None:
         ls = self.ls
         _abi.check(_abi.lib.svimx_rk_stage(_abi.stream(), stage, _abi.ptr(ls.bufs[src]), _abi.ptr(ls.bufs[u0]),
                                            _abi.ptr(ls.bufs[dst]), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
                                            float(dt), _abi.ptr(self.V), _abi.ptr(self.L0)), "rk stage")
-        if self._frac_valid == dst:
-            self._frac_valid = None
 
     def _rk3_tail(self, a: int, b: int, c: int, dt: float) -> None:
         """Stages 1 and 2 after stage 0 wrote bufs[b]; the result lands in bufs[b]."""
@@ -163,7 +199,8 @@ class ErosionSimulation:
         saved = ls.cur
         ls.cur = buf
         try:
-            ls.fractions(self._plan(), self.cutoff[1])
+            with self._span("fractions"):
+                ls.fractions(self._plan(), self.cutoff[1])
             self._frac_valid = buf
             t0 = _time.perf_counter()
             r = ls.results()
@@ -268,7 +305,8 @@ class ErosionSimulation:
             did_reinit = False
             reached = target is not None and vnew <= target * (1 + tol)
             if self._displacement >= self.reinit_distance * ls.h and not reached:
-                ls.reinitialize(self.reinit_iterations)
+                with self._span("reinit"):
+                    ls.reinitialize(self.reinit_iterations)
                 self._alloc_band_arrays()
                 self._frac_valid = None
                 self._displacement = 0.0

@@ -117,6 +117,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-stage HIP events")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of this many MiB (profiler byte calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this workload is a single erosion job on one MI355X "
@@ -124,6 +126,17 @@ def main(argv=None):
         return 2
     import torch
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # calibrate the HBM byte counts read for the workload's kernels
+        nbytes = int(args.copy_calibration) << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(f"[bench] calibration copy of {nbytes} bytes x3 done", file=sys.stderr, flush=True)
     mesh, grid, ls, sim, law, info = build_workload(args)
     print(f"[bench] workload ready: {info}", file=sys.stderr, flush=True)
     sim.run(target_fraction=None, max_steps=args.warmup)

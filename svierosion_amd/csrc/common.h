@@ -66,6 +66,14 @@ struct Workspace {
 
 constexpr int kBandThreads = 256;       // threads per block of band-list kernels
 constexpr int kPartialSlots = 16;       // doubles written per block by reduction kernels
+constexpr int kMaxReduceBlocks = 2048;  // grid cap of reducing kernels (grid-stride loops)
+
+// Grid of a reducing kernel over n items: grid-stride with at most kMaxReduceBlocks
+// blocks, so the finalisation reads a bounded number of partials.
+inline int reduce_grid(int64_t n) {
+  int64_t b = (n + kBandThreads - 1) / kBandThreads;
+  return (int)(b < 1 ? 1 : (b > kMaxReduceBlocks ? kMaxReduceBlocks : b));
+}
 constexpr int kCompactThreads = 256;
 constexpr int kCompactPerThread = 16;
 constexpr int64_t kCompactChunk = (int64_t)kCompactThreads * kCompactPerThread;

@@ -107,24 +107,23 @@ __global__ void __launch_bounds__(kBandThreads) k_band_classify(float* __restric
                                                                 float* __restrict__ occ, Dims d, float cap,
                                                                 float thr, double* __restrict__ partials) {
   __shared__ double red[4];
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = d.size();
   double inside_out = 0.0;
   double mn[3] = {1e30, 1e30, 1e30}, mx[3] = {-1e30, -1e30, -1e30};
-  if (e < n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     float v = fminf(fmaxf(phi[e], -cap), cap);
     phi[e] = v;
     if (c1) c1[e] = v;
     if (c2) c2[e] = v;
     occ[e] = v < 0.0f ? 1.0f : 0.0f;
     bool inb = fabsf(v) < thr;
-    if (!inb && v < 0.0f) inside_out = 1.0;
+    if (!inb && v < 0.0f) inside_out += 1.0;
     if (inb) {
       int i, j, k;
       unflat(d, e, i, j, k);
-      mn[0] = mx[0] = i;
-      mn[1] = mx[1] = j;
-      mn[2] = mx[2] = k;
+      mn[0] = fmin(mn[0], (double)i); mx[0] = fmax(mx[0], (double)i);
+      mn[1] = fmin(mn[1], (double)j); mx[1] = fmax(mx[1], (double)j);
+      mn[2] = fmin(mn[2], (double)k); mx[2] = fmax(mx[2], (double)k);
     }
   }
   double r = block_reduce<0>(inside_out, red);
@@ -166,7 +165,7 @@ int svimx_band_rebuild(void* stream, float* phi, float* phi_copy1, float* phi_co
   const double capd = width * h;
   const float cap = (float)capd;
   const float thr = (float)(capd * (1.0 - 1e-5));
-  const int nblk = blocks_for(n, kBandThreads);
+  const int nblk = reduce_grid(n);
   hipLaunchKernelGGL(k_band_classify, dim3(nblk), dim3(kBandThreads), 0, st, phi, phi_copy1, phi_copy2,
                      occupancy, d, cap, thr, w.partials);
   SVIMX_CHECK_LAUNCH();

@@ -61,7 +61,9 @@ __global__ void __launch_bounds__(256)
   const int PY = kTY + 2 * m, PZ = nr_pitch(m), P = PY * PZ;
   const int ntile = *count;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)ntile;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is uniform per wave; readfirstlane lets the compiler keep dx,
+  // the column table and the weights in scalar registers (s_load)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ly = lane >> 3, lz = lane & 7;
   const int span = kTZ + 2 * m;
   for (int it = blockIdx.x; it < ntile; it += gridDim.x) {

@@ -113,16 +113,15 @@ __global__ void __launch_bounds__(kBandThreads)
                 float* __restrict__ occ, uint8_t* __restrict__ marks, int nty, int ntz, float cut, Hi3 hi,
                 double* __restrict__ partials) {
   __shared__ double red[4];
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double fsum = 0.0, neg = 0.0, pos = 0.0;
-  if (q < nb) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t n = band[q];
     Grad g = central(phi, n, d);
     float fr = cell_fraction(g);
     occ[n] = fr;
-    fsum = fr;
-    neg = g.c < 0.0f ? 1.0 : 0.0;
-    pos = g.c > 0.0f ? 1.0 : 0.0;
+    fsum += fr;
+    neg = g.c < 0.0f ? 1.0 : neg;
+    pos = g.c > 0.0f ? 1.0 : pos;
     if (marks && fabsf(g.c) < cut) {
       int i, j, k;
       unflat(d, n, i, j, k);
@@ -170,12 +169,11 @@ __global__ void __launch_bounds__(kBandThreads)
                const float* __restrict__ k2d, float cut_lo, float cut_hi, Hi3 hi, int stats,
                float* __restrict__ V, double* __restrict__ partials) {
   __shared__ double red[4];
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float vmax = 0.0f;
   float qmin = INFINITY, qmax = -INFINITY, dvmax = -INFINITY, dmax = -INFINITY;
   float qmin_a = INFINITY, qmax_a = -INFINITY, dvmax_a = -INFINITY, dmax_a = -INFINITY;
   double capped = 0, vsum = 0, nnear = 0, capped_a = 0, vsum_a = 0, nall = 0;
-  if (q < nb) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t n = band[q];
     Grad g = central(phi, n, d);
     const float x = fabsf(g.c);
@@ -192,10 +190,12 @@ __global__ void __launch_bounds__(kBandThreads)
         v = vraw * cutoff_factor(x, cut_lo, cut_hi);
         if (stats) {
           float dv = svi_dspeed(L, N);
-          float cp = N > L.n_cap ? 1.0f : 0.0f;
-          qmin_a = N; qmax_a = N; dvmax_a = dv; capped_a = cp; vsum_a = vraw; nall = 1;
+          double cp = N > L.n_cap ? 1.0 : 0.0;
+          qmin_a = fminf(qmin_a, N); qmax_a = fmaxf(qmax_a, N); dvmax_a = fmaxf(dvmax_a, dv);
+          capped_a += cp; vsum_a += vraw; nall += 1;
           if (near) {
-            qmin = N; qmax = N; dvmax = dv; capped = cp; vsum = vraw; nnear = 1;
+            qmin = fminf(qmin, N); qmax = fmaxf(qmax, N); dvmax = fmaxf(dvmax, dv);
+            capped += cp; vsum += vraw; nnear += 1;
           }
         }
       }
@@ -210,14 +210,16 @@ __global__ void __launch_bounds__(kBandThreads)
       v = vraw * cutoff_factor(x, cut_lo, cut_hi);
       if (stats) {
         float H = 0.5f * (k1 + k2);
-        qmin_a = H; qmax_a = H; dmax_a = D; vsum_a = vraw; nall = 1;
+        qmin_a = fminf(qmin_a, H); qmax_a = fmaxf(qmax_a, H); dmax_a = fmaxf(dmax_a, D);
+        vsum_a += vraw; nall += 1;
         if (near) {
-          qmin = H; qmax = H; dmax = D; vsum = vraw; nnear = 1;
+          qmin = fminf(qmin, H); qmax = fmaxf(qmax, H); dmax = fmaxf(dmax, D);
+          vsum += vraw; nnear += 1;
         }
       }
     }
     V[q] = v;
-    vmax = v;
+    vmax = fmaxf(vmax, v);
   }
   if (!stats) return;
   double r[15];
@@ -368,8 +370,8 @@ int svimx_fractions(void* stream, const float* phi, const int32_t* band, int64_t
     nty = plan->nty;
     ntz = plan->ntz;
   }
-  int nblk = blocks_for(nb, kBandThreads);
-  if (nblk > 0) {
+  int nblk = reduce_grid(nb);
+  if (nb > 0) {
     hipLaunchKernelGGL(k_fractions, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
                        occupancy, marks, nty, ntz, (float)(cut_hi * h), hi_of(d), w.partials);
     SVIMX_CHECK_LAUNCH();
@@ -404,8 +406,8 @@ int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t 
   SVIMX_REQUIRE(law->kind != 0 || field, SVIMX_EBADARG, "SVI law needs the N_r field");
   SVIMX_REQUIRE(law->kind == 0 || (k1 && k2), SVIMX_EBADARG, "curvature law needs k1/k2 fields");
   hipStream_t st = as_stream(stream);
-  int nblk = blocks_for(nb, kBandThreads);
-  if (nblk > 0) {
+  int nblk = reduce_grid(nb);
+  if (nb > 0) {
     hipLaunchKernelGGL(k_velocity, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
                        law_to_f(*law), field, k1, k2, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
                        V, w.partials);

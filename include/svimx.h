@@ -78,6 +78,13 @@ const char* svimx_last_error(void);
 int svimx_version(void);
 int64_t svimx_workspace_bytes(int nx, int ny, int nz);
 
+/* The occupancy field f (1 inside, 0 outside, cell fraction on the band) is
+ * stored padded: node (x, y, z) lives at ((x+16)*(ny+32) + (y+16))*pz + z+16
+ * with pz = round_up(nz + 40, 4); the pad cells stay 0.  This is the element
+ * count of that array (float32, 16-byte aligned, zero-initialised by the
+ * caller). */
+int64_t svimx_occupancy_elems(int nx, int ny, int nz);
+
 /* ---------------------------------------------------------------- band
  * Clamp phi to +-width*h, band = nodes with |phi| < width*h*(1-1e-5) (float32
  * compare), write the sorted band, copy the clamped field into phi_copy1/2

@@ -39,6 +39,7 @@ _SIGNATURES = {
     "svimx_last_error": (C.c_char_p, []),
     "svimx_version": (_I, []),
     "svimx_workspace_bytes": (_I64, [_I, _I, _I]),
+    "svimx_occupancy_elems": (_I64, [_I, _I, _I]),
     "svimx_band_rebuild": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _I, _P, _I64, _P, _I64, _P]),
     "svimx_fractions": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _P, _D, _P, _I64]),
     "svimx_curvatures": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _P]),
@@ -125,6 +126,22 @@ def require_device(device) -> torch.device:
 def workspace(shape, device) -> torch.Tensor:
     nbytes = int(lib.svimx_workspace_bytes(*map(int, shape)))
     return torch.zeros(nbytes, dtype=torch.uint8, device=device)
+
+
+OCC_PAD = 16  # zero cells in front of every axis of the occupancy layout
+
+
+def occupancy_zeros(shape, device) -> torch.Tensor:
+    """Zeroed occupancy array in the padded layout of svimx_occupancy_elems."""
+    return torch.zeros(int(lib.svimx_occupancy_elems(*map(int, shape))), dtype=torch.float32, device=device)
+
+
+def occupancy_view(occ: torch.Tensor, shape) -> torch.Tensor:
+    """Dense (nx, ny, nz) view of a padded occupancy array."""
+    nx, ny, nz = map(int, shape)
+    pz = occ.numel() // ((nx + 2 * OCC_PAD) * (ny + 2 * OCC_PAD))
+    v = occ.view(nx + 2 * OCC_PAD, ny + 2 * OCC_PAD, pz)
+    return v[OCC_PAD:OCC_PAD + nx, OCC_PAD:OCC_PAD + ny, OCC_PAD:OCC_PAD + nz]
 
 
 def results_view(ws: torch.Tensor) -> torch.Tensor:

@@ -115,12 +115,12 @@ __global__ void __launch_bounds__(kBandThreads) k_band_classify(float* __restric
     phi[e] = v;
     if (c1) c1[e] = v;
     if (c2) c2[e] = v;
-    occ[e] = v < 0.0f ? 1.0f : 0.0f;
+    int i, j, k;
+    unflat(d, e, i, j, k);
+    occ[OccLayout::of(d.nx, d.ny, d.nz).at(i, j, k)] = v < 0.0f ? 1.0f : 0.0f;
     bool inb = fabsf(v) < thr;
     if (!inb && v < 0.0f) inside_out += 1.0;
     if (inb) {
-      int i, j, k;
-      unflat(d, e, i, j, k);
       mn[0] = fmin(mn[0], (double)i); mx[0] = fmax(mx[0], (double)i);
       mn[1] = fmin(mn[1], (double)j); mx[1] = fmax(mx[1], (double)j);
       mn[2] = fmin(mn[2], (double)k); mx[2] = fmax(mx[2], (double)k);

@@ -1,35 +1,36 @@
-// N_r by direct summation of the ball-weight convolution over marked tiles.
+// N_r by direct summation of the ball-weight convolution.
 //
 // N_r(x) = 1 - h^3 * sum_d w(d) f(x + d) / |B_r|, w the exact ball/cell
 // overlaps (svierosion/svi.py:24-55), f the occupancy (1 inside, cell fraction
 // in the band).  The reference evaluates the same sum as a circular FFT
 // convolution over the whole grid (svierosion/svi.py:84-93); the margin check
 // keeps the solid m+1 cells from the boundary, so the circular and the linear
-// sum agree wherever f can be nonzero, and reading f = 0 outside the grid is
-// exact.
+// sum agree wherever f can be nonzero, and the zero padding of the occupancy
+// layout is exact.  N_r is only needed at the tricubic stencils of the closest
+// points, so only those nodes are evaluated.
 //
-// Work decomposition: one workgroup of 4 waves per 4x8x8 output tile; wave w
-// owns the x-layer x0+w (64 lanes = 8x8 (y, z) nodes).  The source x-planes
-// x0-m .. x0+3+m stream through a double-buffered LDS plane of (8+2m) x
-// (8+2m) values; for each plane every wave sums the columns of its offset
-// dx = X - (x0+w), which is uniform across the wave, so the per-(dx, dy)
-// column table is read through scalar loads and no lane diverges.  Column
-// partial sums (<= 2m+1 taps) are accumulated with Neumaier compensation, so
-// the float32 sum of ~4/3 pi R^3 terms keeps ~1e-8 relative accuracy.
+// Run mode (ball half-width m <= 16).  Work unit: a z-run of 8 nodes.  Each
+// lane owns one run: for every (dx, dy) column of the ball (uniform across the
+// wave, read through scalar loads) it loads the 8+2M' occupancy values of the
+// column window as aligned float4s (M' = m rounded up to 4) and accumulates
+// the 2k+1 nonzero taps (the ball is symmetric, so a column's taps are
+// dz = -k..k) into 8 accumulators with a fully unrolled sequence per k.  Column
+// sums are grouped per dx before entering the total, keeping float32 rounding
+// ~1e-7 relative.
+//
+// Tile mode (larger balls).  One workgroup of 4 waves per 4x8x8 tile, wave w
+// owning x-layer x0+w; source x-planes stream through double-buffered LDS and
+// each wave sums the columns of its (wave-uniform) offset dx = X - (x0+w).
 #include <cmath>
 #include <vector>
 
 #include "common.h"
+#include "compact.h"
 #include "nr_plan.h"
 
 namespace svimx {
 
-__global__ void k_nr_compact_tiles(const uint8_t* __restrict__ marks, int64_t ntiles, int32_t* __restrict__ list,
-                                   int32_t* __restrict__ count) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ntiles && marks[t]) list[atomicAdd(count, 1)] = (int32_t)t;
-}
-
+// ---------------------------------------------------------------- selection
 __global__ void k_nr_zero_count(int32_t* count) { *count = 0; }
 
 __global__ void k_nr_clear_marks(uint8_t* marks, int64_t n) {
@@ -37,14 +38,134 @@ __global__ void k_nr_clear_marks(uint8_t* marks, int64_t n) {
   if (i < n) marks[i] = 0;
 }
 
-__global__ void k_nr_mark_nodes(uint8_t* marks, int nty, int ntz, Dims d, const int32_t* nodes, int64_t n) {
+__global__ void k_nr_compact_tiles(const uint8_t* __restrict__ marks, int64_t ntiles, int32_t* __restrict__ list,
+                                   int32_t* __restrict__ count) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles && marks[t]) list[atomicAdd(count, 1)] = (int32_t)t;
+}
+
+__global__ void k_nr_mark_nodes(uint8_t* marks, int run_mode, int nty, int ntz, int nzr, Dims d,
+                                const int32_t* nodes, int64_t n) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   int i, j, k;
   unflat(d, nodes[q], i, j, k);
-  marks[((int64_t)(i / kTX) * nty + j / kTY) * ntz + k / kTZ] = 1;
+  if (run_mode)
+    marks[((int64_t)i * d.ny + j) * nzr + k / kRunLen] = 1;
+  else
+    marks[((int64_t)(i / kTX) * nty + j / kTY) * ntz + k / kTZ] = 1;
 }
 
+// dil(x, y, zr) = OR_{b=0..3} marks(x, y-b, zr)
+__global__ void k_runs_dilate_y(const uint8_t* __restrict__ marks, uint8_t* __restrict__ dil, int ny, int nzr,
+                                int64_t nruns) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nruns) return;
+  const int y = (int)((r / nzr) % ny);
+  uint8_t v = 0;
+  for (int b = 0; b <= 3 && b <= y; ++b) v |= marks[r - (int64_t)b * nzr];
+  dil[r] = v;
+}
+
+// need(x, y, zr) = OR_{a=0..3} dil(x-a, y, zr)
+struct RunNeed {
+  const uint8_t* dil;
+  int64_t plane;  // ny * nzr
+  __device__ bool operator()(int64_t r) const {
+    const int64_t x = r / plane;
+    uint8_t v = 0;
+    for (int a = 0; a <= 3 && a <= x; ++a) v |= dil[r - a * plane];
+    return v != 0;
+  }
+};
+
+// ---------------------------------------------------------------- run-mode summation
+template <int K, int MP, int WIN>
+__device__ __forceinline__ void column_taps(const float* __restrict__ wc, const float (&win)[WIN],
+                                            float (&part)[kRunLen]) {
+  if constexpr (K <= MP) {
+#pragma unroll
+    for (int t = 0; t <= 2 * K; ++t) {
+      const float wt = wc[t];
+#pragma unroll
+      for (int j = 0; j < kRunLen; ++j) part[j] += wt * win[j + MP - K + t];
+    }
+  }
+}
+
+template <int MP>
+__global__ void __launch_bounds__(256)
+    k_nr_runs(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
+              const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, int ncols,
+              const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
+              double* __restrict__ results) {
+  constexpr int WIN = kRunLen + 2 * MP;
+  const int64_t nrun = *total;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t first = (int64_t)blockIdx.x * blockDim.x; first < nrun; first += stride) {
+    const int64_t it = first + threadIdx.x;
+    const bool live = it < nrun;
+    const int32_t r = list[live ? it : first];
+    const int zr = r % nzr;
+    const int64_t xy = r / nzr;
+    const int y = (int)(xy % OL.ny), x = (int)(xy / OL.ny);
+    const int z0 = zr * kRunLen;
+    float acc[kRunLen], part[kRunLen];
+#pragma unroll
+    for (int j = 0; j < kRunLen; ++j) acc[j] = part[j] = 0.0f;
+    int cur_dx = cols[0].dx;
+    for (int c = 0; c < ncols; ++c) {
+      const NrColumn col = cols[c];
+      if (col.dx != cur_dx) {
+#pragma unroll
+        for (int j = 0; j < kRunLen; ++j) {
+          acc[j] += part[j];
+          part[j] = 0.0f;
+        }
+        cur_dx = col.dx;
+      }
+      const float4* src = reinterpret_cast<const float4*>(occ + OL.at(x + col.dx, y + col.dy, z0 - MP));
+      float win[WIN];
+#pragma unroll
+      for (int q = 0; q < WIN / 4; ++q) {
+        const float4 v = src[q];
+        win[4 * q] = v.x;
+        win[4 * q + 1] = v.y;
+        win[4 * q + 2] = v.z;
+        win[4 * q + 3] = v.w;
+      }
+      const float* wc = w + col.woff;
+      switch (-col.dz0) {
+        case 0: column_taps<0, MP>(wc, win, part); break;
+        case 1: column_taps<1, MP>(wc, win, part); break;
+        case 2: column_taps<2, MP>(wc, win, part); break;
+        case 3: column_taps<3, MP>(wc, win, part); break;
+        case 4: column_taps<4, MP>(wc, win, part); break;
+        case 5: column_taps<5, MP>(wc, win, part); break;
+        case 6: column_taps<6, MP>(wc, win, part); break;
+        case 7: column_taps<7, MP>(wc, win, part); break;
+        case 8: column_taps<8, MP>(wc, win, part); break;
+        case 9: column_taps<9, MP>(wc, win, part); break;
+        case 10: column_taps<10, MP>(wc, win, part); break;
+        case 11: column_taps<11, MP>(wc, win, part); break;
+        case 12: column_taps<12, MP>(wc, win, part); break;
+        case 13: column_taps<13, MP>(wc, win, part); break;
+        case 14: column_taps<14, MP>(wc, win, part); break;
+        case 15: column_taps<15, MP>(wc, win, part); break;
+        case 16: column_taps<16, MP>(wc, win, part); break;
+        default: break;
+      }
+    }
+    if (!live) continue;
+    const int64_t base = xy * OL.nz + z0;
+#pragma unroll
+    for (int j = 0; j < kRunLen; ++j)
+      if (z0 + j < OL.nz) out[base + j] = ((acc[j] + part[j]) * scale) * negbv + 1.0f;
+  }
+}
+
+// ---------------------------------------------------------------- tile-mode summation
 // Row pitch of an LDS plane: >= 8+2m and = 8 (mod 32), so the 4 y-rows of a
 // half-wave fall in disjoint bank groups.
 __host__ __device__ inline int nr_pitch(int m) {
@@ -53,7 +174,7 @@ __host__ __device__ inline int nr_pitch(int m) {
 }
 
 __global__ void __launch_bounds__(256)
-    k_nr_planes(const float* __restrict__ occ, Dims d, const int32_t* __restrict__ list,
+    k_nr_planes(const float* __restrict__ occ, OccLayout OL, const int32_t* __restrict__ list,
                 const int32_t* __restrict__ count, int nty, int ntz, int m, const NrColumn* __restrict__ cols,
                 const int* __restrict__ col_start, const float* __restrict__ w, float scale, float negbv,
                 float* __restrict__ out, double* __restrict__ results) {
@@ -76,12 +197,12 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
     for (int X = x0 - m; X <= x0 + kTX - 1 + m; ++X) {
       float* buf = planes + (X & 1) * P;
-      const bool xin = X >= 0 && X < d.nx;
+      const bool xin = X >= 0 && X < OL.nx;
       for (int q = threadIdx.x; q < PY * span; q += 256) {
         int a = q / span, c = q - a * span;
         int gy = y0 + a, gz = z0 + c;
         float v = 0.0f;
-        if (xin && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz) v = occ[((int64_t)X * d.ny + gy) * d.nz + gz];
+        if (xin && gy >= 0 && gy < OL.ny && gz >= 0 && gz < OL.nz) v = occ[OL.at(X, gy, gz)];
         buf[a * PZ + c] = v;
       }
       __syncthreads();
@@ -96,14 +217,15 @@ __global__ void __launch_bounds__(256)
 #pragma unroll 4
           for (int q = 0; q < col.nz; ++q) s += wc[q] * src[q];
           // Neumaier compensated accumulation of the column sums
-          const float t = acc + s;
-          comp += fabsf(acc) >= fabsf(s) ? (acc - t) + s : (s - t) + acc;
-          acc = t;
+          const float tsum = acc + s;
+          comp += fabsf(acc) >= fabsf(s) ? (acc - tsum) + s : (s - tsum) + acc;
+          acc = tsum;
         }
       }
     }
     const int gx = x0 + wave, gy = ty * kTY + ly, gz = tz * kTZ + lz;
-    if (gx < d.nx && gy < d.ny && gz < d.nz) out[((int64_t)gx * d.ny + gy) * d.nz + gz] = ((acc + comp) * scale) * negbv + 1.0f;
+    if (gx < OL.nx && gy < OL.ny && gz < OL.nz)
+      out[((int64_t)gx * OL.ny + gy) * OL.nz + gz] = ((acc + comp) * scale) * negbv + 1.0f;
   }
 }
 
@@ -113,13 +235,16 @@ using namespace svimx;
 
 extern "C" {
 
+int64_t svimx_occupancy_elems(int nx, int ny, int nz) { return OccLayout::of(nx, ny, nz).size(); }
+
 int svimx_nr_plan_create(const double* weights, int m, double h, double radius, int nx, int ny, int nz,
                          void** plan_out) {
   SVIMX_TRY_BEGIN
   SVIMX_REQUIRE(weights && plan_out && m >= 0 && h > 0 && radius > 0, SVIMX_EBADARG, "bad N_r plan arguments");
   SVIMX_REQUIRE(2 * m + 1 <= nx && 2 * m + 1 <= ny && 2 * m + 1 <= nz, SVIMX_EBADARG,
                 "ball kernel larger than the grid");
-  SVIMX_REQUIRE(sizeof(float) * 2 * (size_t)(kTY + 2 * m) * nr_pitch(m) <= 64 * 1024, SVIMX_EBADARG,
+  const bool run_mode = m <= kRunMaxM;
+  SVIMX_REQUIRE(run_mode || sizeof(float) * 2 * (size_t)(kTY + 2 * m) * nr_pitch(m) <= 64 * 1024, SVIMX_EBADARG,
                 "ball radius of %d cells exceeds the LDS plane limit (m <= 33)", m);
   const int W = 2 * m + 1;
   std::vector<NrColumn> cols;
@@ -137,7 +262,12 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
         }
       }
       if (first < 0) continue;
-      NrColumn col{b - m, first - m, last - first + 1, (int)wts.size()};
+      if (run_mode) {  // symmetric tap range dz = -k..k covering every nonzero tap
+        int k = std::max(m - first, last - m);
+        first = m - k;
+        last = m + k;
+      }
+      NrColumn col{a - m, b - m, first - m, last - first + 1, (int)wts.size(), 0};
       for (int c = first; c <= last; ++c) wts.push_back((float)weights[((int64_t)a * W + b) * W + c]);
       cols.push_back(col);
     }
@@ -148,31 +278,40 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   p->h = h;
   p->radius = radius;
   p->d = Dims{nx, ny, nz};
+  p->run_mode = run_mode ? 1 : 0;
   p->ntx = (nx + kTX - 1) / kTX;
   p->nty = (ny + kTY - 1) / kTY;
   p->ntz = (nz + kTZ - 1) / kTZ;
   p->ntiles = (int64_t)p->ntx * p->nty * p->ntz;
+  p->nzr = (nz + kRunLen - 1) / kRunLen;
+  p->nruns = (int64_t)nx * ny * p->nzr;
+  p->nmarks = run_mode ? p->nruns : p->ntiles;
+  p->nchunks = (p->nruns + kCompactChunk - 1) / kCompactChunk;
   p->ncols = (int)cols.size();
   p->nweights = (int)wts.size();
   p->scale = (float)(h * h * h);
   p->negbv = (float)(-1.0 / (4.0 / 3.0 * M_PI * radius * radius * radius));
-  hipError_t e = hipSuccess;
-  e = hipMalloc(&p->cols, sizeof(NrColumn) * (cols.size() + 1));
+  hipError_t e = hipMalloc(&p->cols, sizeof(NrColumn) * (cols.size() + 1));
   if (e == hipSuccess) e = hipMalloc(&p->col_start, sizeof(int) * col_start.size());
-  if (e == hipSuccess)
-    e = hipMemcpy(p->col_start, col_start.data(), sizeof(int) * col_start.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&p->weights, sizeof(float) * (wts.size() + 1));
-  if (e == hipSuccess) e = hipMalloc(&p->marks, p->ntiles);
-  if (e == hipSuccess) e = hipMalloc(&p->list, sizeof(int32_t) * p->ntiles);
+  if (e == hipSuccess) e = hipMalloc(&p->marks, p->nmarks);
+  if (e == hipSuccess) e = hipMalloc(&p->dil, run_mode ? p->nruns : 1);
+  if (e == hipSuccess) e = hipMalloc(&p->list, sizeof(int32_t) * p->nmarks);
   if (e == hipSuccess) e = hipMalloc(&p->count, sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&p->ccounts, sizeof(int64_t) * (p->nchunks + 2));
   if (e == hipSuccess && !cols.empty())
     e = hipMemcpy(p->cols, cols.data(), sizeof(NrColumn) * cols.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->col_start, col_start.data(), sizeof(int) * col_start.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess && !wts.empty())
     e = hipMemcpy(p->weights, wts.data(), sizeof(float) * wts.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(p->marks, 0, p->ntiles);
+  if (e == hipSuccess) e = hipMemset(p->marks, 0, p->nmarks);
+  if (e == hipSuccess) e = hipMemset(p->list, 0, sizeof(int32_t) * p->nmarks);
+  if (e == hipSuccess) e = hipMemset(p->ccounts, 0, sizeof(int64_t) * (p->nchunks + 2));
   if (e != hipSuccess) {
     set_error("N_r plan allocation failed: %s", hipGetErrorString(e));
-    hipFree(p->cols); hipFree(p->col_start); hipFree(p->weights); hipFree(p->marks); hipFree(p->list); hipFree(p->count);
+    hipFree(p->cols); hipFree(p->col_start); hipFree(p->weights); hipFree(p->marks); hipFree(p->dil);
+    hipFree(p->list); hipFree(p->count); hipFree(p->ccounts);
     delete p;
     return SVIMX_EDEVICE;
   }
@@ -188,8 +327,10 @@ int svimx_nr_plan_destroy(void* plan) {
   hipFree(p->col_start);
   hipFree(p->weights);
   hipFree(p->marks);
+  hipFree(p->dil);
   hipFree(p->list);
   hipFree(p->count);
+  hipFree(p->ccounts);
   delete p;
   return SVIMX_OK;
 }
@@ -197,8 +338,8 @@ int svimx_nr_plan_destroy(void* plan) {
 int svimx_nr_reset(void* stream, void* plan) {
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p, SVIMX_EBADARG, "null plan");
-  hipLaunchKernelGGL(k_nr_clear_marks, dim3(blocks_for(p->ntiles, 256)), dim3(256), 0, as_stream(stream), p->marks,
-                     p->ntiles);
+  hipLaunchKernelGGL(k_nr_clear_marks, dim3(blocks_for(p->nmarks, 256)), dim3(256), 0, as_stream(stream), p->marks,
+                     p->nmarks);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
 }
@@ -207,8 +348,8 @@ int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t 
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p, SVIMX_EBADARG, "null plan");
   if (n <= 0) return SVIMX_OK;
-  hipLaunchKernelGGL(k_nr_mark_nodes, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), p->marks, p->nty,
-                     p->ntz, p->d, nodes, n);
+  hipLaunchKernelGGL(k_nr_mark_nodes, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), p->marks,
+                     p->run_mode, p->nty, p->ntz, p->nzr, p->d, nodes, n);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
 }
@@ -218,6 +359,21 @@ int svimx_nr_select(void* stream, void* plan) {
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p, SVIMX_EBADARG, "null plan");
   hipStream_t st = as_stream(stream);
+  if (p->run_mode) {
+    hipLaunchKernelGGL(k_runs_dilate_y, dim3(blocks_for(p->nruns, 256)), dim3(256), 0, st, p->marks, p->dil,
+                       p->d.ny, p->nzr, p->nruns);
+    SVIMX_CHECK_LAUNCH();
+    RunNeed pred{p->dil, (int64_t)p->d.ny * p->nzr};
+    hipLaunchKernelGGL(k_compact_count<RunNeed>, dim3((unsigned)p->nchunks), dim3(kCompactThreads), 0, st, pred,
+                       p->nruns, p->ccounts);
+    SVIMX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, p->ccounts, p->nchunks);
+    SVIMX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_compact_scatter<RunNeed>, dim3((unsigned)p->nchunks), dim3(kCompactThreads), 0, st, pred,
+                       p->nruns, p->ccounts, p->list, p->nruns);
+    SVIMX_CHECK_LAUNCH();
+    return SVIMX_OK;
+  }
   hipLaunchKernelGGL(k_nr_zero_count, dim3(1), dim3(1), 0, st, p->count);
   SVIMX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_nr_compact_tiles, dim3(blocks_for(p->ntiles, 256)), dim3(256), 0, st, p->marks, p->ntiles,
@@ -231,12 +387,34 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
   SVIMX_TRY_BEGIN
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p && occupancy && nr_out, SVIMX_EBADARG, "null argument");
+  SVIMX_REQUIRE((reinterpret_cast<uintptr_t>(occupancy) & 15) == 0, SVIMX_EBADARG,
+                "occupancy must be 16-byte aligned");
   Workspace w;
   if (!carve_workspace(ws, ws_bytes, p->d.size(), w)) return SVIMX_EBADARG;
+  hipStream_t st = as_stream(stream);
+  const OccLayout OL = OccLayout::of(p->d.nx, p->d.ny, p->d.nz);
+  if (p->run_mode) {
+    const int64_t* total = p->ccounts + p->nchunks;
+    const int grid = (int)std::min<int64_t>((p->nruns + 255) / 256, 256 * 8);
+    const int mp = (p->m + 3) / 4 * 4;
+#define SVIMX_RUNS(MPV)                                                                                            \
+  hipLaunchKernelGGL(k_nr_runs<MPV>, dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, total, p->cols, \
+                     p->ncols, p->weights, p->scale, p->negbv, nr_out, w.results)
+    switch (mp) {
+      case 0:
+      case 4: SVIMX_RUNS(4); break;
+      case 8: SVIMX_RUNS(8); break;
+      case 12: SVIMX_RUNS(12); break;
+      default: SVIMX_RUNS(16); break;
+    }
+#undef SVIMX_RUNS
+    SVIMX_CHECK_LAUNCH();
+    return SVIMX_OK;
+  }
   const size_t lds = sizeof(float) * 2 * (size_t)(kTY + 2 * p->m) * nr_pitch(p->m);
   int grid = (int)std::min<int64_t>(p->ntiles, 256 * 8);
-  hipLaunchKernelGGL(k_nr_planes, dim3(grid), dim3(256), lds, as_stream(stream), occupancy, p->d, p->list, p->count,
-                     p->nty, p->ntz, p->m, p->cols, p->col_start, p->weights, p->scale, p->negbv, nr_out, w.results);
+  hipLaunchKernelGGL(k_nr_planes, dim3(grid), dim3(256), lds, st, occupancy, OL, p->list, p->count, p->nty, p->ntz,
+                     p->m, p->cols, p->col_start, p->weights, p->scale, p->negbv, nr_out, w.results);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
   SVIMX_TRY_END

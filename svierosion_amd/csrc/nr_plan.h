@@ -1,40 +1,57 @@
 // N_r plan: ball weights laid out per (dx, dy) column for the direct
-// convolution, plus the tile marks that select where N_r is evaluated.
+// convolution, plus the marks that select where N_r is evaluated.
+//
+// Two evaluation modes:
+//  * runs  (m <= kRunMaxM): the unit of work is a z-run of kRunLen nodes; the
+//    fractions pass marks the run holding each closest point's tricubic
+//    stencil base, N_r selection dilates the marks by the stencil (+0..3 in x
+//    and y) and compacts them on the device, and one lane sums one run.
+//  * tiles (larger balls): 4x8x8 tiles streamed through LDS x-plane by x-plane.
 #pragma once
 
 #include "common.h"
 
 namespace svimx {
 
-// N_r is evaluated on tiles of kTX x kTY x kTZ nodes: one workgroup per tile,
-// one wave (64 lanes = kTY x kTZ) per x-layer.
-constexpr int kTX = 4, kTY = 8, kTZ = 8;
+constexpr int kTX = 4, kTY = 8, kTZ = 8;   // tile mode: one wave per x-layer of 8x8 nodes
+constexpr int kRunLen = 8;                 // run mode: nodes per z-run
+constexpr int kRunMaxM = 16;               // largest ball half-width handled in run mode (= kOccPad)
 
 struct NrColumn {
-  int dy;         // column offset in y (dx is implied by the column group)
-  int dz0, nz;    // first dz and number of nonzero taps
+  int dx, dy;     // column offset
+  int dz0, nz;    // first dz and number of nonzero taps (tile mode); run mode uses k = -dz0
   int woff;       // offset into the weight array
+  int pad;
 };
 
 struct NrPlan {
   int m;
   double h, radius;
   Dims d;
+  int run_mode;
+  // tile mode
   int ntx, nty, ntz;
   int64_t ntiles;
+  // run mode
+  int nzr;            // runs per z-line
+  int64_t nruns;      // nx * ny * nzr
+  int64_t nmarks;     // entries of `marks` (ntiles or nruns)
   int ncols;
   int nweights;
   float scale;        // float32(h^3)
   float negbv;        // float32(-1 / (4/3 pi r^3))
   NrColumn* cols;     // device, grouped by dx
   int* col_start;     // device, 2m+2 entries: columns of dx = g - m are [col_start[g], col_start[g+1])
-  float* weights;     // device
-  uint8_t* marks;     // device, ntiles
-  int32_t* list;      // device, ntiles
-  int32_t* count;     // device, 1
+  float* weights;     // device; run mode: per column the taps dz = -k..k
+  uint8_t* marks;     // device, nmarks
+  uint8_t* dil;       // device, nruns (run mode: marks dilated in y)
+  int32_t* list;      // device, nmarks
+  int32_t* count;     // device, 1 (tile mode)
+  int64_t* ccounts;   // device, compaction chunk counts + total (run mode)
+  int64_t nchunks;
 };
 
-// Mark the tiles covering the tricubic stencil [base, base+3]^3 of a point.
+// Tile mode: mark the tiles covering the tricubic stencil [base, base+3]^3.
 __device__ __forceinline__ void mark_stencil_tiles(uint8_t* marks, int nty, int ntz, int bx, int by, int bz) {
   int tx0 = bx / kTX, tx1 = (bx + 3) / kTX;
   int ty0 = by / kTY, ty1 = (by + 3) / kTY;
@@ -42,6 +59,14 @@ __device__ __forceinline__ void mark_stencil_tiles(uint8_t* marks, int nty, int 
   for (int tx = tx0; tx <= tx1; ++tx)
     for (int ty = ty0; ty <= ty1; ++ty)
       for (int tz = tz0; tz <= tz1; ++tz) marks[((int64_t)tx * nty + ty) * ntz + tz] = 1;
+}
+
+// Run mode: mark the runs of the stencil base line (x = bx, y = by) that the
+// stencil's z-range [bz, bz+3] touches; selection dilates in x and y.
+__device__ __forceinline__ void mark_stencil_runs(uint8_t* marks, int ny, int nzr, int bx, int by, int bz) {
+  int64_t line = ((int64_t)bx * ny + by) * nzr;
+  marks[line + bz / kRunLen] = 1;
+  marks[line + (bz + 3) / kRunLen] = 1;
 }
 
 }  // namespace svimx

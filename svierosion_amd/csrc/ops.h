@@ -21,6 +21,31 @@ struct Dims {
   __host__ __device__ int64_t size() const { return (int64_t)nx * ny * nz; }
 };
 
+// Layout of the occupancy field f: the grid embedded with kOccPad zero cells in
+// front of every axis and a z pitch that is a multiple of 4 floats, so the
+// N_r summation reads aligned float4 windows and never leaves the array for
+// ball offsets up to kOccPad cells.
+constexpr int kOccPad = 16;
+
+struct OccLayout {
+  int nx, ny, nz;
+  int64_t pitch_y;  // ny + 2 kOccPad
+  int64_t pitch_z;  // multiple of 4, >= nz + 2 kOccPad + 8
+  __host__ __device__ static OccLayout of(int nx, int ny, int nz) {
+    OccLayout L;
+    L.nx = nx;
+    L.ny = ny;
+    L.nz = nz;
+    L.pitch_y = ny + 2 * kOccPad;
+    L.pitch_z = ((int64_t)nz + 2 * kOccPad + 8 + 3) / 4 * 4;
+    return L;
+  }
+  __host__ __device__ int64_t size() const { return ((int64_t)nx + 2 * kOccPad) * pitch_y * pitch_z; }
+  __host__ __device__ int64_t at(int x, int y, int z) const {
+    return ((int64_t)(x + kOccPad) * pitch_y + (y + kOccPad)) * pitch_z + (z + kOccPad);
+  }
+};
+
 __device__ __forceinline__ void unflat(const Dims& d, int64_t n, int& i, int& j, int& k) {
   int64_t syz = (int64_t)d.ny * d.nz;
   i = (int)(n / syz);

@@ -110,24 +110,29 @@ static Hi3 hi_of(const Dims& d) {
 // ---------------------------------------------------------------- fractions
 __global__ void __launch_bounds__(kBandThreads)
     k_fractions(const float* __restrict__ phi, const int32_t* __restrict__ band, int64_t nb, Dims d, float h,
-                float* __restrict__ occ, uint8_t* __restrict__ marks, int nty, int ntz, float cut, Hi3 hi,
-                double* __restrict__ partials) {
+                float* __restrict__ occ, uint8_t* __restrict__ marks, int run_mode, int nty, int ntz, int nzr,
+                float cut, Hi3 hi, double* __restrict__ partials) {
   __shared__ double red[4];
+  const OccLayout OL = OccLayout::of(d.nx, d.ny, d.nz);
   double fsum = 0.0, neg = 0.0, pos = 0.0;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += (int64_t)gridDim.x * blockDim.x) {
     const int64_t n = band[q];
+    int i, j, k;
+    unflat(d, n, i, j, k);
     Grad g = central(phi, n, d);
     float fr = cell_fraction(g);
-    occ[n] = fr;
+    occ[OL.at(i, j, k)] = fr;
     fsum += fr;
     neg = g.c < 0.0f ? 1.0 : neg;
     pos = g.c > 0.0f ? 1.0 : pos;
     if (marks && fabsf(g.c) < cut) {
-      int i, j, k;
-      unflat(d, n, i, j, k);
       float p[3];
       closest_point(g, i, j, k, h, hi.v, p);
-      mark_stencil_tiles(marks, nty, ntz, (int)floorf(p[0]) - 1, (int)floorf(p[1]) - 1, (int)floorf(p[2]) - 1);
+      const int bx = (int)floorf(p[0]) - 1, by = (int)floorf(p[1]) - 1, bz = (int)floorf(p[2]) - 1;
+      if (run_mode)
+        mark_stencil_runs(marks, d.ny, nzr, bx, by, bz);
+      else
+        mark_stencil_tiles(marks, nty, ntz, bx, by, bz);
     }
   }
   double s = block_reduce<0>(fsum, red);
@@ -359,21 +364,23 @@ int svimx_fractions(void* stream, const float* phi, const int32_t* band, int64_t
   hipStream_t st = as_stream(stream);
   NrPlan* plan = static_cast<NrPlan*>(nr_plan);
   uint8_t* marks = nullptr;
-  int nty = 0, ntz = 0;
+  int nty = 0, ntz = 0, nzr = 0, run_mode = 0;
   if (plan) {
     SVIMX_REQUIRE(plan->d.nx == nx && plan->d.ny == ny && plan->d.nz == nz, SVIMX_EBADARG,
                   "N_r plan grid does not match");
-    hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(plan->ntiles, 256)), dim3(256), 0, st, plan->marks,
-                       plan->ntiles, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(plan->nmarks, 256)), dim3(256), 0, st, plan->marks,
+                       plan->nmarks, (uint8_t)0);
     SVIMX_CHECK_LAUNCH();
     marks = plan->marks;
     nty = plan->nty;
     ntz = plan->ntz;
+    nzr = plan->nzr;
+    run_mode = plan->run_mode;
   }
   int nblk = reduce_grid(nb);
   if (nb > 0) {
     hipLaunchKernelGGL(k_fractions, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
-                       occupancy, marks, nty, ntz, (float)(cut_hi * h), hi_of(d), w.partials);
+                       occupancy, marks, run_mode, nty, ntz, nzr, (float)(cut_hi * h), hi_of(d), w.partials);
     SVIMX_CHECK_LAUNCH();
   }
   ReduceSpec spec;

@@ -6,9 +6,10 @@ Device layout (per job, in HBM):
   the other two are the TVD-RK3 stage outputs.  Outside the band every copy
   holds the same clamped value +-width*h, so any copy can serve as a stencil
   input; ``rebuild_band`` re-synchronises them.
-* ``occupancy`` -- the dense occupancy f (1 inside, 0 outside, the tangent-plane
-  cell fraction at band nodes), the input of the N_r summation.  Only band
-  entries change between rebuilds.
+* ``occupancy`` -- the occupancy f (1 inside, 0 outside, the tangent-plane
+  cell fraction at band nodes), the input of the N_r summation, stored in the
+  zero-padded layout of ``svimx_occupancy_elems``.  Only band entries change
+  between rebuilds.
 * the band -- a sorted int32 list of flat indices (capacity = grid size).
 * a workspace with the reduction partials and the result slots.
 
@@ -50,7 +51,7 @@ class LevelSet:
         first = phi.to(self.device, torch.float32).contiguous().reshape(-1).clone()
         self.bufs = [first, torch.empty_like(first), torch.empty_like(first)]
         self.cur = 0
-        self.occupancy = torch.zeros(n, dtype=torch.float32, device=self.device)
+        self.occupancy = _abi.occupancy_zeros(self.dims, self.device)
         self.ws = _abi.workspace(self.dims, self.device)
         self._band = torch.empty(n, dtype=torch.int32, device=self.device)
         self.nb = 0
@@ -143,7 +144,7 @@ class LevelSet:
     def dense_fractions(self) -> torch.Tensor:
         """Dense occupancy (1 inside, 0 outside, cell fractions in the band)."""
         self.fractions()
-        return self.occupancy.view(self.dims).clone()
+        return _abi.occupancy_view(self.occupancy, self.dims).clone()
 
     def volume(self) -> float:
         self.fractions()

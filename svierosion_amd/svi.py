@@ -105,7 +105,8 @@ class BallConvolution:
         With ``nodes`` (flat indices) only the tiles holding them are evaluated and
         all other entries are NaN; without, every node is evaluated.
         """
-        f = fractions.to(self.device, torch.float32).contiguous()
+        f = _abi.occupancy_zeros(self.grid.shape, self.device)
+        _abi.occupancy_view(f, self.grid.shape).copy_(fractions.to(self.device, torch.float32).view(self.grid.shape))
         out = torch.full(self.grid.shape, float("nan"), dtype=torch.float32, device=self.device)
         st = _abi.stream()
         _abi.check(_abi.lib.svimx_nr_reset(st, self.plan))

@@ -155,15 +155,22 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
 /* ---------------------------------------------------------------- reinitialisation
  * Min (2010) PDE reinitialisation on the band dilated by `dilation` cells minus
  * `margin` boundary layers: `iterations` TVD-RK2 pseudo-time iterations with
- * ENO2 + subcell fix, in place on `phi`; `scratch` is a second dense field
- * (overwritten).  work_idx/geo hold work_cap nodes (geo: 8 floats per node);
- * mask_a/mask_b are dense byte fields.  Synchronises (work-set size in
- * *n_work).  Replaces LevelSet.reinitialize (svierosion/levelset.py:154-183)
- * minus the final rebuild_band, which the caller runs next. */
-int svimx_reinit(void* stream, float* phi, float* scratch, const int32_t* band, int64_t nb, int nx,
-                 int ny, int nz, double h, int iterations, double cfl, int dilation, int margin,
-                 uint8_t* mask_a, uint8_t* mask_b, int32_t* work_idx, float* geo, int64_t work_cap,
-                 void* ws, int64_t ws_bytes, int64_t* n_work);
+ * ENO2 + subcell fix, in place on `phi`, then the band rebuild of
+ * svimx_band_rebuild restricted to the work set (the only nodes that changed):
+ * clamp, copy into `scratch` and `phi_copy` (the other two stage fields),
+ * reset `occupancy`, and write the new sorted band over `band` (capacity
+ * nx*ny*nz; on entry it holds the nb nodes of the current band).
+ * `inside_outside` is the count of inside nodes outside the current band.
+ * work_idx/work_info hold work_cap nodes, geo 8*work_cap floats; mask_a/mask_b
+ * are dense byte fields.  Synchronises.  info (host, 10 int64): new band
+ * count, inside nodes outside the new band, margin ok, band min i/j/k, max
+ * i/j/k, work-set size.  Replaces LevelSet.reinitialize
+ * (svierosion/levelset.py:154-183). */
+int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, float* occupancy, int32_t* band,
+                 int64_t nb, int64_t inside_outside, int nx, int ny, int nz, double h, double width,
+                 int iterations, double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b,
+                 int32_t* work_idx, int32_t* work_info, float* geo, int64_t work_cap, void* ws, int64_t ws_bytes,
+                 int64_t* info);
 
 /* ---------------------------------------------------------------- mesh -> signed distance
  * Dense phi = +-min(d, band)*h, negative inside, for a triangle mesh already in

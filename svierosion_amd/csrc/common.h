@@ -58,6 +58,7 @@ inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) /
 //   compaction counts     one int64 per compaction chunk (+ scan)
 struct Workspace {
   double* results;
+  long long* counters;  // 256 integer counters (header bytes 2048..4095)
   double* partials;
   int64_t max_partial_blocks;
   int64_t* chunk_counts;
@@ -67,6 +68,7 @@ struct Workspace {
 constexpr int kBandThreads = 256;       // threads per block of band-list kernels
 constexpr int kPartialSlots = 16;       // doubles written per block by reduction kernels
 constexpr int kMaxReduceBlocks = 2048;  // grid cap of reducing kernels (grid-stride loops)
+constexpr int kXcds = 8;                // MI355X: workgroups go round-robin over 8 XCDs
 
 // Grid of a reducing kernel over n items: grid-stride with at most kMaxReduceBlocks
 // blocks, so the finalisation reads a bounded number of partials.

@@ -28,10 +28,19 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact_count(Pred pred, in
 // Exclusive scan of `n` counts in place into offs[0..n), total into offs[n].
 __global__ void __launch_bounds__(1024) k_scan_counts(int64_t* counts, int64_t n);
 
-template <class Pred>
+// Output value of a selected position: the position itself, or map[position].
+struct IdentityMap {
+  __device__ int32_t operator()(int64_t e) const { return (int32_t)e; }
+};
+struct ListMap {
+  const int32_t* list;
+  __device__ int32_t operator()(int64_t e) const { return list[e]; }
+};
+
+template <class Pred, class Map = IdentityMap>
 __global__ void __launch_bounds__(kCompactThreads) k_compact_scatter(Pred pred, int64_t n,
                                                                      const int64_t* offs,
-                                                                     int32_t* out, int64_t cap) {
+                                                                     int32_t* out, int64_t cap, Map map = Map()) {
   __shared__ int wave_tot[kCompactThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
@@ -52,16 +61,16 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact_scatter(Pred pred, 
     }
     if (f) {
       int64_t o = pos + woff + before;
-      if (o < cap) out[o] = (int32_t)e;
+      if (o < cap) out[o] = map(e);
     }
     pos += tot;
   }
 }
 
 // Runs the three passes; synchronises and returns the selected count.
-template <class Pred>
+template <class Pred, class Map = IdentityMap>
 int compact(hipStream_t st, Pred pred, int64_t n, int32_t* out, int64_t cap, Workspace& ws,
-            int64_t* count) {
+            int64_t* count, Map map = Map()) {
   int64_t chunks = (n + kCompactChunk - 1) / kCompactChunk;
   SVIMX_REQUIRE(chunks + 1 <= ws.max_chunks, SVIMX_EBADARG, "workspace too small for compaction");
   if (chunks == 0) {
@@ -73,8 +82,8 @@ int compact(hipStream_t st, Pred pred, int64_t n, int32_t* out, int64_t cap, Wor
   SVIMX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, ws.chunk_counts, chunks);
   SVIMX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_compact_scatter<Pred>, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st,
-                     pred, n, ws.chunk_counts, out, cap);
+  hipLaunchKernelGGL((k_compact_scatter<Pred, Map>), dim3((unsigned)chunks), dim3(kCompactThreads), 0, st,
+                     pred, n, ws.chunk_counts, out, cap, map);
   SVIMX_CHECK_LAUNCH();
   int64_t total = 0;
   SVIMX_CHECK_HIP(hipMemcpyAsync(&total, ws.chunk_counts + chunks, sizeof(int64_t),

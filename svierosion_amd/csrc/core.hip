@@ -36,6 +36,7 @@ bool carve_workspace(void* ws, int64_t ws_bytes, int64_t n_nodes, Workspace& out
   }
   char* p = static_cast<char*>(ws);
   out.results = reinterpret_cast<double*>(p);
+  out.counters = reinterpret_cast<long long*>(p + 2048);
   out.max_partial_blocks = (n_nodes + kBandThreads - 1) / kBandThreads + 1;
   out.partials = reinterpret_cast<double*>(p + kHeaderBytes);
   out.chunk_counts = reinterpret_cast<int64_t*>(p + kHeaderBytes + out.max_partial_blocks * kPartialSlots *

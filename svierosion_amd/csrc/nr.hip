@@ -103,7 +103,11 @@ __global__ void __launch_bounds__(256)
   const int64_t nrun = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t first = (int64_t)blockIdx.x * blockDim.x; first < nrun; first += stride) {
+  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+  // L2); give XCD x a contiguous slice of the run list so that neighbouring
+  // runs, which read the same occupancy columns, share one L2.
+  const int lb = (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
+  for (int64_t first = (int64_t)lb * blockDim.x; first < nrun; first += stride) {
     const int64_t it = first + threadIdx.x;
     const bool live = it < nrun;
     const int32_t r = list[live ? it : first];
@@ -369,8 +373,8 @@ int svimx_nr_select(void* stream, void* plan) {
     SVIMX_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, p->ccounts, p->nchunks);
     SVIMX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_compact_scatter<RunNeed>, dim3((unsigned)p->nchunks), dim3(kCompactThreads), 0, st, pred,
-                       p->nruns, p->ccounts, p->list, p->nruns);
+    hipLaunchKernelGGL((k_compact_scatter<RunNeed, IdentityMap>), dim3((unsigned)p->nchunks), dim3(kCompactThreads),
+                       0, st, pred, p->nruns, p->ccounts, p->list, p->nruns, IdentityMap());
     SVIMX_CHECK_LAUNCH();
     return SVIMX_OK;
   }
@@ -395,7 +399,8 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
   const OccLayout OL = OccLayout::of(p->d.nx, p->d.ny, p->d.nz);
   if (p->run_mode) {
     const int64_t* total = p->ccounts + p->nchunks;
-    const int grid = (int)std::min<int64_t>((p->nruns + 255) / 256, 256 * 8);
+    // grid: a multiple of the XCD count (the kernel's block remapping needs it)
+    const int grid = (int)std::min<int64_t>((p->nruns + 255) / 256 + kXcds - 1, 256 * 8) / kXcds * kXcds;
     const int mp = (p->m + 3) / 4 * 4;
 #define SVIMX_RUNS(MPV)                                                                                            \
   hipLaunchKernelGGL(k_nr_runs<MPV>, dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, total, p->cols, \

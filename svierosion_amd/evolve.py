@@ -305,6 +305,12 @@ class ErosionSimulation:
             did_reinit = False
             reached = target is not None and vnew <= target * (1 + tol)
             if self._displacement >= self.reinit_distance * ls.h and not reached:
+                if self.k1 is not None:
+                    # the dense curvature fields must read 0 off the band, like the
+                    # reference's freshly zeroed array (svierosion/evolve.py:138)
+                    old = ls.band.long()
+                    self.k1.index_fill_(0, old, 0.0)
+                    self.k2.index_fill_(0, old, 0.0)
                 with self._span("reinit"):
                     ls.reinitialize(self.reinit_iterations)
                 self._alloc_band_arrays()

@@ -172,24 +172,29 @@ class LevelSet:
         n = self.grid.size
         if self._reinit_bufs is None:
             self._reinit_bufs = (torch.empty(n, dtype=torch.uint8, device=self.device),
-                                 torch.empty(n, dtype=torch.uint8, device=self.device),
-                                 torch.empty(n, dtype=torch.int32, device=self.device))
-        ma, mb, widx = self._reinit_bufs
-        geo = torch.empty(8 * max(self.nb * 4, 1024), dtype=torch.float32, device=self.device)
-        scratch = self.bufs[(self.cur + 1) % 3]
-        nwork = C.c_int64()
-        rc = _abi.lib.svimx_reinit(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(scratch), _abi.ptr(self._band),
-                                   self.nb, *self.dims, self.h, int(iterations), float(cfl), int(dilation), MARGIN,
-                                   _abi.ptr(ma), _abi.ptr(mb), _abi.ptr(widx), _abi.ptr(geo), geo.numel() // 8,
-                                   _abi.ptr(self.ws), self.ws.numel(), C.byref(nwork))
-        if rc == _abi.ECAPACITY:
-            geo = torch.empty(8 * int(nwork.value), dtype=torch.float32, device=self.device)
-            rc = _abi.lib.svimx_reinit(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(scratch), _abi.ptr(self._band),
-                                       self.nb, *self.dims, self.h, int(iterations), float(cfl), int(dilation),
-                                       MARGIN, _abi.ptr(ma), _abi.ptr(mb), _abi.ptr(widx), _abi.ptr(geo),
-                                       geo.numel() // 8, _abi.ptr(self.ws), self.ws.numel(), C.byref(nwork))
+                                 torch.empty(n, dtype=torch.uint8, device=self.device))
+        ma, mb = self._reinit_bufs
+        others = [b for i, b in enumerate(self.bufs) if i != self.cur]
+        info = (C.c_int64 * 10)()
+        cap = min(n, 2 * self.nb + 4096)
+        for _ in range(2):
+            widx = torch.empty(cap, dtype=torch.int32, device=self.device)
+            winfo = torch.empty(cap, dtype=torch.int32, device=self.device)
+            geo = torch.empty(8 * cap, dtype=torch.float32, device=self.device)
+            rc = _abi.lib.svimx_reinit(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(others[0]), _abi.ptr(others[1]),
+                                       _abi.ptr(self.occupancy), _abi.ptr(self._band), self.nb,
+                                       self.inside_outside, *self.dims, self.h, self.width, int(iterations),
+                                       float(cfl), int(dilation), MARGIN, _abi.ptr(ma), _abi.ptr(mb),
+                                       _abi.ptr(widx), _abi.ptr(winfo), _abi.ptr(geo), cap, _abi.ptr(self.ws),
+                                       self.ws.numel(), info)
+            if rc != _abi.ECAPACITY:
+                break
+            cap = int(info[9])  # the work set did not fit: retry with its exact size
+        if rc in (_abi.OK, _abi.EMARGIN):
+            self.nb = int(info[0])
+            self.inside_outside = int(info[1])
+        self.last_work_nodes = int(info[9])
         _abi.check(rc, "reinitialisation")
-        self.rebuild_band()
 
 
 # ------------------------------------------------------------- interpolation

@@ -164,9 +164,16 @@ def main(argv=None):
     taps = float((sim.conv.weights > 0).sum())
     flops = 2.0 * taps * n_out
     achieved = alg_bytes / (nr_ms * 1e-3) / 1e9 if nr_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_nr_runs_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        traffic = t["read_bytes_per_dispatch"] + t["write_bytes_per_dispatch"]
+    kname = "k_nr_runs" if sim.conv is not None and sim.conv.m <= 16 else "k_nr_planes"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_nr_planes", "avg_ms": round(nr_ms, 4),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)" if traffic else None,
+                "kernel": kname, "avg_ms": round(nr_ms, 4),
                 "alg_bytes_per_launch": alg_bytes,
                 "valu_tflops": round(flops / (nr_ms * 1e-3) / 1e12, 2) if nr_ms > 0 else 0.0,
                 "valu_frac_packed_peak": round(flops / (nr_ms * 1e-3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)

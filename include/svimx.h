@@ -196,6 +196,19 @@ int svimx_sdf_build(void* stream, const float* tri_local, const int32_t* tri_org
                     double exact_band, float* phi_out, float* dist_out, int32_t* tid_out, void* ws,
                     int64_t ws_bytes);
 
+/* ---------------------------------------------------------------- surface
+ * Zero set of phi (band cells) as a watertight, outward triangle mesh by
+ * marching tetrahedra on the Kuhn split, vertices shared per edge and moved
+ * by one capped Newton step (max_shift cells, 0 = none) onto the zero set of
+ * the tricubic interpolant.  Outputs are device arrays the library allocates:
+ * *xyz (3 float32 index coordinates per vertex) and *faces (3 int32 per
+ * triangle); release them with svimx_free.  Synchronises.  Replaces
+ * extract_surface (svierosion/surface.py:16-56). */
+int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny, int nz,
+                          double max_shift, float** xyz, int64_t* nv, int32_t** faces, int64_t* nf);
+int svimx_free(void* device_ptr);
+int svimx_copy_to_host(void* dst, const void* device_src, int64_t bytes);
+
 /* ---------------------------------------------------------------- operators
  * Node-list forms of the reference's @kernel functions (svierosion/kernels.py),
  * sharing the device code of the fused stage kernels; used by LevelSet methods

@@ -57,7 +57,11 @@ _SIGNATURES = {
                           _P, _I64, _P, _I64, _P]),
     "svimx_sdf_build": (_I, [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _P, _I64, _I, _I, _I, _I, _I, _D, _D, _D,
                              _P, _P, _P, _P, _I64]),
-    "svimx_op_godunov":(_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
+    "svimx_extract_surface": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    "svimx_free": (_I, [_P]),
+    "svimx_copy_to_host": (_I, [_P, _P, _I64]),
+    "svimx_op_godunov": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
     "svimx_op_cell_fractions": (_I, [_P, _P, _P, _I64, _I, _I, _I, _P]),
     "svimx_op_closest_points": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
     "svimx_op_tricubic": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P]),

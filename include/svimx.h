@@ -235,6 +235,108 @@ int svimx_op_reinit_geometry(void* stream, const float* phi, const int32_t* idx,
 int svimx_op_point_triangle_distance(void* stream, const float* p, const float* tri, int64_t n,
                                      float* out);                             /* :382-427 */
 
+/* ---------------------------------------------------------------- batch engine
+ * Many erosion jobs of different grids and laws stepped together: every
+ * engine tick advances each running job by one accepted TVD-RK3 step, with
+ * one launch per kernel and RK stage for all jobs (a device job table maps
+ * each virtual block to a job and its own block index), dt computed on the
+ * device from each job's statistics, and one host synchronisation per tick
+ * (plus the rare secant retries and reinitialisations, which run per job
+ * set).  A job's arithmetic, block decomposition and reduction order are the
+ * same as in the single-job entry points above, so its history and field do
+ * not depend on which other jobs share the ticks.  The tick mirrors
+ * ErosionSimulation.run (svierosion/evolve.py:182-250): stop tests, stable dt
+ * (:162-170), RK3 (:172-179), regula falsi onto the target volume (:215-231),
+ * reinitialisation every reinit_distance*h of displacement (:237-242).
+ * The caller keeps a job's buffers alive until svimx_engine_remove; the
+ * engine owns the band-sized velocity / L0 arrays and the reinit scratch. */
+typedef struct svimx_job_desc {
+  float* phi[3];         /* the three dense field buffers; phi[cur] holds phi */
+  float* occupancy;      /* padded occupancy (svimx_occupancy_elems) */
+  int32_t* band;         /* band list, capacity band_cap */
+  int64_t band_cap;
+  void* ws;              /* workspace (svimx_workspace_bytes) */
+  int64_t ws_bytes;
+  int32_t nx, ny, nz;
+  int32_t cur;           /* buffer index of phi */
+  int64_t nb;            /* band size */
+  int64_t inside_outside;
+  double h, h3;          /* spacing and its cube (volume factor) */
+  double width;          /* band half-width in cells */
+  int32_t margin;        /* band margin in cells */
+  int32_t frac_valid;    /* buffer whose fractions / N_r marks are in occupancy, -1 none */
+  svimx_law_t law;
+  void* nr_plan;         /* SVI laws: the N_r plan */
+  float* field;          /* SVI laws: dense N_r */
+  float* k1;             /* curvature laws: dense principal curvatures */
+  float* k2;
+  double radius;         /* SVI ball radius (feedback CFL) */
+  double cfl, feedback_cfl, parabolic_cfl, reinit_distance;
+  double cutoff_lo, cutoff_hi;  /* velocity cutoff in cells */
+  int32_t reinit_iterations;
+  int32_t reserved;
+  double time, initial_volume, displacement;
+  int64_t steps;
+} svimx_job_desc_t;
+
+typedef struct svimx_step_record {  /* StepRecord, svierosion/evolve.py:42-56 */
+  int64_t step;
+  double time, dt, volume, fraction, vmax, vmean, qmin, qmax;
+  int64_t capped;
+  int32_t reinit;
+  int32_t reserved;
+  double wall;
+} svimx_step_record_t;
+
+enum {
+  SVIMX_STOP_NONE = 0,
+  SVIMX_STOP_TARGET = 1,    /* "target_volume" */
+  SVIMX_STOP_MAX_TIME = 2,  /* "max_time" */
+  SVIMX_STOP_VANISHED = 3,  /* "vanished" */
+  SVIMX_STOP_STALLED = 4,   /* "stalled" */
+  SVIMX_STOP_MAX_STEPS = 5, /* "max_steps" */
+  SVIMX_STOP_ERROR = 6      /* a kernel or reinitialisation failed: see error / message */
+};
+
+typedef struct svimx_job_status {
+  int32_t running;       /* 1 while the job takes part in ticks */
+  int32_t stop_reason;   /* SVIMX_STOP_* of the last run */
+  int32_t error;         /* SVIMX_E* code when stop_reason is SVIMX_STOP_ERROR */
+  int32_t cur, frac_valid;
+  int32_t reserved;
+  int64_t nb, inside_outside, steps, nrecords;
+  double time, volume, displacement;
+  char message[256];
+} svimx_job_status_t;
+
+#define SVIMX_R_DT 31 /* result slot: dt of the last tick (engine jobs) */
+
+/* stream NULL: the engine creates its own (needed for graph capture).  The
+ * caller synchronises the streams that prepared a job before adding it;
+ * every tick and run ends with the engine's stream synchronised. */
+int svimx_engine_create(void* stream, int max_jobs, void** engine);
+int svimx_engine_destroy(void* engine);
+/* Register a prepared job (level set built, fractions state as in desc). */
+int svimx_engine_add(void* engine, const svimx_job_desc_t* desc, int* job_id);
+/* Start (or continue) job_id until its volume reaches target_fraction of
+ * initial_volume (<= 0: no target), time reaches max_time (<= 0: none) or
+ * max_steps more steps were taken.  Computes the current volume; synchronises. */
+int svimx_engine_run(void* engine, int job_id, double target_fraction, double max_time, int64_t max_steps,
+                     double tol);
+/* One step of every running job; *n_running = jobs still running after it. */
+int svimx_engine_tick(void* engine, int* n_running);
+int svimx_engine_status(void* engine, int job_id, svimx_job_status_t* out);
+/* Copy step records [first, first + n) of job_id (records accumulate over runs). */
+int svimx_engine_records(void* engine, int job_id, int64_t first, int64_t n, svimx_step_record_t* out);
+/* Forget job_id (its buffers are the caller's again); frees the engine-owned arrays. */
+int svimx_engine_remove(void* engine, int job_id);
+/* Capture the main tick as a HIP graph and replay it (1) or launch eagerly (0). */
+int svimx_engine_use_graphs(void* engine, int on);
+/* Ids (up to cap) of jobs that stopped since the last call, oldest first. */
+int svimx_engine_stopped(void* engine, int* ids, int cap, int* n);
+/* Milliseconds spent in ticks / secant retries / reinitialisations so far. */
+int svimx_engine_timers(void* engine, double* out3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,38 @@ class Law(C.Structure):
 
 _P, _I, _I64, _D, _U8P = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_void_p
 
+
+class JobDesc(C.Structure):
+    """svimx_job_desc_t."""
+    _fields_ = [("phi", C.c_void_p * 3), ("occupancy", C.c_void_p), ("band", C.c_void_p), ("band_cap", C.c_int64),
+                ("ws", C.c_void_p), ("ws_bytes", C.c_int64), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("nz", C.c_int32), ("cur", C.c_int32), ("nb", C.c_int64), ("inside_outside", C.c_int64),
+                ("h", C.c_double), ("h3", C.c_double), ("width", C.c_double), ("margin", C.c_int32),
+                ("frac_valid", C.c_int32), ("law", Law), ("nr_plan", C.c_void_p), ("field", C.c_void_p),
+                ("k1", C.c_void_p), ("k2", C.c_void_p), ("radius", C.c_double), ("cfl", C.c_double),
+                ("feedback_cfl", C.c_double), ("parabolic_cfl", C.c_double), ("reinit_distance", C.c_double),
+                ("cutoff_lo", C.c_double), ("cutoff_hi", C.c_double), ("reinit_iterations", C.c_int32),
+                ("reserved", C.c_int32), ("time", C.c_double), ("initial_volume", C.c_double),
+                ("displacement", C.c_double), ("steps", C.c_int64)]
+
+
+class StepRec(C.Structure):
+    """svimx_step_record_t."""
+    _fields_ = [("step", C.c_int64), ("time", C.c_double), ("dt", C.c_double), ("volume", C.c_double),
+                ("fraction", C.c_double), ("vmax", C.c_double), ("vmean", C.c_double), ("qmin", C.c_double),
+                ("qmax", C.c_double), ("capped", C.c_int64), ("reinit", C.c_int32), ("reserved", C.c_int32),
+                ("wall", C.c_double)]
+
+
+class JobStatus(C.Structure):
+    """svimx_job_status_t."""
+    _fields_ = [("running", C.c_int32), ("stop_reason", C.c_int32), ("error", C.c_int32), ("cur", C.c_int32),
+                ("frac_valid", C.c_int32), ("reserved", C.c_int32), ("nb", C.c_int64),
+                ("inside_outside", C.c_int64), ("steps", C.c_int64), ("nrecords", C.c_int64),
+                ("time", C.c_double), ("volume", C.c_double), ("displacement", C.c_double),
+                ("message", C.c_char * 256)]
+
+
 _SIGNATURES = {
     "svimx_last_error": (C.c_char_p, []),
     "svimx_version": (_I, []),
@@ -61,6 +93,17 @@ _SIGNATURES = {
                                    C.POINTER(C.c_int64), C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "svimx_free": (_I, [_P]),
     "svimx_copy_to_host": (_I, [_P, _P, _I64]),
+    "svimx_engine_create": (_I, [_P, _I, C.POINTER(_P)]),
+    "svimx_engine_destroy": (_I, [_P]),
+    "svimx_engine_add": (_I, [_P, C.POINTER(JobDesc), C.POINTER(C.c_int)]),
+    "svimx_engine_run": (_I, [_P, _I, _D, _D, _I64, _D]),
+    "svimx_engine_tick": (_I, [_P, C.POINTER(C.c_int)]),
+    "svimx_engine_status": (_I, [_P, _I, C.POINTER(JobStatus)]),
+    "svimx_engine_records": (_I, [_P, _I, _I64, _I64, C.POINTER(StepRec)]),
+    "svimx_engine_remove": (_I, [_P, _I]),
+    "svimx_engine_use_graphs": (_I, [_P, _I]),
+    "svimx_engine_stopped": (_I, [_P, C.POINTER(C.c_int), _I, C.POINTER(C.c_int)]),
+    "svimx_engine_timers": (_I, [_P, C.POINTER(_D)]),
     "svimx_op_godunov": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
     "svimx_op_cell_fractions": (_I, [_P, _P, _P, _I64, _I, _I, _I, _P]),
     "svimx_op_closest_points": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),

@@ -224,9 +224,28 @@ def initial_max_nr(sim: ErosionSimulation) -> float:
     return float(tricubic(N.reshape(-1), ls.dims, cp).max())
 
 
-def erode(mesh: TriMesh, params: ErosionParams, callback=None, verbose: bool = False,
-          keep_state: bool = False) -> ErosionResult:
-    """Erode ``mesh`` under ``params.law`` until ``params.target_fraction`` of its volume remains."""
+@dataclass
+class ErosionJob:
+    """One erosion between set-up and result: the pieces of erode() shared with
+    the batched and served drivers (svierosion_amd.batch)."""
+
+    mesh: TriMesh
+    params: ErosionParams
+    law: SpeedLaw
+    grid: Grid
+    ls: LevelSet
+    sim: ErosionSimulation
+    targets: list
+    timings: dict = field(default_factory=dict)
+    snaps: list = field(default_factory=list)
+    next_target: int = 0
+    stop_reason: str = ""
+    fraction: float = 1.0
+    tmesh: float = 0.0
+
+
+def prepare_job(mesh: TriMesh, params: ErosionParams, verbose: bool = False) -> ErosionJob:
+    """Grid, mesh SDF, level set, simulation and the t > max N_r check (svierosion/api.py:197-228)."""
     dev = _abi.require_device(get_device(params.device))
     topo = mesh.edge_topology()
     if topo["boundary_edges"] or topo["nonmanifold_edges"]:
@@ -240,7 +259,7 @@ def erode(mesh: TriMesh, params: ErosionParams, callback=None, verbose: bool = F
     grid = make_grid(mesh, params)
     phi = mesh_to_sdf(mesh, grid, band=params.band, device=dev, verbose=verbose)
     ls = LevelSet(phi, grid, width=params.band)
-    torch.cuda.synchronize(dev)
+    torch.cuda.current_stream(dev).synchronize()
     timings["sdf"] = time.time() - t0
     if verbose:
         print(f"[erode] {grid.describe()}, device {dev}; law: {law.describe()}")
@@ -256,28 +275,47 @@ def erode(mesh: TriMesh, params: ErosionParams, callback=None, verbose: bool = F
             print(f"[erode] max N_r on initial surface = {nmax:.4f} (t = {law.t})")
     targets = sorted({float(f) for f in params.snapshots if f > params.target_fraction},
                      reverse=True) + [params.target_fraction]
-    snaps = []
-    t1 = time.time()
-    tmesh = 0.0
-    res = None
-    for frac in targets:
-        res = sim.run(target_fraction=frac, max_steps=params.max_steps, callback=callback)
-        tm = time.time()
-        m = extract_surface(ls, project=params.project_vertices)
-        _annotate(sim, m, mesh, law)
-        tmesh += time.time() - tm
-        snaps.append((res.fraction, sim.time, m))
-        if res.stop_reason != "target_volume":
-            break
-    timings["evolve"] = time.time() - t1 - tmesh
-    timings["mesh"] = tmesh
-    final = snaps[-1][2]
-    out = ErosionResult(final, snaps[:-1], sim.history, grid, res.stop_reason, sim.time, res.fraction,
-                        sim.initial_volume, mesh.volume(), timings,
-                        {**asdict(params), "law_description": law.describe(), **law.params()}, mesh)
+    return ErosionJob(mesh, params, law, grid, ls, sim, targets, timings)
+
+
+def finish_target(job: ErosionJob, stop_reason: str, fraction: float) -> bool:
+    """Mesh and annotate the state reached for the current target; True if the
+    job goes on to the next target (svierosion/api.py:230-245)."""
+    tm = time.time()
+    m = extract_surface(job.ls, project=job.params.project_vertices)
+    _annotate(job.sim, m, job.mesh, job.law)
+    job.tmesh += time.time() - tm
+    job.snaps.append((fraction, job.sim.time, m))
+    job.stop_reason = stop_reason
+    job.fraction = fraction
+    job.next_target += 1
+    return stop_reason == "target_volume" and job.next_target < len(job.targets)
+
+
+def job_result(job: ErosionJob, evolve_seconds: float, keep_state: bool = False) -> ErosionResult:
+    job.timings["evolve"] = evolve_seconds
+    job.timings["mesh"] = job.tmesh
+    final = job.snaps[-1][2]
+    p, law = job.params, job.law
+    out = ErosionResult(final, job.snaps[:-1], job.sim.history, job.grid, job.stop_reason, job.sim.time,
+                        job.fraction, job.sim.initial_volume, job.mesh.volume(), job.timings,
+                        {**asdict(p), "law_description": law.describe(), **law.params()}, job.mesh)
     if keep_state:
-        out.level_set, out.simulation = ls, sim
+        out.level_set, out.simulation = job.ls, job.sim
     return out
+
+
+def erode(mesh: TriMesh, params: ErosionParams, callback=None, verbose: bool = False,
+          keep_state: bool = False) -> ErosionResult:
+    """Erode ``mesh`` under ``params.law`` until ``params.target_fraction`` of its volume remains."""
+    job = prepare_job(mesh, params, verbose)
+    t1 = time.time()
+    while True:
+        res = job.sim.run(target_fraction=job.targets[job.next_target], max_steps=params.max_steps,
+                          callback=callback)
+        if not finish_target(job, res.stop_reason, res.fraction):
+            break
+    return job_result(job, time.time() - t1 - job.tmesh, keep_state)
 
 
 def volume_fraction_report(res: ErosionResult) -> str:
@@ -287,5 +325,5 @@ def volume_fraction_report(res: ErosionResult) -> str:
             f"level-set volume ratio {res.fraction:.5f}")
 
 
-__all__ = ["ErosionParams", "ErosionResult", "erode", "make_grid", "point_mesh_distance", "volume_fraction_report",
+__all__ = ["ErosionParams", "ErosionResult", "ErosionJob", "erode", "prepare_job", "finish_target", "job_result", "make_grid", "point_mesh_distance", "volume_fraction_report",
            "StepRecord"]

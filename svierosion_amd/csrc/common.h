@@ -72,7 +72,7 @@ constexpr int kXcds = 8;                // MI355X: workgroups go round-robin ove
 
 // Grid of a reducing kernel over n items: grid-stride with at most kMaxReduceBlocks
 // blocks, so the finalisation reads a bounded number of partials.
-inline int reduce_grid(int64_t n) {
+__host__ __device__ inline int reduce_grid(int64_t n) {
   int64_t b = (n + kBandThreads - 1) / kBandThreads;
   return (int)(b < 1 ? 1 : (b > kMaxReduceBlocks ? kMaxReduceBlocks : b));
 }
@@ -131,5 +131,24 @@ struct ReduceSpec {
 };
 int launch_finalize(hipStream_t st, const double* partials, int64_t nblocks, double* results,
                     const ReduceSpec& spec);
+
+// Body of the finalisation (one block of 256 threads): slot by slot, thread t
+// folds partial blocks t, t+256, ... and the block reduction combines them.
+__device__ __forceinline__ void finalize_body(const double* __restrict__ partials, int64_t nblocks,
+                                              double* __restrict__ results, const ReduceSpec& spec) {
+  __shared__ double red[4];
+  for (int s = 0; s < kPartialSlots; ++s) {
+    int op = spec.op[s];
+    if (op < 0) continue;
+    double v = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
+    for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
+      double u = partials[b * kPartialSlots + s];
+      v = op == 0 ? v + u : (op == 1 ? fmax(v, u) : fmin(v, u));
+    }
+    double r = op == 0 ? block_reduce<0>(v, red) : (op == 1 ? block_reduce<1>(v, red) : block_reduce<2>(v, red));
+    if (threadIdx.x == 0) results[spec.dst[s]] = r;
+    __syncthreads();
+  }
+}
 
 }  // namespace svimx

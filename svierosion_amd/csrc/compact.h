@@ -9,20 +9,55 @@
 
 namespace svimx {
 
-// Pred: __device__ bool operator()(int64_t node) const.
+// Pred: __device__ bool operator()(int64_t node) const.  Chunk c of the
+// count pass; the kernels below run chunk blockIdx.x, the batch engine runs
+// its jobs' chunks through the same bodies.
 template <class Pred>
-__global__ void __launch_bounds__(kCompactThreads) k_compact_count(Pred pred, int64_t n,
-                                                                   int64_t* chunk_counts) {
+__device__ __forceinline__ void compact_count_body(const Pred& pred, int64_t n, int64_t* chunk_counts, int64_t c) {
   __shared__ double red[4];
-  const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
-  int64_t c = 0;
+  const int64_t base = c * kCompactChunk;
+  int64_t cnt = 0;
 #pragma unroll 4
   for (int r = 0; r < kCompactPerThread; ++r) {
     int64_t e = base + (int64_t)r * kCompactThreads + threadIdx.x;
-    if (e < n && pred(e)) ++c;
+    if (e < n && pred(e)) ++cnt;
   }
-  double tot = block_reduce<0>((double)c, red);
-  if (threadIdx.x == 0) chunk_counts[blockIdx.x] = (int64_t)tot;
+  double tot = block_reduce<0>((double)cnt, red);
+  if (threadIdx.x == 0) chunk_counts[c] = (int64_t)tot;
+}
+
+template <class Pred>
+__global__ void __launch_bounds__(kCompactThreads) k_compact_count(Pred pred, int64_t n,
+                                                                   int64_t* chunk_counts) {
+  compact_count_body(pred, n, chunk_counts, blockIdx.x);
+}
+
+// Exclusive scan of n counts in place into counts[0..n), total into
+// counts[n]; one block of 1024 threads, thread t owning a contiguous range.
+__device__ __forceinline__ void scan_counts_body(int64_t* counts, int64_t n) {
+  __shared__ int64_t sums[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t lo = t * per < n ? t * per : n;
+  const int64_t hi = lo + per < n ? lo + per : n;
+  int64_t s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += counts[i];
+  sums[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int64_t v = t >= o ? sums[t - o] : 0;
+    __syncthreads();
+    sums[t] += v;
+    __syncthreads();
+  }
+  int64_t run = sums[t] - s;  // exclusive prefix of this thread's range
+  for (int64_t i = lo; i < hi; ++i) {
+    int64_t c = counts[i];
+    counts[i] = run;
+    run += c;
+  }
+  if (t == 1023) counts[n] = sums[1023];
+  __syncthreads();
 }
 
 // Exclusive scan of `n` counts in place into offs[0..n), total into offs[n].
@@ -37,14 +72,13 @@ struct ListMap {
   __device__ int32_t operator()(int64_t e) const { return list[e]; }
 };
 
-template <class Pred, class Map = IdentityMap>
-__global__ void __launch_bounds__(kCompactThreads) k_compact_scatter(Pred pred, int64_t n,
-                                                                     const int64_t* offs,
-                                                                     int32_t* out, int64_t cap, Map map = Map()) {
+template <class Pred, class Map>
+__device__ __forceinline__ void compact_scatter_body(const Pred& pred, int64_t n, const int64_t* offs, int32_t* out,
+                                                     int64_t cap, const Map& map, int64_t c) {
   __shared__ int wave_tot[kCompactThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
-  int64_t pos = offs[blockIdx.x];
+  const int64_t base = c * kCompactChunk;
+  int64_t pos = offs[c];
   for (int r = 0; r < kCompactPerThread; ++r) {
     int64_t e = base + (int64_t)r * kCompactThreads + threadIdx.x;
     bool f = e < n && pred(e);
@@ -65,6 +99,14 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact_scatter(Pred pred, 
     }
     pos += tot;
   }
+  __syncthreads();
+}
+
+template <class Pred, class Map = IdentityMap>
+__global__ void __launch_bounds__(kCompactThreads) k_compact_scatter(Pred pred, int64_t n,
+                                                                     const int64_t* offs,
+                                                                     int32_t* out, int64_t cap, Map map = Map()) {
+  compact_scatter_body(pred, n, offs, out, cap, map, blockIdx.x);
 }
 
 // Runs the three passes; synchronises and returns the selected count.

@@ -48,19 +48,7 @@ bool carve_workspace(void* ws, int64_t ws_bytes, int64_t n_nodes, Workspace& out
 // ---------------------------------------------------------------- reductions
 __global__ void __launch_bounds__(256) k_finalize(const double* __restrict__ partials, int64_t nblocks,
                                                   double* __restrict__ results, ReduceSpec spec) {
-  __shared__ double red[4];
-  for (int s = 0; s < kPartialSlots; ++s) {
-    int op = spec.op[s];
-    if (op < 0) continue;
-    double v = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
-    for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
-      double u = partials[b * kPartialSlots + s];
-      v = op == 0 ? v + u : (op == 1 ? fmax(v, u) : fmin(v, u));
-    }
-    double r = op == 0 ? block_reduce<0>(v, red) : (op == 1 ? block_reduce<1>(v, red) : block_reduce<2>(v, red));
-    if (threadIdx.x == 0) results[spec.dst[s]] = r;
-    __syncthreads();
-  }
+  finalize_body(partials, nblocks, results, spec);
 }
 
 int launch_finalize(hipStream_t st, const double* partials, int64_t nblocks, double* results,
@@ -70,29 +58,7 @@ int launch_finalize(hipStream_t st, const double* partials, int64_t nblocks, dou
   return SVIMX_OK;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_counts(int64_t* counts, int64_t n) {
-  __shared__ int64_t sums[1024];
-  const int t = threadIdx.x;
-  const int64_t per = (n + 1023) / 1024;
-  const int64_t lo = t * per, hi = lo + per < n ? lo + per : n;
-  int64_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += counts[i];
-  sums[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int64_t v = t >= o ? sums[t - o] : 0;
-    __syncthreads();
-    sums[t] += v;
-    __syncthreads();
-  }
-  int64_t run = sums[t] - s;  // exclusive prefix of this thread's range
-  for (int64_t i = lo; i < hi; ++i) {
-    int64_t c = counts[i];
-    counts[i] = run;
-    run += c;
-  }
-  if (t == 1023) counts[n] = sums[1023];
-}
+__global__ void __launch_bounds__(1024) k_scan_counts(int64_t* counts, int64_t n) { scan_counts_body(counts, n); }
 
 // ---------------------------------------------------------------- band rebuild
 struct BandPred {

@@ -26,6 +26,7 @@
 
 #include "common.h"
 #include "compact.h"
+#include "nr_dev.h"
 #include "nr_plan.h"
 
 namespace svimx {
@@ -56,117 +57,24 @@ __global__ void k_nr_mark_nodes(uint8_t* marks, int run_mode, int nty, int ntz, 
     marks[((int64_t)(i / kTX) * nty + j / kTY) * ntz + k / kTZ] = 1;
 }
 
-// dil(x, y, zr) = OR_{b=0..3} marks(x, y-b, zr)
 __global__ void k_runs_dilate_y(const uint8_t* __restrict__ marks, uint8_t* __restrict__ dil, int ny, int nzr,
                                 int64_t nruns) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nruns) return;
-  const int y = (int)((r / nzr) % ny);
-  uint8_t v = 0;
-  for (int b = 0; b <= 3 && b <= y; ++b) v |= marks[r - (int64_t)b * nzr];
-  dil[r] = v;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nruns) dilate_y_run(marks, dil, ny, nzr, r);
 }
-
-// need(x, y, zr) = OR_{a=0..3} dil(x-a, y, zr)
-struct RunNeed {
-  const uint8_t* dil;
-  int64_t plane;  // ny * nzr
-  __device__ bool operator()(int64_t r) const {
-    const int64_t x = r / plane;
-    uint8_t v = 0;
-    for (int a = 0; a <= 3 && a <= x; ++a) v |= dil[r - a * plane];
-    return v != 0;
-  }
-};
 
 // ---------------------------------------------------------------- run-mode summation
-template <int K, int MP, int WIN>
-__device__ __forceinline__ void column_taps(const float* __restrict__ wc, const float (&win)[WIN],
-                                            float (&part)[kRunLen]) {
-  if constexpr (K <= MP) {
-#pragma unroll
-    for (int t = 0; t <= 2 * K; ++t) {
-      const float wt = wc[t];
-#pragma unroll
-      for (int j = 0; j < kRunLen; ++j) part[j] += wt * win[j + MP - K + t];
-    }
-  }
-}
-
 template <int MP>
 __global__ void __launch_bounds__(256)
     k_nr_runs(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
               const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, int ncols,
               const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
               double* __restrict__ results) {
-  constexpr int WIN = kRunLen + 2 * MP;
   const int64_t nrun = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-  // L2); give XCD x a contiguous slice of the run list so that neighbouring
-  // runs, which read the same occupancy columns, share one L2.
-  const int lb = (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
-  for (int64_t first = (int64_t)lb * blockDim.x; first < nrun; first += stride) {
-    const int64_t it = first + threadIdx.x;
-    const bool live = it < nrun;
-    const int32_t r = list[live ? it : first];
-    const int zr = r % nzr;
-    const int64_t xy = r / nzr;
-    const int y = (int)(xy % OL.ny), x = (int)(xy / OL.ny);
-    const int z0 = zr * kRunLen;
-    float acc[kRunLen], part[kRunLen];
-#pragma unroll
-    for (int j = 0; j < kRunLen; ++j) acc[j] = part[j] = 0.0f;
-    int cur_dx = cols[0].dx;
-    for (int c = 0; c < ncols; ++c) {
-      const NrColumn col = cols[c];
-      if (col.dx != cur_dx) {
-#pragma unroll
-        for (int j = 0; j < kRunLen; ++j) {
-          acc[j] += part[j];
-          part[j] = 0.0f;
-        }
-        cur_dx = col.dx;
-      }
-      const float4* src = reinterpret_cast<const float4*>(occ + OL.at(x + col.dx, y + col.dy, z0 - MP));
-      float win[WIN];
-#pragma unroll
-      for (int q = 0; q < WIN / 4; ++q) {
-        const float4 v = src[q];
-        win[4 * q] = v.x;
-        win[4 * q + 1] = v.y;
-        win[4 * q + 2] = v.z;
-        win[4 * q + 3] = v.w;
-      }
-      const float* wc = w + col.woff;
-      switch (-col.dz0) {
-        case 0: column_taps<0, MP>(wc, win, part); break;
-        case 1: column_taps<1, MP>(wc, win, part); break;
-        case 2: column_taps<2, MP>(wc, win, part); break;
-        case 3: column_taps<3, MP>(wc, win, part); break;
-        case 4: column_taps<4, MP>(wc, win, part); break;
-        case 5: column_taps<5, MP>(wc, win, part); break;
-        case 6: column_taps<6, MP>(wc, win, part); break;
-        case 7: column_taps<7, MP>(wc, win, part); break;
-        case 8: column_taps<8, MP>(wc, win, part); break;
-        case 9: column_taps<9, MP>(wc, win, part); break;
-        case 10: column_taps<10, MP>(wc, win, part); break;
-        case 11: column_taps<11, MP>(wc, win, part); break;
-        case 12: column_taps<12, MP>(wc, win, part); break;
-        case 13: column_taps<13, MP>(wc, win, part); break;
-        case 14: column_taps<14, MP>(wc, win, part); break;
-        case 15: column_taps<15, MP>(wc, win, part); break;
-        case 16: column_taps<16, MP>(wc, win, part); break;
-        default: break;
-      }
-    }
-    if (!live) continue;
-    const int64_t base = xy * OL.nz + z0;
-#pragma unroll
-    for (int j = 0; j < kRunLen; ++j)
-      if (z0 + j < OL.nz) out[base + j] = ((acc[j] + part[j]) * scale) * negbv + 1.0f;
-  }
+  // neighbouring runs read the same occupancy columns: keep them on one XCD's L2
+  nr_runs_body<MP>(occ, OL, nzr, list, nrun, cols, ncols, w, scale, negbv, out, xcd_slice(blockIdx.x, gridDim.x),
+                   gridDim.x);
 }
 
 // ---------------------------------------------------------------- tile-mode summation

@@ -3,8 +3,8 @@
 Each RHS evaluation (one TVD-RK3 stage) is four libsvimx launches over the
 band plus the N_r summation over the marked tiles:
 
-    fractions (+ volume partials, interface flags, N_r tile marks)
-    N_r direct summation      (SVI laws)   | curvature scatter (curvature laws)
+    fractions (+ N_r tile marks)   (SVI laws)
+    N_r direct summation           (SVI laws)   | curvature scatter (curvature laws)
     velocity (+ dt statistics on stage 0)
     RK stage (Godunov/WENO5, L = V |grad phi|, stage combination)
 
@@ -148,7 +148,9 @@ class ErosionSimulation:
         saved = ls.cur
         ls.cur = buf
         try:
-            if self._frac_valid != buf:
+            # the occupancy feeds N_r only; curvature laws need the fractions
+            # just for the volume at the end of the step
+            if self.conv is not None and self._frac_valid != buf:
                 with self._span("fractions"):
                     ls.fractions(self._plan(), self.cutoff[1])
                 self._frac_valid = buf

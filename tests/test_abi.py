@@ -34,3 +34,32 @@ def test_error_mapping_without_gpu():
     rc = _abi.lib.svimx_fractions(None, None, None, 0, 10, 10, 10, 0.1, None, None, 5.5, None, 0)
     assert rc == _abi.EBADARG
     assert "workspace" in _abi.lib.svimx_last_error().decode()
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """ctypes mirrors of the header's structs have the C compiler's offsets and sizes."""
+    import shutil
+    import subprocess
+    from svierosion_amd import _abi
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        import pytest
+        pytest.skip("no host C compiler")
+    structs = {"svimx_law_t": _abi.Law, "svimx_job_desc_t": _abi.JobDesc, "svimx_step_record_t": _abi.StepRec,
+               "svimx_job_status_t": _abi.JobStatus}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "svimx.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

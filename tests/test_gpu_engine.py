@@ -1,0 +1,122 @@
+"""Batch engine: every job's history and field are bit-identical to the
+single-job path (ErosionSimulation.run, svierosion/evolve.py:182-250), alone
+and in a mix of jobs with other grids and laws, with and without HIP graphs."""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    from svierosion_amd.api import ErosionParams
+    from svierosion_amd.shapes import cube, icosphere
+    if name == "svi_cube":
+        return cube(1.0), ErosionParams(law="svi", radius=0.2, k=1, s=1, t=0.9, resolution=40), 0.93, 400
+    if name == "svi_sphere_k2":
+        return icosphere(3, 0.8), ErosionParams(law="svi", radius=0.25, k=2, s=0, resolution=36), 0.95, 400
+    if name == "mean":
+        return icosphere(3, 0.8), ErosionParams(law="mean", alpha=1.0, resolution=32), 0.96, 400
+    if name == "gauss":
+        return cube(1.0), ErosionParams(law="gauss", alpha=0.25, resolution=24), 0.97, 40
+    raise KeyError(name)
+
+
+_PHI = {}
+
+
+def _sim(name):
+    """A fresh simulation of case `name` (the signed distance is computed once and cloned)."""
+    from svierosion_amd.api import make_grid
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.levelset import LevelSet
+    from svierosion_amd.sdf import mesh_to_sdf
+    mesh, p, target, max_steps = _case(name)
+    grid = make_grid(mesh, p)
+    if name not in _PHI:
+        _PHI[name] = mesh_to_sdf(mesh, grid, band=p.band)
+    lo, hi = mesh.bounds
+    law = p.make_law(length_scale=float((hi - lo).max()))
+    ls = LevelSet(_PHI[name].clone(), grid, width=p.band)
+    sim = ErosionSimulation(ls, law, cfl=p.cfl, feedback_cfl=p.feedback_cfl, reinit_distance=p.reinit_distance,
+                            reinit_iterations=p.reinit_iterations)
+    return sim, target, max_steps
+
+
+def _same(a, b):
+    return (math.isnan(a) and math.isnan(b)) or a == b
+
+
+def _assert_same_run(sa, sb):
+    import torch
+    assert len(sa.history) == len(sb.history) > 0
+    for ra, rb in zip(sa.history, sb.history):
+        for f in ("step", "time", "dt", "volume", "fraction", "vmax", "vmean", "qmin", "qmax", "capped", "reinit"):
+            assert _same(getattr(ra, f), getattr(rb, f)), (ra.step, f, getattr(ra, f), getattr(rb, f))
+    la, lb = sa.ls, sb.ls
+    assert la.nb == lb.nb and la.inside_outside == lb.inside_outside
+    assert torch.equal(la.band, lb.band)
+    assert torch.equal(la.flat, lb.flat)
+    assert sa.time == sb.time and sa.steps == sb.steps
+
+
+def _engine_run(names, graphs):
+    from svierosion_amd.engine import BatchEngine
+    eng = BatchEngine(max_jobs=8, graphs=graphs)
+    jobs = {}
+    for nm in names:
+        sim, target, max_steps = _sim(nm)
+        j = eng.add(sim)
+        eng.run(j, target_fraction=target, max_steps=max_steps)
+        jobs[nm] = j
+    eng.run_all()
+    out = {}
+    for nm, j in jobs.items():
+        sim = eng.sync(j)
+        out[nm] = (sim, eng.stop_reason(j))
+    eng.close()
+    return out
+
+
+@pytest.mark.parametrize("name", ["svi_cube", "svi_sphere_k2", "mean", "gauss"])
+def test_engine_job_equals_single_job_path(device, name):
+    sim, target, max_steps = _sim(name)
+    r = sim.run(target_fraction=target, max_steps=max_steps)
+    got = _engine_run([name], graphs=False)[name]
+    assert got[1] == r.stop_reason
+    _assert_same_run(sim, got[0])
+    if name != "gauss":
+        assert r.stop_reason == "target_volume"
+        assert abs(sim.history[-1].fraction - target) < 5e-6  # the regula falsi ran
+    if name == "svi_cube":
+        assert any(rec.reinit for rec in sim.history)
+
+
+def test_mix_equals_alone_with_graphs(device):
+    names = ["svi_cube", "mean", "svi_sphere_k2", "gauss"]
+    mix = _engine_run(names, graphs=True)
+    for nm in names:
+        alone = _engine_run([nm], graphs=False)[nm]
+        assert mix[nm][1] == alone[1]
+        _assert_same_run(mix[nm][0], alone[0])
+
+
+def test_engine_rerun_with_lower_target_continues(device):
+    """Snapshots: a job stopped at one target continues to the next one."""
+    from svierosion_amd.engine import BatchEngine
+    sim_a, _, _ = _sim("svi_cube")
+    sim_a.run(target_fraction=0.97)
+    sim_a.run(target_fraction=0.94)
+    sim_b, _, _ = _sim("svi_cube")
+    eng = BatchEngine(max_jobs=2)
+    j = eng.add(sim_b)
+    for tf in (0.97, 0.94):
+        eng.run(j, target_fraction=tf)
+        eng.run_all()
+        assert eng.stop_reason(j) == "target_volume"
+    eng.sync(j)
+    eng.close()
+    _assert_same_run(sim_a, sim_b)
+    assert abs(sim_b.history[-1].fraction - 0.94) < 5e-6

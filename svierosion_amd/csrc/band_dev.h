@@ -49,7 +49,7 @@ __device__ __forceinline__ float svi_speed(const LawF& L, float N) {
   float V;
   if (L.s) {
     N = fminf(N, L.n_cap);
-    V = ipow(N, L.k) / (L.t - N);
+    V = ipow(N, L.k) * rcp(L.t - N);
   } else {
     V = L.k ? ipow(N, L.k) : 1.0f;
   }
@@ -61,24 +61,31 @@ __device__ __forceinline__ float svi_dspeed(const LawF& L, float N) {
   N = fminf(fmaxf(N, 1e-12f), 1.0f);
   if (L.s) {
     N = fminf(N, L.n_cap);
-    float d = L.t - N;
-    float V = ipow(N, L.k) / d;
-    return V * ((float)L.k / N + 1.0f / d);
+    const float id = rcp(L.t - N);
+    float V = ipow(N, L.k) * id;
+    return V * ((float)L.k * rcp(N) + id);
   }
   if (L.k == 0) return 0.0f;
   return (float)L.k * ipow(N, L.k - 1);
 }
 
+// x^e for x >= 0 as exp2(e log2 x) with the hardware log2 / exp2 (v_log_f32,
+// v_exp_f32); relative error ~1e-6 for the exponents and ranges used here,
+// x = 0 gives 0 (e > 0) or +inf (e < 0) as powf does.
+__device__ __forceinline__ float pow_pos(float x, float e) {
+  return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+}
+
 // x^alpha regularised for alpha < 1 and its derivative (svierosion/flows.py:139-158).
 __device__ __forceinline__ float curv_power(const LawF& L, float x) {
   if (L.alpha == 1.0f) return x;
-  if (L.alpha < 1.0f) return x * powf(x + L.delta, L.alpha_m1);
-  return powf(x, L.alpha);
+  if (L.alpha < 1.0f) return x * pow_pos(x + L.delta, L.alpha_m1);
+  return pow_pos(x, L.alpha);
 }
 __device__ __forceinline__ float curv_dpower(const LawF& L, float x) {
   if (L.alpha == 1.0f) return 1.0f;
-  if (L.alpha < 1.0f) return powf(x + L.delta, L.alpha_m2) * (L.alpha * x + L.delta);
-  return L.alpha * powf(fmaxf(x, 0.0f), L.alpha_m1);
+  if (L.alpha < 1.0f) return pow_pos(x + L.delta, L.alpha_m2) * (L.alpha * x + L.delta);
+  return L.alpha * pow_pos(fmaxf(x, 0.0f), L.alpha_m1);
 }
 
 // Speed and diffusivity dV/dk1 + dV/dk2 of the curvature laws (svierosion/flows.py:170-205).
@@ -99,7 +106,7 @@ __device__ __forceinline__ void curv_speed(const LawF& L, float k1, float k2, fl
 
 // Smoothstep cutoff (svierosion/evolve.py:108-112), a/b are float32(c*h).
 __device__ __forceinline__ float cutoff_factor(float x, float a, float b) {
-  float t = fminf(fmaxf((b - x) / (b - a), 0.0f), 1.0f);
+  float t = fminf(fmaxf((b - x) * rcp(b - a), 0.0f), 1.0f);
   return t * t * (3.0f - 2.0f * t);
 }
 
@@ -273,7 +280,7 @@ __device__ __forceinline__ void rk_node(int stage, const float* __restrict__ in,
   } else if (stage == 1) {
     out[n] = 0.75f * u0[n] + 0.25f * (in[n] + dt * L);
   } else {
-    out[n] = u0[n] / 3.0f + (2.0f / 3.0f) * (in[n] + dt * L);
+    out[n] = u0[n] * kThird + (2.0f / 3.0f) * (in[n] + dt * L);
   }
 }
 

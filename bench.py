@@ -107,6 +107,63 @@ def cpu_baseline(ls, law, budget_s=30.0):
                        f"oracle setup (FFT of the ball kernel) {t1 - t0:.1f}s excluded")}
 
 
+def _same_history(a, b) -> bool:
+    return len(a) == len(b) and all(x.volume == y.volume and x.dt == y.dt and x.reinit == y.reinit
+                                    for x, y in zip(a, b))
+
+
+def batched_leg(n_jobs: int, workers: int):
+    """The fragment mix (first n_jobs of the 64-fragment set) eroded batched and one at a time."""
+    from svierosion_amd.batch import Request, serve, serve_sequential
+    from svierosion_amd.workloads import fragment_mix
+    t0 = time.time()
+    jobs = fragment_mix(n_jobs)
+    reqs = [Request(m, p) for m, p in jobs]
+    gen_s = time.time() - t0
+    rb = serve(reqs, workers=workers)
+    print(f"[bench] batched {n_jobs} jobs: {rb.summary()}", file=sys.stderr, flush=True)
+    rs = serve_sequential(reqs)
+    print(f"[bench] sequential {n_jobs} jobs: {rs.summary()}", file=sys.stderr, flush=True)
+    same = all(cb.error is None and cs.error is None and _same_history(cb.result.history, cs.result.history)
+               for cb, cs in zip(rb.completions, rs.completions))
+    laws = {}
+    for _, p in jobs:
+        laws[p.law] = laws.get(p.law, 0) + 1
+    steps = [c.steps for c in rb.completions]
+    return {"workload": (f"first {n_jobs} of the 64-fragment mix (icosphere level 4-7 + SH displacement, "
+                         "res log-uniform 96-320, 50% SVI / 25% mean / 25% Gauss curvature, target 0.9)"),
+            "jobs": n_jobs, "laws": laws, "steps_min_max": [min(steps), max(steps)],
+            "erosions_per_hour": round(rb.erosions_per_hour(), 1),
+            "sequential_erosions_per_hour": round(rs.erosions_per_hour(), 1),
+            "speedup": round(rb.erosions_per_hour() / max(rs.erosions_per_hour(), 1e-9), 3),
+            "makespan_s": round(rb.makespan, 3), "sequential_makespan_s": round(rs.makespan, 3),
+            "identical_histories": bool(same), "peak_device_gb": round(rb.peak_device_bytes / 1e9, 3),
+            "engine_ticks": rb.ticks, "mesh_generation_s": round(gen_s, 2)}, rs
+
+
+def served_leg(n_req: int, rate: float, workers: int):
+    """Poisson arrivals of the served workload (first n_req requests), continuous batching vs FIFO."""
+    from svierosion_amd.batch import serve, serve_sequential
+    from svierosion_amd.workloads import poisson_requests
+    reqs = poisson_requests(n_req, rate=rate)
+    rv = serve(reqs, workers=workers)
+    print(f"[bench] served {n_req} requests at {rate:.4f}/s: {rv.summary()}", file=sys.stderr, flush=True)
+    rq = serve_sequential(reqs)
+    print(f"[bench] sequential FIFO: {rq.summary()}", file=sys.stderr, flush=True)
+    same = all(a.error is None and b.error is None and _same_history(a.result.history, b.result.history)
+               for a, b in zip(rv.completions, rq.completions))
+    return {"workload": (f"first {n_req} of 512 requests (fragments as in the mix, res log-uniform 64-400, "
+                         f"target U[0.85, 0.97]), seeded Poisson arrivals at {rate:.4f}/s"),
+            "requests": n_req, "rate_per_s": round(rate, 5),
+            "erosions_per_hour": round(rv.erosions_per_hour(), 1),
+            "latency_p50_s": round(rv.latency(0.5), 3), "latency_p99_s": round(rv.latency(0.99), 3),
+            "sequential_erosions_per_hour": round(rq.erosions_per_hour(), 1),
+            "sequential_latency_p50_s": round(rq.latency(0.5), 3),
+            "sequential_latency_p99_s": round(rq.latency(0.99), 3),
+            "speedup": round(rv.erosions_per_hour() / max(rq.erosions_per_hour(), 1e-9), 3),
+            "identical_histories": bool(same), "peak_device_gb": round(rv.peak_device_bytes / 1e9, 3)}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +174,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-stage HIP events")
+    ap.add_argument("--batch-jobs", type=int, default=12, help="fragment-mix jobs of the batched leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=12, help="requests of the served leg (0: skip)")
+    ap.add_argument("--serve-load", type=float, default=2.0,
+                    help="offered load of the served leg as a multiple of the measured sequential throughput")
+    ap.add_argument("--workers", type=int, default=2, help="set-up / meshing threads of the batched forms")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of this many MiB (profiler byte calibration)")
     args = ap.parse_args(argv)
@@ -182,6 +244,16 @@ def main(argv=None):
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(ls, law)
+    del sim, ls
+    torch.cuda.empty_cache()
+    batched = served = None
+    seq_rate = None
+    if args.batch_jobs > 0:
+        batched, rs = batched_leg(args.batch_jobs, args.workers)
+        seq_rate = len(rs.done) / rs.makespan if rs.makespan > 0 else None
+    if args.serve_requests > 0:
+        rate = args.serve_load * (seq_rate if seq_rate else 0.5)
+        served = served_leg(args.serve_requests, rate, args.workers)
     line = {"metric": "RK3 steps/s + HBM GB/s @res400 1.3M-tri; batched erosions/hour, p99 latency",
             "value": round(value, 3), "unit": "RK3 steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
             "ms_per_step": round(1000 * elapsed / max(steps, 1), 3), "higher_is_better": True, "scaling": "weak",
@@ -190,7 +262,7 @@ def main(argv=None):
                        "triangles": info["triangles"], "grid": info["grid"], "band_nodes": info["band_nodes"],
                        "reinit_in_window": reinits, "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
                        "launches": launches},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "served": served}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -7,9 +7,9 @@ the batch engine (svierosion_amd.engine):
 * admission -- a request is admitted when it has arrived, an engine slot is
   free and its estimated device memory (:func:`job_device_bytes`) fits the
   HBM budget next to the jobs already resident;
-* set-up and finishing -- a worker thread, on a stream of its own, builds
+* set-up and finishing -- worker threads, each on a stream of its own, build
   each admitted job (mesh SDF, level set, N_r plan: ``prepare_job``) and
-  meshes and annotates each job that reached a target (``finish_target``),
+  mesh and annotate each job that reached a target (``finish_target``),
   while the main thread keeps ticking the engine;
 * stepping -- every tick advances all running jobs by one RK3 step; jobs join
   and leave between ticks, so new requests start while others are running.
@@ -117,10 +117,10 @@ class ServeReport:
 
 
 class _Worker:
-    """One host thread with its own torch stream for set-up and meshing."""
+    """Host threads, each with its own torch stream, for set-up and meshing."""
 
-    def __init__(self):
-        self.pool = cf.ThreadPoolExecutor(1, thread_name_prefix="svimx-worker")
+    def __init__(self, threads: int = 2):
+        self.pool = cf.ThreadPoolExecutor(max(1, int(threads)), thread_name_prefix="svimx-worker")
         self._local = threading.local()
 
     def _stream(self):
@@ -145,7 +145,7 @@ class _Worker:
 
 
 def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, graphs: bool = True,
-          keep_state: bool = False, verbose: bool = False) -> ServeReport:
+          keep_state: bool = False, verbose: bool = False, workers: int = 2) -> ServeReport:
     """Continuous batching of ``requests`` (arrival times in seconds from now)."""
     import torch
     from .engine import STOP_NAMES, BatchEngine
@@ -156,7 +156,7 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
     order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
     pending = deque(order)
     eng = BatchEngine(max_jobs=max_jobs, graphs=graphs)
-    worker = _Worker()
+    worker = _Worker(workers)
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0  # noqa: E731
     comps: dict[int, Completion] = {}
@@ -286,11 +286,11 @@ def serve_sequential(requests: list, keep_state: bool = False, verbose: bool = F
 
 
 def erode_batch(meshes: list, params, max_jobs: int = 64, graphs: bool = True, keep_state: bool = False,
-                hbm_budget: float | None = None) -> list:
+                hbm_budget: float | None = None, workers: int = 2) -> list:
     """``erode`` of every mesh (one ErosionParams or one per mesh), batched on one GPU.
 
     Returns ErosionResult per mesh, or the exception a job raised."""
     plist = params if isinstance(params, (list, tuple)) else [params] * len(meshes)
     rep = serve([Request(m, p) for m, p in zip(meshes, plist)], max_jobs=max_jobs, hbm_budget=hbm_budget,
-                graphs=graphs, keep_state=keep_state)
+                graphs=graphs, keep_state=keep_state, workers=workers)
     return [c.result if c.error is None else RuntimeError(c.error) for c in rep.completions]

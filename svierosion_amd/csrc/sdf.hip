@@ -263,11 +263,17 @@ __global__ void k_sdf_finish(const float* __restrict__ dist, const uint8_t* __re
 
 template <class T>
 struct DevBuf {
+  // stream-ordered temporaries: no device-wide synchronisation on free, so
+  // other jobs' kernels on other streams keep running
   T* p = nullptr;
+  hipStream_t st = nullptr;
   ~DevBuf() {
-    if (p) hipFree(p);
+    if (p) hipFreeAsync(p, st);
   }
-  hipError_t alloc(int64_t n) { return hipMalloc(&p, sizeof(T) * (n > 0 ? n : 1)); }
+  hipError_t alloc(int64_t n, hipStream_t s) {
+    st = s;
+    return hipMallocAsync(reinterpret_cast<void**>(&p), sizeof(T) * (n > 0 ? n : 1), s);
+  }
 };
 
 }  // namespace svimx
@@ -296,22 +302,22 @@ int svimx_sdf_build(void* stream, const float* tri_local, const int32_t* tri_org
   DevBuf<float> dist_own, nd;
   DevBuf<int32_t> tid_own, list, prev, nt, acc, changed;
   DevBuf<uint8_t> known, ring;
-  SVIMX_CHECK_HIP(keys.alloc(n));
-  SVIMX_CHECK_HIP(known.alloc(n));
-  SVIMX_CHECK_HIP(ring.alloc(n));
-  SVIMX_CHECK_HIP(list.alloc(n));
-  SVIMX_CHECK_HIP(prev.alloc(n));
-  SVIMX_CHECK_HIP(nd.alloc(2 * n));
-  SVIMX_CHECK_HIP(nt.alloc(2 * n));
-  SVIMX_CHECK_HIP(changed.alloc(1));
+  SVIMX_CHECK_HIP(keys.alloc(n, st));
+  SVIMX_CHECK_HIP(known.alloc(n, st));
+  SVIMX_CHECK_HIP(ring.alloc(n, st));
+  SVIMX_CHECK_HIP(list.alloc(n, st));
+  SVIMX_CHECK_HIP(prev.alloc(n, st));
+  SVIMX_CHECK_HIP(nd.alloc(2 * n, st));
+  SVIMX_CHECK_HIP(nt.alloc(2 * n, st));
+  SVIMX_CHECK_HIP(changed.alloc(1, st));
   float* dist = dist_out;
   int32_t* tid = tid_out;
   if (!dist) {
-    SVIMX_CHECK_HIP(dist_own.alloc(n));
+    SVIMX_CHECK_HIP(dist_own.alloc(n, st));
     dist = dist_own.p;
   }
   if (!tid) {
-    SVIMX_CHECK_HIP(tid_own.alloc(n));
+    SVIMX_CHECK_HIP(tid_own.alloc(n, st));
     tid = tid_own.p;
   }
   TriSoup T{tri_local, tri_org};
@@ -375,7 +381,7 @@ int svimx_sdf_build(void* stream, const float* tri_local, const int32_t* tri_org
     const int na = dimsv[axis], nb = dimsv[(axis + 1) % 3], nc = dimsv[(axis + 2) % 3];
     const int64_t nacc = (int64_t)(na + 1) * nb * nc;
     DevBuf<int32_t> accb;
-    SVIMX_CHECK_HIP(accb.alloc(nacc));
+    SVIMX_CHECK_HIP(accb.alloc(nacc, st));
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(nacc, 256)), tb, 0, st, accb.p, nacc, 0);
     SVIMX_CHECK_LAUNCH();
     if (n_faces > 0) {

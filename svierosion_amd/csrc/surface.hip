@@ -245,13 +245,18 @@ __global__ void k_mt_emit(MtArgs A, const int64_t* toffs, const uint32_t* masks,
   for (int t = 0; t < 6; ++t) tet_triangles(A, n0, cv, t, put);
 }
 
+// stream-ordered device arrays (no device-wide synchronisation on free)
 template <class T>
 struct Buf {
   T* p = nullptr;
+  hipStream_t st = nullptr;
   ~Buf() {
-    if (p) hipFree(p);
+    if (p) hipFreeAsync(p, st);
   }
-  hipError_t alloc(int64_t n) { return hipMalloc(&p, sizeof(T) * (n > 0 ? n : 1)); }
+  hipError_t alloc(int64_t n, hipStream_t s) {
+    st = s;
+    return hipMallocAsync(reinterpret_cast<void**>(&p), sizeof(T) * (n > 0 ? n : 1), s);
+  }
   T* release() {
     T* r = p;
     p = nullptr;
@@ -282,12 +287,12 @@ int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, i
   Buf<int32_t> pos, ntri, vcnt;
   Buf<uint32_t> masks;
   Buf<int64_t> toffs, voffs;
-  SVIMX_CHECK_HIP(pos.alloc(n));
-  SVIMX_CHECK_HIP(ntri.alloc(nb));
-  SVIMX_CHECK_HIP(vcnt.alloc(nb));
-  SVIMX_CHECK_HIP(masks.alloc(nb));
-  SVIMX_CHECK_HIP(toffs.alloc(nb + 1));
-  SVIMX_CHECK_HIP(voffs.alloc(nb + 1));
+  SVIMX_CHECK_HIP(pos.alloc(n, st));
+  SVIMX_CHECK_HIP(ntri.alloc(nb, st));
+  SVIMX_CHECK_HIP(vcnt.alloc(nb, st));
+  SVIMX_CHECK_HIP(masks.alloc(nb, st));
+  SVIMX_CHECK_HIP(toffs.alloc(nb + 1, st));
+  SVIMX_CHECK_HIP(voffs.alloc(nb + 1, st));
   const unsigned gb = (unsigned)blocks_for(nb, 256);
   hipLaunchKernelGGL(k_mt_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos.p, n, -1);
   SVIMX_CHECK_LAUNCH();
@@ -309,8 +314,8 @@ int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, i
   if (nt == 0) return SVIMX_OK;
   Buf<float> xyz;
   Buf<int32_t> faces;
-  SVIMX_CHECK_HIP(xyz.alloc(3 * nv));
-  SVIMX_CHECK_HIP(faces.alloc(3 * nt));
+  SVIMX_CHECK_HIP(xyz.alloc(3 * nv, st));
+  SVIMX_CHECK_HIP(faces.alloc(3 * nt, st));
   hipLaunchKernelGGL(k_mt_vertices, dim3(gb), dim3(256), 0, st, phi, band, nb, d, masks.p, voffs.p, xyz.p);
   SVIMX_CHECK_LAUNCH();
   if (max_shift > 0) {
@@ -330,7 +335,9 @@ int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, i
 }
 
 int svimx_free(void* p) {
-  if (p) SVIMX_CHECK_HIP(hipFree(p));
+  // the arrays come from the stream-ordered allocator; the caller has copied
+  // them out (synchronously), so a free ordered on the null stream is safe
+  if (p) SVIMX_CHECK_HIP(hipFreeAsync(p, nullptr));
   return SVIMX_OK;
 }
 

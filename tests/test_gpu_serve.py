@@ -1,0 +1,71 @@
+"""Served and batched erosion: each result is the same alone (sequential erode)
+and in the mix, under continuous batching with arrivals and an HBM budget."""
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    from svierosion_amd.api import ErosionParams
+    from svierosion_amd.batch import Request
+    from svierosion_amd.shapes import cube, icosphere, sh_fragment
+    return [
+        Request(cube(1.0), ErosionParams(law="svi", radius=0.2, k=1, s=1, t=0.9, resolution=40,
+                                         target_fraction=0.93, snapshots=(0.97,)), 0.0),
+        Request(icosphere(3, 0.8), ErosionParams(law="mean", alpha=1.0, resolution=32, target_fraction=0.95), 0.05),
+        Request(sh_fragment(level=4, radius=50.0, seed=3), ErosionParams(law="svi", radius=2.0, k=2, s=0,
+                                                                         resolution=64, target_fraction=0.9), 0.1),
+        Request(icosphere(3, 0.8), ErosionParams(law="gauss", alpha=0.5, resolution=24, target_fraction=0.97), 0.1),
+    ]
+
+
+def _check_same(rep_a, rep_b):
+    import numpy as np
+    for a, b in zip(rep_a.completions, rep_b.completions):
+        assert a.error is None and b.error is None, (a.error, b.error)
+        ra, rb = a.result, b.result
+        assert ra.stop_reason == rb.stop_reason
+        assert len(ra.history) == len(rb.history)
+        for x, y in zip(ra.history, rb.history):
+            assert x.volume == y.volume and x.dt == y.dt and x.reinit == y.reinit
+        assert len(ra.snapshots) == len(rb.snapshots)
+        assert np.array_equal(ra.mesh.vertices, rb.mesh.vertices)
+        assert np.array_equal(ra.mesh.faces, rb.mesh.faces)
+        for k in rb.mesh.vertex_data:
+            assert np.array_equal(ra.mesh.vertex_data[k], rb.mesh.vertex_data[k]), k
+
+
+def test_served_mix_equals_sequential(device):
+    from svierosion_amd.batch import serve, serve_sequential
+    reqs = _requests()
+    mix = serve(reqs, max_jobs=8, workers=2)
+    seq = serve_sequential(reqs)
+    assert len(mix.done) == len(reqs)
+    _check_same(mix, seq)
+    assert all(c.admitted >= c.arrival for c in mix.completions)
+    assert mix.latency(0.99) >= mix.latency(0.5) > 0
+
+
+def test_budget_admits_one_job_at_a_time(device):
+    from svierosion_amd.batch import serve, serve_sequential
+    reqs = _requests()[:3]
+    tight = serve(reqs, max_jobs=8, hbm_budget=1.0, workers=1)  # every job exceeds the budget alone
+    seq = serve_sequential(reqs)
+    assert len(tight.done) == 3
+    # with the budget taken by one job, the next is admitted only after it finished
+    ordered = sorted(tight.completions, key=lambda c: c.admitted)
+    for prev, nxt in zip(ordered, ordered[1:]):
+        assert nxt.admitted >= prev.finished
+    _check_same(tight, seq)
+
+
+def test_erode_batch_reports_parameter_errors(device):
+    from svierosion_amd.api import ErosionParams
+    from svierosion_amd.batch import erode_batch
+    from svierosion_amd.shapes import cube
+    out = erode_batch([cube(1.0), cube(1.0)],
+                      [ErosionParams(law="svi", radius=0.2, k=1, s=1, t=0.8, resolution=40),
+                       ErosionParams(law="svi", radius=0.2, k=1, s=0, resolution=40, target_fraction=0.95)])
+    assert isinstance(out[0], Exception) and "max N_r" in str(out[0])
+    assert out[1].stop_reason == "target_volume"

@@ -94,3 +94,24 @@ def test_marching_tetrahedra_closed_and_accurate():
     m = TriMesh(-1.3 + h * v, f)
     assert m.is_watertight() and m.is_consistently_oriented()
     assert abs(m.signed_volume() / (4 / 3 * np.pi / 1.2) - 1) < 2e-3
+
+
+def test_workloads_are_seeded_and_prefix_stable():
+    from svierosion_amd.workloads import _job
+    a = _job(4, 7, 96, 320, 0.9)
+    b = _job(4, 7, 96, 320, 0.9)
+    assert a[1] == b[1]
+    assert np.array_equal(a[0].vertices, b[0].vertices)
+    res = [_job(4, i, 96, 320, 0.9)[1].resolution for i in range(20)]
+    assert min(res) >= 96 and max(res) <= 320
+
+
+def test_job_device_bytes_scales_with_grid():
+    from svierosion_amd.api import ErosionParams
+    from svierosion_amd.batch import job_device_bytes
+    from svierosion_amd.shapes import icosphere
+    m = icosphere(3, 1.0)
+    small = job_device_bytes(m, ErosionParams(law="svi", radius=0.1, resolution=64))
+    big = job_device_bytes(m, ErosionParams(law="svi", radius=0.1, resolution=128))
+    assert 4.0 < big / small < 9.0  # ~ the node count ratio (160^3 / 90^3 = 5.6)
+    assert 15 < small / 90 ** 3 < 80  # tens of bytes per node, SDF transients included

@@ -1,9 +1,11 @@
 // PDE reinitialisation (Min 2010) on the dilated band, followed by the band
 // rebuild (svierosion/levelset.py:154-183).
 //
-// Work set: the band scattered into a byte mask, dilated by a (2r+1)^3 cube
-// with three separable line passes, minus `margin` layers at the grid faces,
-// then compacted in node order -- svierosion/levelset.py:166-174.
+// Work set: the band scattered into a bit mask (one bit per node; z-line
+// (x, y) holds wpl = ceil(nz/32) words), dilated by a (2r+1)^3 cube with three
+// separable word passes (shifts and ORs along z, ORs of whole words along x
+// and y), minus `margin` layers at the grid faces, then compacted in node
+// order straight from the set bits -- svierosion/levelset.py:166-174.
 //
 // Geometry (subcell crossings and local dtau, svierosion/kernels.py:335-359)
 // is computed once from phi0.  Only nodes next to the interface have a
@@ -27,41 +29,156 @@
 
 namespace svimx {
 
-__global__ void k_u8_fill(uint8_t* p, int64_t n, uint8_t v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+__global__ void k_bits_fill(uint32_t* m, int64_t nw) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nw) m[i] = 0u;
 }
 
-__global__ void k_scatter_mark(uint8_t* mask, const int32_t* idx, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) mask[idx[i]] = 1;
+__global__ void k_bits_scatter(uint32_t* m, const int32_t* __restrict__ idx, int64_t n, int nz, int wpl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t node = (uint32_t)idx[i];
+  const uint32_t line = node / (uint32_t)nz, z = node - line * (uint32_t)nz;
+  atomicOr(&m[(int64_t)line * wpl + (z >> 5)], 1u << (z & 31u));
 }
 
-// out = OR over offsets -r..r along `axis` of in; the last pass also clears
-// `margin` layers at every face.
-__global__ void k_dilate_axis(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, Dims d, int axis, int r,
-                              int margin) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= d.size()) return;
-  int c[3];
-  unflat(d, e, c[0], c[1], c[2]);
-  const int n[3] = {d.nx, d.ny, d.nz};
-  const int64_t st[3] = {d.sx(), d.sy(), 1};
-  int lo = c[axis] - r < 0 ? 0 : c[axis] - r;
-  int hi = c[axis] + r > n[axis] - 1 ? n[axis] - 1 : c[axis] + r;
-  uint8_t v = 0;
-  for (int q = lo; q <= hi && !v; ++q) v = in[e + (int64_t)(q - c[axis]) * st[axis]];
-  if (margin > 0) {
-    for (int a = 0; a < 3; ++a)
-      if (c[a] < margin || c[a] > n[a] - 1 - margin) v = 0;
+// bits of word wi of a z-line that are nodes (z < nz)
+__device__ __forceinline__ uint32_t valid_bits(int wi, int nz) {
+  const int left = nz - 32 * wi;
+  return left >= 32 ? 0xffffffffu : ((1u << left) - 1u);
+}
+
+// out = OR over dz in [-r, r] (r <= 31) of in, within each z-line.
+__global__ void k_bits_dilate_z(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t nw, int wpl,
+                                int nz, int r) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  const int wi = (int)((uint64_t)w % (uint32_t)wpl);
+  const uint32_t cur = in[w];
+  const uint32_t prev = wi > 0 ? in[w - 1] : 0u;
+  const uint32_t next = wi < wpl - 1 ? in[w + 1] : 0u;
+  uint32_t acc = cur;
+  for (int s = 1; s <= r; ++s) {
+    acc |= (cur >> s) | (next << (32 - s));  // bit i <- bit i + s
+    acc |= (cur << s) | (prev >> (32 - s));  // bit i <- bit i - s
   }
-  out[e] = v;
+  out[w] = acc & valid_bits(wi, nz);
 }
 
-struct MaskPred {
-  const uint8_t* m;
-  __device__ bool operator()(int64_t e) const { return m[e] != 0; }
-};
+// out = OR over [-r, r] along x (axis 0) or y (axis 1) of whole words; with
+// margin > 0 the `margin` layers at every face are cleared.
+__global__ void k_bits_dilate_xy(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int nx, int ny,
+                                 int wpl, int nz, int axis, int r, int margin) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nw = (int64_t)nx * ny * wpl;
+  if (w >= nw) return;
+  const int64_t xy = w / wpl;
+  const int wi = (int)(w - xy * wpl);
+  const int y = (int)(xy % ny), x = (int)(xy / ny);
+  const int c = axis == 0 ? x : y, n = axis == 0 ? nx : ny;
+  const int64_t st = axis == 0 ? (int64_t)ny * wpl : (int64_t)wpl;
+  const int lo = c - r < 0 ? -c : -r, hi = c + r > n - 1 ? n - 1 - c : r;
+  uint32_t acc = 0u;
+  for (int q = lo; q <= hi; ++q) acc |= in[w + q * st];
+  if (margin > 0) {
+    if (x < margin || x > nx - 1 - margin || y < margin || y > ny - 1 - margin) acc = 0u;
+    // keep z in [margin, nz - 1 - margin]
+    const int z0 = 32 * wi;
+    for (int b = 0; b < 32; ++b) {
+      const int z = z0 + b;
+      if (z < margin || z > nz - 1 - margin) acc &= ~(1u << b);
+    }
+  }
+  out[w] = acc & valid_bits(wi, nz);
+}
+
+// Set bits per compaction chunk (kCompactChunk words).
+__global__ void __launch_bounds__(kCompactThreads) k_bits_count(const uint32_t* __restrict__ m, int64_t nw,
+                                                                int64_t* chunk_counts) {
+  __shared__ double red[4];
+  const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
+  int64_t cnt = 0;
+  for (int r = 0; r < kCompactPerThread; ++r) {
+    const int64_t w = base + (int64_t)r * kCompactThreads + threadIdx.x;
+    if (w < nw) cnt += __popc(m[w]);
+  }
+  const double tot = block_reduce<0>((double)cnt, red);
+  if (threadIdx.x == 0) chunk_counts[blockIdx.x] = (int64_t)tot;
+}
+
+// Exclusive prefix of v over the block (kCompactThreads threads); *total = sum.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds_waves, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  __syncthreads();
+  if (lane == 63) lds_waves[wave] = inc;
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int q = 0; q < kCompactThreads / 64; ++q) {
+    const int t = lds_waves[q];
+    before += q < wave ? t : 0;
+    tot += t;
+  }
+  *total = tot;
+  return before + inc - v;
+}
+
+// Node indices of the set bits, in node order, at the scanned chunk offsets.
+__global__ void __launch_bounds__(kCompactThreads) k_bits_scatter_list(const uint32_t* __restrict__ m, int64_t nw,
+                                                                       int wpl, int nz, const int64_t* offs,
+                                                                       int32_t* __restrict__ out, int64_t cap) {
+  __shared__ int wave_tot[kCompactThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
+  int64_t pos = offs[blockIdx.x];
+  for (int r = 0; r < kCompactPerThread; ++r) {
+    const int64_t w = base + (int64_t)r * kCompactThreads + threadIdx.x;
+    uint32_t bits = w < nw ? m[w] : 0u;
+    int total;
+    const int excl = block_exclusive_scan(__popc(bits), wave_tot, &total);
+    if (bits) {
+      const int64_t line = w / wpl;
+      const int64_t node0 = line * nz + (w - line * wpl) * 32;
+      int64_t o = pos + excl;
+      while (bits) {
+        const int b = __ffs(bits) - 1;
+        if (o < cap) out[o] = (int32_t)(node0 + b);
+        ++o;
+        bits &= bits - 1u;
+      }
+    }
+    pos += total;
+  }
+}
+
+// Order-preserving list of the set bits of m (synchronises, returns the count).
+static int compact_bits(hipStream_t st, const uint32_t* m, int64_t nw, int wpl, int nz, int32_t* out, int64_t cap,
+                        Workspace& ws, int64_t* count) {
+  const int64_t chunks = (nw + kCompactChunk - 1) / kCompactChunk;
+  SVIMX_REQUIRE(chunks + 1 <= ws.max_chunks, SVIMX_EBADARG, "workspace too small for compaction");
+  if (chunks == 0) {
+    *count = 0;
+    return SVIMX_OK;
+  }
+  hipLaunchKernelGGL(k_bits_count, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, m, nw, ws.chunk_counts);
+  SVIMX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, ws.chunk_counts, chunks);
+  SVIMX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_bits_scatter_list, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, m, nw, wpl, nz,
+                     ws.chunk_counts, out, cap);
+  SVIMX_CHECK_LAUNCH();
+  int64_t total = 0;
+  SVIMX_CHECK_HIP(hipMemcpyAsync(&total, ws.chunk_counts + chunks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SVIMX_CHECK_HIP(hipStreamSynchronize(st));
+  *count = total;
+  SVIMX_REQUIRE(total <= cap, SVIMX_ECAPACITY, "compaction output needs %lld entries, capacity %lld",
+                (long long)total, (long long)cap);
+  return SVIMX_OK;
+}
 
 // partials[b][slot] = #{i in block b's share : phi[list[i]] < 0}
 __global__ void __launch_bounds__(kBandThreads)
@@ -191,14 +308,23 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   Workspace w;
   if (!carve_workspace(ws, ws_bytes, n, w)) return SVIMX_EBADARG;
   hipStream_t st = as_stream(stream);
-  const dim3 gn(blocks_for(n, 256)), tb(256);
+  const dim3 tb(256);
   ReduceSpec spec;
   auto one_slot = [&](int slot, int op, int dst) {
     for (int s = 0; s < kPartialSlots; ++s) spec.op[s] = -1, spec.dst[s] = 0;
     spec.op[slot] = (int8_t)op;
     spec.dst[slot] = (int8_t)dst;
   };
-  hipLaunchKernelGGL(k_u8_fill, gn, tb, 0, st, mask_a, n, (uint8_t)0);
+  // bit masks in the caller's n-byte mask buffers
+  const int wpl = (nz + 31) / 32;
+  const int64_t nwords = (int64_t)nx * ny * wpl;
+  SVIMX_REQUIRE(nwords * 4 <= n && (reinterpret_cast<uintptr_t>(mask_a) & 3) == 0 &&
+                    (reinterpret_cast<uintptr_t>(mask_b) & 3) == 0 && dilation <= 31,
+                SVIMX_EBADARG, "reinit masks need nz >= 4, 4-byte alignment and dilation <= 31");
+  uint32_t* ba = reinterpret_cast<uint32_t*>(mask_a);
+  uint32_t* bb = reinterpret_cast<uint32_t*>(mask_b);
+  const dim3 gwd(blocks_for(nwords, 256));
+  hipLaunchKernelGGL(k_bits_fill, gwd, tb, 0, st, ba, nwords);
   SVIMX_CHECK_LAUNCH();
   int nblk = reduce_grid(nb);
   hipLaunchKernelGGL(k_count_neg, dim3(nblk), tb, 0, st, phi, band, nb, w.partials, 0);
@@ -207,18 +333,18 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   int rc = launch_finalize(st, w.partials, nblk, w.results, spec);
   if (rc) return rc;
   if (nb > 0) {
-    hipLaunchKernelGGL(k_scatter_mark, dim3(blocks_for(nb, 256)), tb, 0, st, mask_a, band, nb);
+    hipLaunchKernelGGL(k_bits_scatter, dim3(blocks_for(nb, 256)), tb, 0, st, ba, band, nb, nz, wpl);
     SVIMX_CHECK_LAUNCH();
   }
-  // x, y, z passes: a -> b -> a -> b; margin cleared on the last
-  hipLaunchKernelGGL(k_dilate_axis, gn, tb, 0, st, mask_a, mask_b, d, 0, dilation, 0);
+  // z, x, y passes: a -> b -> a -> b; margin cleared on the last
+  hipLaunchKernelGGL(k_bits_dilate_z, gwd, tb, 0, st, ba, bb, nwords, wpl, nz, dilation);
   SVIMX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_dilate_axis, gn, tb, 0, st, mask_b, mask_a, d, 1, dilation, 0);
+  hipLaunchKernelGGL(k_bits_dilate_xy, gwd, tb, 0, st, bb, ba, nx, ny, wpl, nz, 0, dilation, 0);
   SVIMX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_dilate_axis, gn, tb, 0, st, mask_a, mask_b, d, 2, dilation, margin);
+  hipLaunchKernelGGL(k_bits_dilate_xy, gwd, tb, 0, st, ba, bb, nx, ny, wpl, nz, 1, dilation, margin);
   SVIMX_CHECK_LAUNCH();
   int64_t nw = 0;
-  rc = compact(st, MaskPred{mask_b}, n, work_idx, work_cap, w, &nw);
+  rc = compact_bits(st, bb, nwords, wpl, nz, work_idx, work_cap, w, &nw);
   info[9] = nw;
   if (rc) return rc;
   const float capf = (float)(width * h);

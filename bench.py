@@ -55,7 +55,8 @@ def build_workload(args):
         raise SystemExit(f"workload invalid: max N_r {nmax:.4f} >= t {law.t}")
     info = {"triangles": int(len(mesh.faces)), "grid": list(grid.shape), "nodes": int(grid.size),
             "band_nodes": int(ls.nb), "mesh_s": round(t1 - t0, 2), "sdf_setup_s": round(t2 - t1, 2),
-            "nmax0": round(nmax, 5), "m": int(sim.conv.m)}
+            "nmax0": round(nmax, 5), "m": int(sim.conv.m), "nr_run_length": sim.conv.run_length,
+            "nr_tuning_ms": {str(k): round(v, 4) for k, v in sim.tuning.items()}}
     return mesh, grid, ls, sim, law, info
 
 
@@ -261,6 +262,7 @@ def main(argv=None):
             "config": {"workload": "1.3M-triangle noisy ellipsoid scan, SVI k=2 s=1 t=0.95 r=2.0, res 400",
                        "triangles": info["triangles"], "grid": info["grid"], "band_nodes": info["band_nodes"],
                        "reinit_in_window": reinits, "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
+                       "nr_run_length": info["nr_run_length"], "nr_tuning_ms": info["nr_tuning_ms"],
                        "launches": launches},
             "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "served": served}
     print(json.dumps(line), flush=True)

@@ -145,6 +145,17 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
 int svimx_nr_plan_destroy(void* plan);
 int svimx_nr_reset(void* stream, void* plan);
 int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t n);
+/* Autotuner: times the run-mode summation with z-runs of 4, 8 and 16 nodes on
+ * this job's field (fractions and selection recomputed per candidate; the
+ * N_r values at the nodes the velocity reads are identical for every run
+ * length) and keeps the fastest.  ms_out (3 doubles, may be NULL) receives the
+ * per-launch times.  Leaves occupancy, marks and run list those of the chosen
+ * length for phi.  Synchronises.  No counterpart in the reference. */
+int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
+                       int nz, double h, double cut_hi, float* occupancy, float* nr_out, void* ws, int64_t ws_bytes,
+                       double* ms_out);
+/* Run length of a run-mode plan (0: tile mode). */
+int svimx_nr_plan_run_length(void* plan);
 int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws,
                   int64_t ws_bytes);
 /* The two halves of svimx_nr_exec: compact the marked tiles into the plan's

@@ -147,7 +147,7 @@ __device__ __forceinline__ void fractions_body(const float* __restrict__ phi, co
       closest_point(g, i, j, k, h, hi.v, p);
       const int bx = (int)floorf(p[0]) - 1, by = (int)floorf(p[1]) - 1, bz = (int)floorf(p[2]) - 1;
       if (run_mode)
-        mark_stencil_runs(marks, d.ny, nzr, bx, by, bz);
+        mark_stencil_runs(marks, d.ny, nzr, run_mode, bx, by, bz);
       else
         mark_stencil_tiles(marks, nty, ntz, bx, by, bz);
     }

@@ -77,9 +77,8 @@ struct Seg {
   int32_t slot[kMaxEngineJobs];
 };
 
-enum SegId {
-  T_ALL, T_SVI, T_MARKS, T_RUNS, T_CHUNKS, T_JOBS, T_RUNJOBS, T_CURV, T_NR4, T_NR8, T_NR12, T_NR16, T_COUNT
-};
+// N_r tables: T_NR + 3 * (MP / 4 - 1) + (0, 1, 2 for run lengths 4, 8, 16)
+enum SegId { T_ALL, T_SVI, T_MARKS, T_RUNS, T_CHUNKS, T_JOBS, T_RUNJOBS, T_CURV, T_NR, T_COUNT = T_NR + 12 };
 
 struct Tables {
   Seg seg[T_COUNT];
@@ -177,13 +176,13 @@ __global__ void __launch_bounds__(kCompactThreads) k_eng_need_scatter(const JobD
   }
 }
 
-template <int MP>
+template <int MP, int RL>
 __global__ void __launch_bounds__(256) k_eng_nr_runs(const JobDev* __restrict__ J, const Seg* S) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
     const int64_t nrun = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
-    nr_runs_body<MP>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.weights,
+    nr_runs_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.weights,
                      j.scale, j.negbv, const_cast<float*>(j.field), xcd_slice(lb, G), G);
   }
 }
@@ -454,8 +453,8 @@ static int upload(Engine& E) {
       seg_add(T.seg[T_RUNS], slot, (e.plan->nruns + 255) / 256);
       seg_add(T.seg[T_CHUNKS], slot, e.plan->nchunks);
       seg_add(T.seg[T_RUNJOBS], slot, 1);
-      const int mp = mp_of(e.plan);
-      const int t = mp == 4 ? T_NR4 : mp == 8 ? T_NR8 : mp == 12 ? T_NR12 : T_NR16;
+      const int rl = e.plan->run_mode;
+      const int t = T_NR + 3 * (mp_of(e.plan) / 4 - 1) + (rl == 4 ? 0 : rl == 8 ? 1 : 2);
       seg_add(T.seg[t], slot, nr_grid(e));
     }
   }
@@ -498,10 +497,12 @@ static int prepare(Engine& E, int rel, int stats, int only_needed) {
   ENG_LAUNCH(k_eng_need_count, T_CHUNKS, kCompactThreads);
   ENG_LAUNCH(k_eng_need_scan, T_RUNJOBS, 1024);
   ENG_LAUNCH(k_eng_need_scatter, T_CHUNKS, kCompactThreads);
-  ENG_LAUNCH(k_eng_nr_runs<4>, T_NR4, 256);
-  ENG_LAUNCH(k_eng_nr_runs<8>, T_NR8, 256);
-  ENG_LAUNCH(k_eng_nr_runs<12>, T_NR12, 256);
-  ENG_LAUNCH(k_eng_nr_runs<16>, T_NR16, 256);
+#define ENG_NR(MPV, RLV, I) ENG_LAUNCH((k_eng_nr_runs<MPV, RLV>), T_NR + (I), 256)
+  ENG_NR(4, 4, 0); ENG_NR(4, 8, 1); ENG_NR(4, 16, 2);
+  ENG_NR(8, 4, 3); ENG_NR(8, 8, 4); ENG_NR(8, 16, 5);
+  ENG_NR(12, 4, 6); ENG_NR(12, 8, 7); ENG_NR(12, 16, 8);
+  ENG_NR(16, 4, 9); ENG_NR(16, 8, 10); ENG_NR(16, 16, 11);
+#undef ENG_NR
   // tile-mode plans (balls wider than 16 cells) run their own launches
   for (int slot : E.set) {
     EngJob& e = E.jobs[slot];

@@ -52,7 +52,7 @@ __global__ void k_nr_mark_nodes(uint8_t* marks, int run_mode, int nty, int ntz, 
   int i, j, k;
   unflat(d, nodes[q], i, j, k);
   if (run_mode)
-    marks[((int64_t)i * d.ny + j) * nzr + k / kRunLen] = 1;
+    marks[((int64_t)i * d.ny + j) * nzr + k / run_mode] = 1;
   else
     marks[((int64_t)(i / kTX) * nty + j / kTY) * ntz + k / kTZ] = 1;
 }
@@ -64,7 +64,7 @@ __global__ void k_runs_dilate_y(const uint8_t* __restrict__ marks, uint8_t* __re
 }
 
 // ---------------------------------------------------------------- run-mode summation
-template <int MP>
+template <int MP, int RL>
 __global__ void __launch_bounds__(256)
     k_nr_runs(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
               const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, int ncols,
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256)
   const int64_t nrun = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
   // neighbouring runs read the same occupancy columns: keep them on one XCD's L2
-  nr_runs_body<MP>(occ, OL, nzr, list, nrun, cols, ncols, w, scale, negbv, out, xcd_slice(blockIdx.x, gridDim.x),
+  nr_runs_body<MP, RL>(occ, OL, nzr, list, nrun, cols, ncols, w, scale, negbv, out, xcd_slice(blockIdx.x, gridDim.x),
                    gridDim.x);
 }
 
@@ -141,6 +141,15 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Run length of a run-mode plan (the arrays are sized for kRunLenMin).
+void nr_plan_set_run_len(NrPlan* p, int rl) {
+  p->run_mode = rl;
+  p->nzr = (p->d.nz + rl - 1) / rl;
+  p->nruns = (int64_t)p->d.nx * p->d.ny * p->nzr;
+  p->nmarks = p->nruns;
+  p->nchunks = (p->nruns + kCompactChunk - 1) / kCompactChunk;
+}
+
 }  // namespace svimx
 
 using namespace svimx;
@@ -190,15 +199,17 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   p->h = h;
   p->radius = radius;
   p->d = Dims{nx, ny, nz};
-  p->run_mode = run_mode ? 1 : 0;
+  p->run_mode = 0;
   p->ntx = (nx + kTX - 1) / kTX;
   p->nty = (ny + kTY - 1) / kTY;
   p->ntz = (nz + kTZ - 1) / kTZ;
   p->ntiles = (int64_t)p->ntx * p->nty * p->ntz;
-  p->nzr = (nz + kRunLen - 1) / kRunLen;
-  p->nruns = (int64_t)nx * ny * p->nzr;
-  p->nmarks = run_mode ? p->nruns : p->ntiles;
-  p->nchunks = (p->nruns + kCompactChunk - 1) / kCompactChunk;
+  // run-mode arrays are sized for the shortest run length the tuner may pick
+  const int64_t max_runs = (int64_t)nx * ny * ((nz + kRunLenMin - 1) / kRunLenMin);
+  const int64_t cap_marks = run_mode ? max_runs : p->ntiles;
+  const int64_t cap_chunks = (max_runs + kCompactChunk - 1) / kCompactChunk;
+  p->nmarks = p->ntiles;
+  if (run_mode) nr_plan_set_run_len(p, kRunLen);
   p->ncols = (int)cols.size();
   p->nweights = (int)wts.size();
   p->scale = (float)(h * h * h);
@@ -206,20 +217,20 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   hipError_t e = hipMalloc(&p->cols, sizeof(NrColumn) * (cols.size() + 1));
   if (e == hipSuccess) e = hipMalloc(&p->col_start, sizeof(int) * col_start.size());
   if (e == hipSuccess) e = hipMalloc(&p->weights, sizeof(float) * (wts.size() + 1));
-  if (e == hipSuccess) e = hipMalloc(&p->marks, p->nmarks);
-  if (e == hipSuccess) e = hipMalloc(&p->dil, run_mode ? p->nruns : 1);
-  if (e == hipSuccess) e = hipMalloc(&p->list, sizeof(int32_t) * p->nmarks);
+  if (e == hipSuccess) e = hipMalloc(&p->marks, cap_marks);
+  if (e == hipSuccess) e = hipMalloc(&p->dil, run_mode ? max_runs : 1);
+  if (e == hipSuccess) e = hipMalloc(&p->list, sizeof(int32_t) * cap_marks);
   if (e == hipSuccess) e = hipMalloc(&p->count, sizeof(int32_t));
-  if (e == hipSuccess) e = hipMalloc(&p->ccounts, sizeof(int64_t) * (p->nchunks + 2));
+  if (e == hipSuccess) e = hipMalloc(&p->ccounts, sizeof(int64_t) * (cap_chunks + 2));
   if (e == hipSuccess && !cols.empty())
     e = hipMemcpy(p->cols, cols.data(), sizeof(NrColumn) * cols.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(p->col_start, col_start.data(), sizeof(int) * col_start.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess && !wts.empty())
     e = hipMemcpy(p->weights, wts.data(), sizeof(float) * wts.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(p->marks, 0, p->nmarks);
-  if (e == hipSuccess) e = hipMemset(p->list, 0, sizeof(int32_t) * p->nmarks);
-  if (e == hipSuccess) e = hipMemset(p->ccounts, 0, sizeof(int64_t) * (p->nchunks + 2));
+  if (e == hipSuccess) e = hipMemset(p->marks, 0, cap_marks);
+  if (e == hipSuccess) e = hipMemset(p->list, 0, sizeof(int32_t) * cap_marks);
+  if (e == hipSuccess) e = hipMemset(p->ccounts, 0, sizeof(int64_t) * (cap_chunks + 2));
   if (e != hipSuccess) {
     set_error("N_r plan allocation failed: %s", hipGetErrorString(e));
     hipFree(p->cols); hipFree(p->col_start); hipFree(p->weights); hipFree(p->marks); hipFree(p->dil);
@@ -310,16 +321,23 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
     // grid: a multiple of the XCD count (the kernel's block remapping needs it)
     const int grid = (int)std::min<int64_t>((p->nruns + 255) / 256 + kXcds - 1, 256 * 8) / kXcds * kXcds;
     const int mp = (p->m + 3) / 4 * 4;
-#define SVIMX_RUNS(MPV)                                                                                            \
-  hipLaunchKernelGGL(k_nr_runs<MPV>, dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, total, p->cols, \
-                     p->ncols, p->weights, p->scale, p->negbv, nr_out, w.results)
-    switch (mp) {
-      case 0:
-      case 4: SVIMX_RUNS(4); break;
-      case 8: SVIMX_RUNS(8); break;
-      case 12: SVIMX_RUNS(12); break;
-      default: SVIMX_RUNS(16); break;
+#define SVIMX_RUNS(MPV, RLV)                                                                                       \
+  hipLaunchKernelGGL((k_nr_runs<MPV, RLV>), dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, total,   \
+                     p->cols, p->ncols, p->weights, p->scale, p->negbv, nr_out, w.results)
+#define SVIMX_RUNS_MP(RLV)                  \
+  switch (mp) {                             \
+    case 0:                                 \
+    case 4: SVIMX_RUNS(4, RLV); break;      \
+    case 8: SVIMX_RUNS(8, RLV); break;      \
+    case 12: SVIMX_RUNS(12, RLV); break;    \
+    default: SVIMX_RUNS(16, RLV); break;    \
+  }
+    switch (p->run_mode) {
+      case 4: SVIMX_RUNS_MP(4); break;
+      case 16: SVIMX_RUNS_MP(16); break;
+      default: SVIMX_RUNS_MP(8); break;
     }
+#undef SVIMX_RUNS_MP
 #undef SVIMX_RUNS
     SVIMX_CHECK_LAUNCH();
     return SVIMX_OK;
@@ -331,6 +349,53 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
   SVIMX_TRY_END
+}
+
+int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
+                       int nz, double h, double cut_hi, float* occupancy, float* nr_out, void* ws, int64_t ws_bytes,
+                       double* ms_out) {
+  SVIMX_TRY_BEGIN
+  NrPlan* p = static_cast<NrPlan*>(plan);
+  SVIMX_REQUIRE(p && phi && band && occupancy && nr_out, SVIMX_EBADARG, "null argument");
+  if (!p->run_mode) return SVIMX_OK;  // tile mode has a single shape
+  hipStream_t st = as_stream(stream);
+  const int cands[3] = {4, 8, 16};
+  hipEvent_t ev[2];
+  SVIMX_CHECK_HIP(hipEventCreate(&ev[0]));
+  SVIMX_CHECK_HIP(hipEventCreate(&ev[1]));
+  int best_rl = p->run_mode, rc = SVIMX_OK;
+  double best = INFINITY;
+  for (int i = 0; i < 3 && !rc; ++i) {
+    nr_plan_set_run_len(p, cands[i]);
+    rc = svimx_fractions(stream, phi, band, nb, nx, ny, nz, h, occupancy, p, cut_hi, ws, ws_bytes);
+    if (!rc) rc = svimx_nr_select(stream, p);
+    if (!rc) rc = svimx_nr_sum(stream, p, occupancy, nr_out, ws, ws_bytes);  // warm-up
+    if (rc) break;
+    hipEventRecord(ev[0], st);
+    for (int r = 0; r < 3 && !rc; ++r) rc = svimx_nr_sum(stream, p, occupancy, nr_out, ws, ws_bytes);
+    hipEventRecord(ev[1], st);
+    if (hipEventSynchronize(ev[1]) != hipSuccess) rc = SVIMX_EDEVICE;
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (ms_out) ms_out[i] = ms / 3.0;
+    if (ms / 3.0 < best) {
+      best = ms / 3.0;
+      best_rl = cands[i];
+    }
+  }
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  // leave the plan's marks and run list those of the chosen run length
+  nr_plan_set_run_len(p, best_rl);
+  if (!rc) rc = svimx_fractions(stream, phi, band, nb, nx, ny, nz, h, occupancy, p, cut_hi, ws, ws_bytes);
+  if (!rc) rc = svimx_nr_select(stream, p);
+  return rc;
+  SVIMX_TRY_END
+}
+
+int svimx_nr_plan_run_length(void* plan) {
+  const NrPlan* p = static_cast<const NrPlan*>(plan);
+  return p ? p->run_mode : 0;
 }
 
 int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws, int64_t ws_bytes) {

@@ -30,28 +30,28 @@ struct RunNeed {
 };
 
 // ---------------------------------------------------------------- run-mode summation
-template <int K, int MP, int WIN>
+template <int K, int MP, int RL, int WIN>
 __device__ __forceinline__ void column_taps(const float* __restrict__ wc, const float (&win)[WIN],
-                                            float (&part)[kRunLen]) {
+                                            float (&part)[RL]) {
   if constexpr (K <= MP) {
 #pragma unroll
     for (int t = 0; t <= 2 * K; ++t) {
       const float wt = wc[t];
 #pragma unroll
-      for (int j = 0; j < kRunLen; ++j) part[j] += wt * win[j + MP - K + t];
+      for (int j = 0; j < RL; ++j) part[j] += wt * win[j + MP - K + t];
     }
   }
 }
 
-// One lane per z-run of the run list; lb is the (XCD-remapped) block index
-// among G blocks of blockDim.x threads.
-template <int MP>
+// One lane per z-run (RL nodes) of the run list; lb is the (XCD-remapped)
+// block index among G blocks of blockDim.x threads.
+template <int MP, int RL>
 __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
                                              const int32_t* __restrict__ list, int64_t nrun,
                                              const NrColumn* __restrict__ cols, int ncols,
                                              const float* __restrict__ w, float scale, float negbv,
                                              float* __restrict__ out, int64_t lb, int64_t G) {
-  constexpr int WIN = kRunLen + 2 * MP;
+  constexpr int WIN = RL + 2 * MP;
   const int64_t stride = G * blockDim.x;
   for (int64_t first = lb * blockDim.x; first < nrun; first += stride) {
     const int64_t it = first + threadIdx.x;
@@ -60,16 +60,16 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
     const int zr = r % nzr;
     const int64_t xy = r / nzr;
     const int y = (int)(xy % OL.ny), x = (int)(xy / OL.ny);
-    const int z0 = zr * kRunLen;
-    float acc[kRunLen], part[kRunLen];
+    const int z0 = zr * RL;
+    float acc[RL], part[RL];
 #pragma unroll
-    for (int j = 0; j < kRunLen; ++j) acc[j] = part[j] = 0.0f;
+    for (int j = 0; j < RL; ++j) acc[j] = part[j] = 0.0f;
     int cur_dx = cols[0].dx;
     for (int c = 0; c < ncols; ++c) {
       const NrColumn col = cols[c];
       if (col.dx != cur_dx) {
 #pragma unroll
-        for (int j = 0; j < kRunLen; ++j) {
+        for (int j = 0; j < RL; ++j) {
           acc[j] += part[j];
           part[j] = 0.0f;
         }
@@ -87,30 +87,30 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
       }
       const float* wc = w + col.woff;
       switch (-col.dz0) {
-        case 0: column_taps<0, MP>(wc, win, part); break;
-        case 1: column_taps<1, MP>(wc, win, part); break;
-        case 2: column_taps<2, MP>(wc, win, part); break;
-        case 3: column_taps<3, MP>(wc, win, part); break;
-        case 4: column_taps<4, MP>(wc, win, part); break;
-        case 5: column_taps<5, MP>(wc, win, part); break;
-        case 6: column_taps<6, MP>(wc, win, part); break;
-        case 7: column_taps<7, MP>(wc, win, part); break;
-        case 8: column_taps<8, MP>(wc, win, part); break;
-        case 9: column_taps<9, MP>(wc, win, part); break;
-        case 10: column_taps<10, MP>(wc, win, part); break;
-        case 11: column_taps<11, MP>(wc, win, part); break;
-        case 12: column_taps<12, MP>(wc, win, part); break;
-        case 13: column_taps<13, MP>(wc, win, part); break;
-        case 14: column_taps<14, MP>(wc, win, part); break;
-        case 15: column_taps<15, MP>(wc, win, part); break;
-        case 16: column_taps<16, MP>(wc, win, part); break;
+        case 0: column_taps<0, MP, RL>(wc, win, part); break;
+        case 1: column_taps<1, MP, RL>(wc, win, part); break;
+        case 2: column_taps<2, MP, RL>(wc, win, part); break;
+        case 3: column_taps<3, MP, RL>(wc, win, part); break;
+        case 4: column_taps<4, MP, RL>(wc, win, part); break;
+        case 5: column_taps<5, MP, RL>(wc, win, part); break;
+        case 6: column_taps<6, MP, RL>(wc, win, part); break;
+        case 7: column_taps<7, MP, RL>(wc, win, part); break;
+        case 8: column_taps<8, MP, RL>(wc, win, part); break;
+        case 9: column_taps<9, MP, RL>(wc, win, part); break;
+        case 10: column_taps<10, MP, RL>(wc, win, part); break;
+        case 11: column_taps<11, MP, RL>(wc, win, part); break;
+        case 12: column_taps<12, MP, RL>(wc, win, part); break;
+        case 13: column_taps<13, MP, RL>(wc, win, part); break;
+        case 14: column_taps<14, MP, RL>(wc, win, part); break;
+        case 15: column_taps<15, MP, RL>(wc, win, part); break;
+        case 16: column_taps<16, MP, RL>(wc, win, part); break;
         default: break;
       }
     }
     if (!live) continue;
     const int64_t base = xy * OL.nz + z0;
 #pragma unroll
-    for (int j = 0; j < kRunLen; ++j)
+    for (int j = 0; j < RL; ++j)
       if (z0 + j < OL.nz) out[base + j] = ((acc[j] + part[j]) * scale) * negbv + 1.0f;
   }
 }

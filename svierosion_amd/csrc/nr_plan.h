@@ -2,7 +2,8 @@
 // convolution, plus the marks that select where N_r is evaluated.
 //
 // Two evaluation modes:
-//  * runs  (m <= kRunMaxM): the unit of work is a z-run of kRunLen nodes; the
+//  * runs  (m <= kRunMaxM): the unit of work is a z-run of run_len nodes
+//    (4, 8 or 16: `run_mode` holds the run length, 0 means tile mode); the
 //    fractions pass marks the run holding each closest point's tricubic
 //    stencil base, N_r selection dilates the marks by the stencil (+0..3 in x
 //    and y) and compacts them on the device, and one lane sums one run.
@@ -14,7 +15,8 @@
 namespace svimx {
 
 constexpr int kTX = 4, kTY = 8, kTZ = 8;   // tile mode: one wave per x-layer of 8x8 nodes
-constexpr int kRunLen = 8;                 // run mode: nodes per z-run
+constexpr int kRunLen = 8;                 // run mode: default nodes per z-run
+constexpr int kRunLenMin = 4, kRunLenMax = 16;  // run lengths the tuner chooses from
 constexpr int kRunMaxM = 16;               // largest ball half-width handled in run mode (= kOccPad)
 
 struct NrColumn {
@@ -28,7 +30,7 @@ struct NrPlan {
   int m;
   double h, radius;
   Dims d;
-  int run_mode;
+  int run_mode;       // 0: tile mode; else the run length (4, 8 or 16)
   // tile mode
   int ntx, nty, ntz;
   int64_t ntiles;
@@ -51,6 +53,9 @@ struct NrPlan {
   int64_t nchunks;
 };
 
+// Run length of a run-mode plan (its arrays are sized for kRunLenMin).
+void nr_plan_set_run_len(NrPlan* p, int rl);
+
 // Tile mode: mark the tiles covering the tricubic stencil [base, base+3]^3.
 __device__ __forceinline__ void mark_stencil_tiles(uint8_t* marks, int nty, int ntz, int bx, int by, int bz) {
   int tx0 = bx / kTX, tx1 = (bx + 3) / kTX;
@@ -61,12 +66,13 @@ __device__ __forceinline__ void mark_stencil_tiles(uint8_t* marks, int nty, int 
       for (int tz = tz0; tz <= tz1; ++tz) marks[((int64_t)tx * nty + ty) * ntz + tz] = 1;
 }
 
-// Run mode: mark the runs of the stencil base line (x = bx, y = by) that the
-// stencil's z-range [bz, bz+3] touches; selection dilates in x and y.
-__device__ __forceinline__ void mark_stencil_runs(uint8_t* marks, int ny, int nzr, int bx, int by, int bz) {
+// Run mode: mark the runs (of rl nodes) of the stencil base line (x = bx,
+// y = by) that the stencil's z-range [bz, bz+3] touches; selection dilates in
+// x and y.
+__device__ __forceinline__ void mark_stencil_runs(uint8_t* marks, int ny, int nzr, int rl, int bx, int by, int bz) {
   int64_t line = ((int64_t)bx * ny + by) * nzr;
-  marks[line + bz / kRunLen] = 1;
-  marks[line + (bz + 3) / kRunLen] = 1;
+  marks[line + bz / rl] = 1;
+  marks[line + (bz + 3) / rl] = 1;
 }
 
 }  // namespace svimx

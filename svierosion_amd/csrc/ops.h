@@ -37,7 +37,7 @@ struct OccLayout {
     L.ny = ny;
     L.nz = nz;
     L.pitch_y = ny + 2 * kOccPad;
-    L.pitch_z = ((int64_t)nz + 2 * kOccPad + 8 + 3) / 4 * 4;
+    L.pitch_z = ((int64_t)nz + 2 * kOccPad + 16 + 3) / 4 * 4;
     return L;
   }
   __host__ __device__ int64_t size() const { return ((int64_t)nx + 2 * kOccPad) * pitch_y * pitch_z; }

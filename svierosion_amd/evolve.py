@@ -62,7 +62,7 @@ class ErosionSimulation:
 
     def __init__(self, levelset: LevelSet, law: SpeedLaw, cfl: float = 0.5, feedback_cfl: float = 1.0,
                  parabolic_cfl: float = 0.25, reinit_distance: float = 1.5, reinit_iterations: int = 20,
-                 cutoff=(4.0, 5.5), newton_steps: int = 1):
+                 cutoff=(4.0, 5.5), newton_steps: int = 1, autotune: bool = True):
         self.ls = levelset
         self.law = law
         self.cfl = float(cfl)
@@ -91,6 +91,10 @@ class ErosionSimulation:
             self.k2 = torch.zeros(n, dtype=torch.float32, device=dev)
         self._alloc_band_arrays()
         self._frac_valid = None     # buffer index whose fractions/marks are current
+        self.tuning = {}
+        if self.conv is not None and autotune and levelset.nb > 0:
+            # pick the N_r z-run length on this job's field (results do not depend on it)
+            self.tuning = self.conv.tune(levelset, self.cutoff[1])
         self.initial_volume = self.ls.volume()
         self.time = 0.0
         self.steps = 0

@@ -117,6 +117,25 @@ class BallConvolution:
         self.exec_marked(f, out, self._workspace())
         return out
 
+    @property
+    def run_length(self) -> int:
+        """Nodes per z-run of the direct summation (0: tile mode)."""
+        return int(_abi.lib.svimx_nr_plan_run_length(self.plan))
+
+    def tune(self, ls, cut_hi: float) -> dict:
+        """Autotune the summation's z-run length (4, 8 or 16 nodes) on the level
+        set's current field; returns {run length: ms per launch}.  The N_r
+        values the velocity reads do not depend on the choice.  Leaves the
+        occupancy and N_r marks of ``ls``'s current field in place."""
+        ms = (C.c_double * 3)()
+        field = torch.empty(ls.grid.size, dtype=torch.float32, device=self.device)
+        _abi.check(_abi.lib.svimx_nr_plan_tune(_abi.stream(), self.plan, _abi.ptr(ls.flat), _abi.ptr(ls._band),
+                                               ls.nb, *ls.dims, ls.h, float(cut_hi), _abi.ptr(ls.occupancy),
+                                               _abi.ptr(field), _abi.ptr(ls.ws), ls.ws.numel(), ms), "N_r tuning")
+        if self.run_length == 0:
+            return {}
+        return {4: ms[0], 8: ms[1], 16: ms[2]}
+
     def interior_volume(self, fractions: torch.Tensor, nodes: torch.Tensor | None = None) -> torch.Tensor:
         """Dense |B_r(x) cap Omega| (derived from N_r)."""
         return (1.0 - self.exterior_fraction(fractions, nodes).double()) * self.ball_volume

@@ -196,3 +196,28 @@ def test_reinitialize_preserves_interface(device):
         ex = np.linalg.norm(x - CTR, axis=1) - 1.0
         near = np.abs(ex) < 1.5 * h
         assert np.abs(ph - ex)[near].max() / h < tol
+
+
+def test_nr_run_length_tuning_is_result_neutral(device):
+    """The autotuner's z-run length changes which nodes are summed, never the
+    N_r values the velocity reads: histories and fields are bit-identical."""
+    from svierosion_amd.api import ErosionParams, make_grid
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.levelset import LevelSet
+    from svierosion_amd.sdf import mesh_to_sdf
+    from svierosion_amd.shapes import cube
+    m = cube(1.0)
+    p = ErosionParams(law="svi", radius=0.2, k=1, s=1, t=0.9, resolution=40)
+    g = make_grid(m, p)
+    phi = mesh_to_sdf(m, g)
+    sims = []
+    for tune in (False, True):
+        ls = LevelSet(phi.clone(), g, width=p.band)
+        sim = ErosionSimulation(ls, p.make_law(), autotune=tune)
+        sim.run(target_fraction=None, max_steps=8)
+        sims.append(sim)
+    a, b = sims
+    assert a.conv.run_length == 8 and b.conv.run_length in (4, 8, 16)
+    assert set(b.tuning) == {4, 8, 16} and all(v > 0 for v in b.tuning.values())
+    assert [(r.volume, r.dt) for r in a.history] == [(r.volume, r.dt) for r in b.history]
+    assert torch.equal(a.ls.flat, b.ls.flat)

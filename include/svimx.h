@@ -207,6 +207,19 @@ int svimx_sdf_build(void* stream, const float* tri_local, const int32_t* tri_org
                     double exact_band, float* phi_out, float* dist_out, int32_t* tid_out, void* ws,
                     int64_t ws_bytes);
 
+/* ---------------------------------------------------------------- point-mesh distance
+ * Unsigned distance (cells) from npts points (float32 index coordinates, 3
+ * per point) to a triangle mesh given, as for svimx_sdf_build, by split
+ * triangles in local frames (tri_local ntri x 9 float32 relative to tri_org
+ * ntri x 3 int32), binned into buckets of bucket_cells cells over the
+ * nx*ny*nz grid; the ring search stops on a lower bound, so the result is the
+ * exact minimum.  Allocates stream-ordered temporaries; synchronises.
+ * Replaces point_mesh_distance (svierosion/api.py:104-168) for the depth
+ * annotation (:171-194). */
+int svimx_point_mesh_distance(void* stream, const float* tri_local, const int32_t* tri_org, int64_t ntri, int nx,
+                              int ny, int nz, double bucket_cells, const float* points, int64_t npts,
+                              float* dist_out);
+
 /* ---------------------------------------------------------------- surface
  * Zero set of phi (band cells) as a watertight, outward triangle mesh by
  * marching tetrahedra on the Kuhn split, vertices shared per edge and moved

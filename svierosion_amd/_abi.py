@@ -93,6 +93,7 @@ _SIGNATURES = {
                              _P, _P, _P, _P, _I64]),
     "svimx_extract_surface": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    "svimx_point_mesh_distance": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _I64, _P]),
     "svimx_free": (_I, [_P]),
     "svimx_copy_to_host": (_I, [_P, _P, _I64]),
     "svimx_engine_create": (_I, [_P, _I, C.POINTER(_P)]),

@@ -23,7 +23,7 @@ from .flows import GaussCurvatureLaw, MeanCurvatureLaw, SpeedLaw, SVIPowerLaw
 from .grid import Grid, get_device
 from .levelset import MARGIN, LevelSet, tricubic
 from .meshio import TriMesh
-from .sdf import mesh_to_sdf, point_triangle_distance2
+from .sdf import mesh_to_sdf, triangle_distances, triangle_frames
 from .surface import extract_surface, sample_field
 
 
@@ -106,75 +106,33 @@ def make_grid(mesh: TriMesh, params: ErosionParams) -> Grid:
 
 
 def point_mesh_distance(points: np.ndarray, mesh: TriMesh, max_edge: float | None = None, k: int = 16,
-                        device=None, exact: bool = False) -> np.ndarray:
-    """Unsigned distance from points to a mesh: k-d tree candidates over triangle centroids
-    (bisected to <= max_edge), exact point-triangle distances on the device, and the
-    certified re-query of svierosion/api.py:104-168."""
-    from scipy.spatial import cKDTree
+                        device=None, exact: bool = True, grid: Grid | None = None,
+                        triangles: dict | None = None) -> np.ndarray:
+    """Unsigned distance from points to a mesh (svierosion/api.py:104-168).
+
+    Exact on the device (``svimx_point_mesh_distance``): the mesh's triangles,
+    split to edges <= 8 cells of ``grid`` and stored in triangle-local frames,
+    are binned into 4-cell buckets and every point searches rings of buckets
+    until a lower bound proves the minimum.  ``grid`` defaults to a frame
+    around the mesh and the points with the mesh's median edge as the cell;
+    ``triangles`` reuses frames already on the device.  ``max_edge``, ``k`` and
+    ``exact`` of the reference's k-d tree search are accepted and have no
+    effect (the result is always the certified minimum)."""
     dev = _abi.require_device(get_device(device))
-    points = np.asarray(points, dtype=np.float64)
-    tri = mesh.triangles()
-    e = np.linalg.norm(tri[:, 1] - tri[:, 0], axis=1)
-    max_edge = max(max_edge or 0.0, 2.0 * float(np.median(e)) if len(e) else 0.0)
-    tri = _bisect_long(tri, max_edge)
-    cen = tri.mean(1)
-    rho = float(np.linalg.norm(tri - cen[:, None, :], axis=2).max()) if len(tri) else 0.0
-    tree = cKDTree(cen)
-    ref = points.mean(0) if len(points) else np.zeros(3)
-    T = torch.as_tensor(tri - ref, dtype=torch.float32, device=dev)
-
-    def evaluate(pidx, cand):
-        res = np.empty(len(pidx))
-        step = max(1, (1 << 22) // max(cand.shape[1], 1))
-        for s0 in range(0, len(pidx), step):
-            P = torch.as_tensor(points[pidx[s0:s0 + step]] - ref, dtype=torch.float32, device=dev)
-            Cc = torch.as_tensor(cand[s0:s0 + step], device=dev)
-            t = T[Cc.clamp_min(0)]
-            d2 = point_triangle_distance2(P[:, None, :].expand(-1, Cc.shape[1], -1), t[:, :, 0], t[:, :, 1],
-                                          t[:, :, 2])
-            d2 = torch.where(Cc >= 0, d2, torch.full_like(d2, float("inf")))
-            res[s0:s0 + step] = d2.min(1).values.sqrt().double().cpu().numpy()
-        return res
-
-    out = np.full(len(points), np.inf)
-    todo = np.arange(len(points))
-    kk = min(k, len(tri))
-    while len(todo):
-        dk, cand = tree.query(points[todo], k=kk, workers=-1)
-        dk, cand = dk.reshape(len(todo), kk), cand.reshape(len(todo), kk)
-        out[todo] = np.minimum(out[todo], evaluate(todo, cand))
-        complete = (dk[:, -1] >= out[todo] + rho) | (kk >= len(tri))
-        todo = todo[~complete]
-        if kk >= 256 and len(todo):
-            if not exact:
-                break
-            lists = tree.query_ball_point(points[todo], out[todo] + rho, workers=-1)
-            width = max(len(l) for l in lists)
-            cand = np.full((len(todo), width), -1, dtype=np.int64)
-            for i, l in enumerate(lists):
-                cand[i, :len(l)] = l
-            out[todo] = np.minimum(out[todo], evaluate(todo, cand))
-            break
-        kk = min(4 * kk, len(tri))
-    return out
-
-
-def _bisect_long(tri: np.ndarray, max_edge: float) -> np.ndarray:
-    """Longest-edge bisection until every edge <= max_edge (svierosion/sdf.py:73-96)."""
-    done = []
-    while len(tri):
-        e = np.stack([np.linalg.norm(tri[:, (q + 1) % 3] - tri[:, (q + 2) % 3], axis=1) for q in range(3)], 1)
-        ok = e.max(1) <= max_edge
-        done.append(tri[ok])
-        big = tri[~ok]
-        if not len(big):
-            break
-        j = e[~ok].argmax(1)
-        r = np.arange(len(big))
-        a, b, c = big[r, j], big[r, (j + 1) % 3], big[r, (j + 2) % 3]
-        mid = 0.5 * (b + c)
-        tri = np.concatenate([np.stack([a, b, mid], 1), np.stack([a, mid, c], 1)])
-    return np.concatenate(done) if done else np.zeros((0, 3, 3))
+    points = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    if grid is None:
+        e = np.linalg.norm(mesh.vertices[mesh.faces[:, 1]] - mesh.vertices[mesh.faces[:, 0]], axis=1)
+        lo = np.minimum(mesh.vertices.min(0), points.min(0) if len(points) else mesh.vertices.min(0))
+        hi = np.maximum(mesh.vertices.max(0), points.max(0) if len(points) else mesh.vertices.max(0))
+        cell = max(float(np.median(e)) if len(e) else 1.0, float((hi - lo).max()) / 512.0, 1e-12)
+        shape = tuple(int(v) for v in np.ceil((hi - lo) / cell) + 3)
+        grid = Grid(tuple(lo - cell), cell, shape)
+        triangles = None
+    if triangles is None:
+        triangles = triangle_frames(mesh, grid, dev)
+    pidx = torch.as_tensor((points - np.asarray(grid.origin)) / grid.spacing, dtype=torch.float32, device=dev)
+    d = triangle_distances(triangles, grid, pidx)
+    return d.double().cpu().numpy() * grid.spacing
 
 
 def _stencil_nodes(p: torch.Tensor, shape) -> torch.Tensor:
@@ -187,13 +145,14 @@ def _stencil_nodes(p: torch.Tensor, shape) -> torch.Tensor:
     return torch.unique(q[:, 0] * (shape[1] * shape[2]) + q[:, 1] * shape[2] + q[:, 2])
 
 
-def _annotate(sim: ErosionSimulation, mesh: TriMesh, initial: TriMesh, law: SpeedLaw) -> None:
+def _annotate(sim: ErosionSimulation, mesh: TriMesh, initial: TriMesh, law: SpeedLaw,
+              triangles: dict | None = None) -> None:
     """Per-vertex depth, N_r or curvatures, and speed (svierosion/api.py:171-194)."""
     if len(mesh.vertices) == 0:
         return
     grid = sim.ls.grid
-    mesh.vertex_data["depth"] = point_mesh_distance(mesh.vertices, initial, max_edge=4 * grid.spacing,
-                                                    device=sim.ls.device).astype(np.float32)
+    mesh.vertex_data["depth"] = point_mesh_distance(mesh.vertices, initial, device=sim.ls.device, grid=grid,
+                                                    triangles=triangles).astype(np.float32)
     if law.requires_svi:
         p = torch.as_tensor(grid.to_index(mesh.vertices), dtype=torch.float32, device=sim.ls.device)
         N = sim.svi_field(_stencil_nodes(p, grid.shape))
@@ -242,6 +201,7 @@ class ErosionJob:
     stop_reason: str = ""
     fraction: float = 1.0
     tmesh: float = 0.0
+    triangles: dict = field(default_factory=dict)  # device split triangles of the input mesh
 
 
 def prepare_job(mesh: TriMesh, params: ErosionParams, verbose: bool = False) -> ErosionJob:
@@ -257,7 +217,8 @@ def prepare_job(mesh: TriMesh, params: ErosionParams, verbose: bool = False) -> 
     timings = {}
     t0 = time.time()
     grid = make_grid(mesh, params)
-    phi = mesh_to_sdf(mesh, grid, band=params.band, device=dev, verbose=verbose)
+    tris = {}
+    phi = mesh_to_sdf(mesh, grid, band=params.band, device=dev, verbose=verbose, keep_triangles=tris)
     ls = LevelSet(phi, grid, width=params.band)
     torch.cuda.current_stream(dev).synchronize()
     timings["sdf"] = time.time() - t0
@@ -275,7 +236,7 @@ def prepare_job(mesh: TriMesh, params: ErosionParams, verbose: bool = False) -> 
             print(f"[erode] max N_r on initial surface = {nmax:.4f} (t = {law.t})")
     targets = sorted({float(f) for f in params.snapshots if f > params.target_fraction},
                      reverse=True) + [params.target_fraction]
-    return ErosionJob(mesh, params, law, grid, ls, sim, targets, timings)
+    return ErosionJob(mesh, params, law, grid, ls, sim, targets, timings, triangles=tris)
 
 
 def finish_target(job: ErosionJob, stop_reason: str, fraction: float) -> bool:
@@ -283,7 +244,7 @@ def finish_target(job: ErosionJob, stop_reason: str, fraction: float) -> bool:
     job goes on to the next target (svierosion/api.py:230-245)."""
     tm = time.time()
     m = extract_surface(job.ls, project=job.params.project_vertices)
-    _annotate(job.sim, m, job.mesh, job.law)
+    _annotate(job.sim, m, job.mesh, job.law, job.triangles)
     job.tmesh += time.time() - tm
     job.snaps.append((fraction, job.sim.time, m))
     job.stop_reason = stop_reason

@@ -68,11 +68,13 @@ def _prepare(mesh: TriMesh, grid: Grid, exact_band: float):
 
 def mesh_to_sdf(mesh: TriMesh, grid: Grid, band: float = 6.0, exact_band: float = 2.0, device=None,
                 sign_axes=None, max_pairs: int | None = None, verbose: bool = False,
-                return_aux: bool = False):
+                return_aux: bool = False, keep_triangles: dict | None = None):
     """Signed distance of ``mesh`` on ``grid`` (negative inside), clamped to +-band*h.
 
     Returns a dense float32 device tensor of shape ``grid.shape``; with
     ``return_aux`` also the unclamped distance (cells) and closest split triangle.
+    ``keep_triangles`` (a dict) receives the device split triangles in local
+    frames ("local", "org"), reusable by :func:`triangle_distances`.
     """
     dev = _abi.require_device(get_device(device))
     t0 = time.time()
@@ -107,12 +109,41 @@ def mesh_to_sdf(mesh: TriMesh, grid: Grid, band: float = 6.0, exact_band: float 
                                         *map(int, grid.shape), float(grid.spacing), float(band), float(exact_band),
                                         _abi.ptr(phi), _abi.ptr(dist), _abi.ptr(tid), _abi.ptr(ws), ws.numel()),
                "mesh_to_sdf")
+    if keep_triangles is not None:
+        keep_triangles["local"], keep_triangles["org"] = d_local, d_org
     if verbose:
         print(f"[sdf] {len(mesh.faces)} triangles ({len(tri)} after splitting), {grid.describe()}: "
               f"host prep {t1 - t0:.2f}s, device {time.time() - t1:.2f}s")
     if return_aux:
         return phi, dist, tid
     return phi
+
+
+def triangle_frames(mesh: TriMesh, grid: Grid, device=None) -> dict:
+    """Split triangles of ``mesh`` (edges <= 8 cells) in triangle-local float32
+    frames on the device, in index coordinates of ``grid``."""
+    dev = _abi.require_device(get_device(device))
+    tri_idx = (mesh.vertices[mesh.faces] - np.asarray(grid.origin)) / grid.spacing
+    tri = split_long_triangles(tri_idx, 8.0)
+    org = np.floor(tri[:, 0]).astype(np.int64)
+    local = (tri - org[:, None, :]).astype(np.float32)
+    up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+    return {"local": up(local.reshape(-1, 9), np.float32), "org": up(org, np.int32)}
+
+
+def triangle_distances(triangles: dict, grid: Grid, points_index: torch.Tensor, bucket_cells: float = 4.0):
+    """Exact unsigned distance (cells) from index-space points (m, 3) to the
+    triangles of :func:`triangle_frames` / ``mesh_to_sdf(keep_triangles=...)``:
+    bucketed ring search on the device (``svimx_point_mesh_distance``)."""
+    loc, org = triangles["local"], triangles["org"]
+    p = points_index.to(loc.device, torch.float32).contiguous()
+    out = torch.empty(p.shape[0], dtype=torch.float32, device=loc.device)
+    if p.shape[0] == 0:
+        return out
+    _abi.check(_abi.lib.svimx_point_mesh_distance(_abi.stream(), _abi.ptr(loc), _abi.ptr(org), org.shape[0],
+                                                  *map(int, grid.shape), float(bucket_cells), _abi.ptr(p),
+                                                  p.shape[0], _abi.ptr(out)), "point-mesh distance")
+    return out
 
 
 def point_triangle_distance2(p: torch.Tensor, a: torch.Tensor, b: torch.Tensor, c: torch.Tensor) -> torch.Tensor:

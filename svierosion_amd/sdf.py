@@ -37,10 +37,19 @@ def split_long_triangles(tri: np.ndarray, max_edge: float) -> np.ndarray:
     return np.concatenate(done) if done else np.zeros((0, 3, 3))
 
 
+def _point_ids(points: np.ndarray) -> np.ndarray:
+    """Ids shared by exactly coincident points (n, 3): unique over the 24-byte
+    rows viewed as one opaque key (cheaper than a row-wise lexicographic
+    unique; the ids only group coincident points)."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    key = p.view(np.dtype((np.void, 24))).ravel()
+    _, inv = np.unique(key, return_inverse=True)
+    return inv.reshape(-1)
+
+
 def _vertex_ids(tri: np.ndarray) -> np.ndarray:
     """Ids of exactly coincident vertices of a triangle soup (n, 3)."""
-    _, inv = np.unique(tri.reshape(-1, 3), axis=0, return_inverse=True)
-    return inv.reshape(-1)[: 3 * len(tri)].reshape(-1, 3)
+    return _point_ids(tri.reshape(-1, 3))[: 3 * len(tri)].reshape(-1, 3)
 
 
 def _prepare(mesh: TriMesh, grid: Grid, exact_band: float):
@@ -54,12 +63,13 @@ def _prepare(mesh: TriMesh, grid: Grid, exact_band: float):
     hi = np.clip(np.floor(tri.max(1) + exact_band).astype(np.int64), 0, shape - 1)
     box = np.concatenate([lo, hi], 1).astype(np.int32)
     if len(tri) == len(mesh.faces):
-        _, inv = np.unique(mesh.vertices, axis=0, return_inverse=True)
-        vid = inv.reshape(-1)[mesh.faces]
+        vid = _point_ids(mesh.vertices)[mesh.faces]
     else:
         vid = _vertex_ids(tri)
-    flat_v = vid.reshape(-1)
-    order = np.argsort(flat_v, kind="stable")
+    flat_v = vid.reshape(-1).astype(np.int64)
+    # stable order of the triangle corners by vertex id (one sort of a combined key)
+    n_c = len(flat_v)
+    order = np.sort(flat_v * n_c + np.arange(n_c)) % n_c if n_c else np.zeros(0, np.int64)
     nv = int(flat_v.max()) + 1 if flat_v.size else 0
     deg = np.bincount(flat_v, minlength=nv)
     first = np.cumsum(deg) - deg

@@ -145,6 +145,14 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
 int svimx_nr_plan_destroy(void* plan);
 int svimx_nr_reset(void* stream, void* plan);
 int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t n);
+/* Evaluation method of a plan: 1 direct summation (at the marked nodes), 2
+ * FFT over the whole grid (the reference's circular convolution,
+ * svierosion/svi.py:84-93: forward 3-D FFT of f, product with the real
+ * spectrum of the ball, and a forward FFT of the conjugate; mixed-radix
+ * Stockham line transforms in LDS).  Plans of balls wider than 33 cells are
+ * created with the FFT.  The FFT allocates 12 bytes per grid node. */
+int svimx_nr_plan_set_method(void* plan, int method);
+int svimx_nr_plan_method(void* plan);
 /* Autotuner: times the run-mode summation with z-runs of 4, 8 and 16 nodes on
  * this job's field (fractions and selection recomputed per candidate; the
  * N_r values at the nodes the velocity reads are identical for every run

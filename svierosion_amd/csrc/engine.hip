@@ -322,7 +322,7 @@ struct EngJob {
   int64_t capped = 0;
 
   bool svi() const { return desc.law.kind == 0; }
-  bool run_plan() const { return plan && plan->run_mode; }
+  bool run_plan() const { return plan && plan->run_mode && !plan->fft; }
 };
 
 struct Engine {
@@ -377,7 +377,7 @@ static void fill_jobdev(const EngJob& e, JobDev& j) {
   carve_workspace(D.ws, D.ws_bytes, j.d.size(), w);
   j.results = w.results;
   j.partials = w.partials;
-  if (e.plan) {
+  if (e.plan && !e.plan->fft) {
     const NrPlan* p = e.plan;
     j.has_plan = 1;
     j.run_mode = p->run_mode;
@@ -448,7 +448,7 @@ static int upload(Engine& E) {
     seg_add(T.seg[T_JOBS], slot, 1);
     if (e.svi()) seg_add(T.seg[T_SVI], slot, reduce_grid(e.nb));
     else seg_add(T.seg[T_CURV], slot, reduce_grid(e.nb));
-    if (e.plan) seg_add(T.seg[T_MARKS], slot, (e.plan->nmarks + 255) / 256);
+    if (e.plan && !e.plan->fft) seg_add(T.seg[T_MARKS], slot, (e.plan->nmarks + 255) / 256);
     if (e.run_plan()) {
       seg_add(T.seg[T_RUNS], slot, (e.plan->nruns + 255) / 256);
       seg_add(T.seg[T_CHUNKS], slot, e.plan->nchunks);
@@ -506,7 +506,7 @@ static int prepare(Engine& E, int rel, int stats, int only_needed) {
   // tile-mode plans (balls wider than 16 cells) run their own launches
   for (int slot : E.set) {
     EngJob& e = E.jobs[slot];
-    if (!e.plan || e.plan->run_mode) continue;
+    if (!e.plan || e.run_plan()) continue;
     int rc = svimx_nr_select(E.st, e.plan);
     if (!rc) rc = svimx_nr_sum(E.st, e.plan, e.desc.occupancy, e.desc.field, e.desc.ws, e.desc.ws_bytes);
     if (rc) return rc;

@@ -1,0 +1,24 @@
+// FFT evaluation of N_r (see fft.hip): plan state and entry points used by
+// the N_r plan in nr.hip.
+#pragma once
+
+#include "common.h"
+
+namespace svimx {
+
+constexpr int kFftTileElems = 2048;  // (L + 1) * lines per tile; lines longer than 2047 are not supported
+
+struct FftPlan {
+  Dims d{0, 0, 0};
+  int len[3] = {0, 0, 0};
+  float2* C = nullptr;       // complex work array, nx*ny*nz
+  float* khat = nullptr;     // real spectrum of the ball kernel, nx*ny*nz
+  float2* tw[3] = {nullptr, nullptr, nullptr};  // twiddles exp(-2 pi i t / L) per axis
+};
+
+int fft_plan_create(FftPlan& F, const double* weights, int m, Dims d);
+void fft_plan_destroy(FftPlan& F);
+// N_r = 1 + negbv * scale * conv(f, K) at every node (dense nr_out).
+int fft_nr(hipStream_t st, const FftPlan& F, const float* occupancy, float* nr_out, float scale, float negbv);
+
+}  // namespace svimx

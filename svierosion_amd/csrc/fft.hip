@@ -1,0 +1,284 @@
+// N_r by FFT: the reference's method (svierosion/svi.py:66-93), for balls too
+// wide for the direct summation and as a selectable alternative.
+//
+// N_r = 1 - h^3 (w * f) / |B_r| with the circular convolution computed as
+//     conv = IFFT(FFT(f) * K^),   K = ball weights at wrapped offsets.
+// K is real and even (the weights depend on |dx|, |dy|, |dz|), so K^ is real,
+// stored as one float per node.  The inverse transform is a forward one on the
+// conjugate: conv = conj(FFT(conj(FFT(f) K^))) / N, and only Re(conv) is used,
+// so six forward line passes do the whole evaluation:
+//     z (real f from the padded occupancy) -> y -> x (epilogue: conj(. K^))
+//     -> x -> y -> z (epilogue: N_r = 1 + negbv h^3 Re(.) / N)
+// over one complex buffer (in place per line tile).
+//
+// Line transform: mixed-radix Stockham autosort (radices 4, 2, 3, 5, 7; every
+// grid extent is a product of 2, 3, 5, 7 by construction) on a tile of T lines
+// held in LDS (two ping-pong buffers, row pitch L+1 against bank conflicts).
+// Stage with radix r after m points: butterfly j of L/r (k = j mod m) reads
+// x_q = A[j + q L/r] w^(qk / (m r)), forms the r-point DFT y_p, and writes
+// B[(j / m) m r + k + p m].  Lines along y and x are tiled with consecutive z
+// so global loads and stores coalesce across the lines of a tile.  Twiddles
+// come from per-length tables computed in FP64 on the host.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "fft.h"
+
+namespace svimx {
+
+__constant__ float2 c_root[8][8];  // c_root[r][k] = exp(-2 pi i k / r), r in {2, 3, 4, 5, 7}
+
+enum FftIn { kInComplex = 0, kInOccupancy = 1 };
+enum FftOut { kOutComplex = 0, kOutConjMulK = 1, kOutNr = 2, kOutRealK = 3 };
+
+struct FftPass {
+  int axis;        // 0 x, 1 y, 2 z
+  int L;           // line length
+  int T;           // lines per tile
+  int nst;         // radix stages
+  int radix[16];
+  int64_t nlines;  // lines of this axis
+  const float2* tw;
+  Dims d;
+  OccLayout OL;
+  int in_mode, out_mode;
+  float coef;      // kOutNr: negbv * h^3 / N
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// element t of line l: dense offset (line indices fit in 32 bits)
+__device__ __forceinline__ int64_t line_offset(const FftPass& P, int64_t l, int t) {
+  const Dims& d = P.d;
+  if (P.axis == 2) return l * d.nz + t;
+  if (P.axis == 1) {
+    const uint32_t x = (uint32_t)l / (uint32_t)d.nz, z = (uint32_t)l - x * (uint32_t)d.nz;
+    return (int64_t)x * d.sx() + (int64_t)t * d.nz + z;
+  }
+  return l + (int64_t)t * d.sx();
+}
+
+__global__ void __launch_bounds__(256) k_fft_lines(FftPass P, float2* __restrict__ data,
+                                                   const float* __restrict__ occ, const float* __restrict__ khat,
+                                                   float* __restrict__ out, float* __restrict__ khat_out) {
+  extern __shared__ float2 lds[];
+  const int L = P.L, T = P.T, pitch = L + 1;
+  float2* A = lds;
+  float2* B = lds + T * pitch;
+  const int64_t l0 = (int64_t)blockIdx.x * T;
+  const int tl = (int)min<int64_t>(T, P.nlines - l0);
+  const int n = tl * L;
+  // load: z lines element-major within a line, y/x lines line-major (coalesced z)
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    int l, t;
+    if (P.axis == 2) { l = e / L; t = e - l * L; } else { t = e / tl; l = e - t * tl; }
+    const int64_t off = line_offset(P, l0 + l, t);
+    float2 v;
+    if (P.in_mode == kInOccupancy) {
+      const uint32_t line = (uint32_t)(l0 + l);  // axis z: line = x * ny + y
+      const int x = (int)(line / (uint32_t)P.d.ny), y = (int)(line - (uint32_t)x * (uint32_t)P.d.ny);
+      v = make_float2(occ[P.OL.at(x, y, t)], 0.0f);
+    } else {
+      v = data[off];
+    }
+    A[l * pitch + t] = v;
+  }
+  __syncthreads();
+  int m = 1;
+  for (int s = 0; s < P.nst; ++s) {
+    const int r = P.radix[s];
+    const int nb = L / r, mr = m * r, twstep = L / mr;
+    for (int b = threadIdx.x; b < tl * nb; b += blockDim.x) {
+      const int l = b / nb, j = b - l * nb, k = j % m;
+      float2 x[7];
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        if (q < r) {
+          const float2 a = A[l * pitch + j + q * nb];
+          x[q] = q ? cmul(a, P.tw[(q * k * twstep) % L]) : a;
+        }
+      }
+      const int base = l * pitch + (j / m) * mr + k;
+#pragma unroll
+      for (int p = 0; p < 7; ++p) {
+        if (p < r) {
+          float2 y = x[0];
+#pragma unroll
+          for (int q = 1; q < 7; ++q)
+            if (q < r) {
+              const float2 w = c_root[r][(p * q) % r];
+              y.x += x[q].x * w.x - x[q].y * w.y;
+              y.y += x[q].x * w.y + x[q].y * w.x;
+            }
+          B[base + p * m] = y;
+        }
+      }
+    }
+    __syncthreads();
+    float2* tmp = A;
+    A = B;
+    B = tmp;
+    m = mr;
+  }
+  // store with the pass epilogue
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    int l, t;
+    if (P.axis == 2) { l = e / L; t = e - l * L; } else { t = e / tl; l = e - t * tl; }
+    const int64_t off = line_offset(P, l0 + l, t);
+    const float2 v = A[l * pitch + t];
+    switch (P.out_mode) {
+      case kOutConjMulK: {
+        const float k = khat[off];
+        data[off] = make_float2(v.x * k, -v.y * k);
+        break;
+      }
+      case kOutNr: out[off] = 1.0f + P.coef * v.x; break;
+      case kOutRealK: khat_out[off] = v.x; break;
+      default: data[off] = v;
+    }
+  }
+}
+
+__global__ void k_fft_zero(float2* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_float2(0.0f, 0.0f);
+}
+
+// ball weights at wrapped offsets: K[(dx mod nx, dy mod ny, dz mod nz)] = w(dx, dy, dz)
+__global__ void k_fft_place_kernel(float2* C, const float* __restrict__ w, int m, Dims d) {
+  const int W = 2 * m + 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)W * W * W) return;
+  const int c = (int)(i % W), b = (int)((i / W) % W), a = (int)(i / ((int64_t)W * W));
+  const int x = ((a - m) % d.nx + d.nx) % d.nx, y = ((b - m) % d.ny + d.ny) % d.ny, z = ((c - m) % d.nz + d.nz) % d.nz;
+  C[((int64_t)x * d.ny + y) * d.nz + z] = make_float2(w[i], 0.0f);
+}
+
+static std::vector<int> fft_radices(int L) {
+  std::vector<int> r;
+  for (int q : {4, 2, 3, 5, 7})
+    while (L % q == 0) {
+      r.push_back(q);
+      L /= q;
+    }
+  if (L != 1) r.clear();
+  return r;
+}
+
+static bool roots_ready = false;
+
+static int upload_roots() {
+  if (roots_ready) return SVIMX_OK;
+  float2 h[8][8] = {};
+  for (int r : {2, 3, 4, 5, 7})
+    for (int k = 0; k < r; ++k) {
+      const double a = -2.0 * M_PI * k / r;
+      h[r][k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+  SVIMX_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_root), h, sizeof(h)));
+  roots_ready = true;
+  return SVIMX_OK;
+}
+
+static FftPass make_pass(const FftPlan& F, int axis, int in_mode, int out_mode, float coef) {
+  FftPass P{};
+  P.axis = axis;
+  P.L = F.len[axis];
+  const int L = P.L;
+  P.T = std::max(1, std::min(16, (kFftTileElems / (L + 1))));
+  const std::vector<int> r = fft_radices(L);
+  P.nst = (int)r.size();
+  for (int s = 0; s < P.nst; ++s) P.radix[s] = r[s];
+  const Dims& d = F.d;
+  P.nlines = axis == 2 ? (int64_t)d.nx * d.ny : (axis == 1 ? (int64_t)d.nx * d.nz : (int64_t)d.ny * d.nz);
+  P.tw = F.tw[axis];
+  P.d = d;
+  P.OL = OccLayout::of(d.nx, d.ny, d.nz);
+  P.in_mode = in_mode;
+  P.out_mode = out_mode;
+  P.coef = coef;
+  return P;
+}
+
+static int run_pass(hipStream_t st, const FftPass& P, float2* data, const float* occ, const float* khat, float* out,
+                    float* khat_out) {
+  const unsigned grid = (unsigned)((P.nlines + P.T - 1) / P.T);
+  const size_t lds = sizeof(float2) * 2 * (size_t)P.T * (P.L + 1);
+  hipLaunchKernelGGL(k_fft_lines, dim3(grid), dim3(256), lds, st, P, data, occ, khat, out, khat_out);
+  SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+}
+
+int fft_plan_create(FftPlan& F, const double* weights, int m, Dims d) {
+  F = FftPlan{};
+  F.d = d;
+  F.len[0] = d.nx;
+  F.len[1] = d.ny;
+  F.len[2] = d.nz;
+  for (int a = 0; a < 3; ++a) {
+    SVIMX_REQUIRE(F.len[a] >= 1 && F.len[a] + 1 <= kFftTileElems && !fft_radices(F.len[a]).empty(), SVIMX_EBADARG,
+                  "FFT needs grid extents that are products of 2, 3, 5, 7 and below %d (got %d)", kFftTileElems,
+                  F.len[a]);
+  }
+  int rc = upload_roots();
+  if (rc) return rc;
+  const int64_t n = d.size();
+  SVIMX_CHECK_HIP(hipMalloc(&F.C, sizeof(float2) * n));
+  SVIMX_CHECK_HIP(hipMalloc(&F.khat, sizeof(float) * n));
+  for (int a = 0; a < 3; ++a) {
+    const int L = F.len[a];
+    std::vector<float2> tw(L);
+    for (int t = 0; t < L; ++t) {
+      const double ang = -2.0 * M_PI * t / L;
+      tw[t] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    SVIMX_CHECK_HIP(hipMalloc(&F.tw[a], sizeof(float2) * L));
+    SVIMX_CHECK_HIP(hipMemcpy(F.tw[a], tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+  }
+  // K^ = Re FFT(K): place the weights, three passes, keep the real part
+  const int W = 2 * m + 1;
+  std::vector<float> wf((size_t)W * W * W);
+  for (size_t i = 0; i < wf.size(); ++i) wf[i] = (float)weights[i];
+  float* dw = nullptr;
+  SVIMX_CHECK_HIP(hipMalloc(&dw, sizeof(float) * wf.size()));
+  hipError_t e = hipMemcpy(dw, wf.data(), sizeof(float) * wf.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(dw);
+    SVIMX_CHECK_HIP(e);
+  }
+  hipStream_t st = nullptr;
+  hipLaunchKernelGGL(k_fft_zero, dim3(blocks_for(n, 256)), dim3(256), 0, st, F.C, n);
+  hipLaunchKernelGGL(k_fft_place_kernel, dim3(blocks_for((int64_t)W * W * W, 256)), dim3(256), 0, st, F.C, dw, m, d);
+  rc = run_pass(st, make_pass(F, 2, kInComplex, kOutComplex, 0.0f), F.C, nullptr, nullptr, nullptr, nullptr);
+  if (!rc) rc = run_pass(st, make_pass(F, 1, kInComplex, kOutComplex, 0.0f), F.C, nullptr, nullptr, nullptr, nullptr);
+  if (!rc) rc = run_pass(st, make_pass(F, 0, kInComplex, kOutRealK, 0.0f), F.C, nullptr, nullptr, nullptr, F.khat);
+  e = hipStreamSynchronize(st);
+  hipFree(dw);
+  if (rc) return rc;
+  SVIMX_CHECK_HIP(e);
+  return SVIMX_OK;
+}
+
+void fft_plan_destroy(FftPlan& F) {
+  hipFree(F.C);
+  hipFree(F.khat);
+  for (int a = 0; a < 3; ++a) hipFree(F.tw[a]);
+  F = FftPlan{};
+}
+
+int fft_nr(hipStream_t st, const FftPlan& F, const float* occupancy, float* nr_out, float scale, float negbv) {
+  const float coef = (float)((double)negbv * (double)scale / (double)F.d.size());
+  int rc = run_pass(st, make_pass(F, 2, kInOccupancy, kOutComplex, 0.0f), F.C, occupancy, nullptr, nullptr, nullptr);
+  if (!rc) rc = run_pass(st, make_pass(F, 1, kInComplex, kOutComplex, 0.0f), F.C, nullptr, nullptr, nullptr, nullptr);
+  if (!rc) rc = run_pass(st, make_pass(F, 0, kInComplex, kOutConjMulK, 0.0f), F.C, nullptr, F.khat, nullptr, nullptr);
+  if (!rc) rc = run_pass(st, make_pass(F, 0, kInComplex, kOutComplex, 0.0f), F.C, nullptr, nullptr, nullptr, nullptr);
+  if (!rc) rc = run_pass(st, make_pass(F, 1, kInComplex, kOutComplex, 0.0f), F.C, nullptr, nullptr, nullptr, nullptr);
+  if (!rc) rc = run_pass(st, make_pass(F, 2, kInComplex, kOutNr, coef), F.C, nullptr, nullptr, nr_out, nullptr);
+  return rc;
+}
+
+}  // namespace svimx

@@ -165,8 +165,8 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   SVIMX_REQUIRE(2 * m + 1 <= nx && 2 * m + 1 <= ny && 2 * m + 1 <= nz, SVIMX_EBADARG,
                 "ball kernel larger than the grid");
   const bool run_mode = m <= kRunMaxM;
-  SVIMX_REQUIRE(run_mode || sizeof(float) * 2 * (size_t)(kTY + 2 * m) * nr_pitch(m) <= 64 * 1024, SVIMX_EBADARG,
-                "ball radius of %d cells exceeds the LDS plane limit (m <= 33)", m);
+  // balls wider than the LDS plane of the tile kernel (m > 33) go to the FFT
+  const bool use_fft = !run_mode && sizeof(float) * 2 * (size_t)(kTY + 2 * m) * nr_pitch(m) > 64 * 1024;
   const int W = 2 * m + 1;
   std::vector<NrColumn> cols;
   std::vector<int> col_start(W + 1, 0);
@@ -196,6 +196,7 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   col_start[W] = (int)cols.size();
   NrPlan* p = new NrPlan();
   p->m = m;
+  p->weights_host.assign(weights, weights + (size_t)W * W * W);
   p->h = h;
   p->radius = radius;
   p->d = Dims{nx, ny, nz};
@@ -238,9 +239,47 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
     delete p;
     return SVIMX_EDEVICE;
   }
+  if (use_fft) {
+    p->fft = 1;
+    p->nmarks = 0;
+    int rc = fft_plan_create(p->fftp, weights, m, p->d);
+    if (rc) {
+      svimx_nr_plan_destroy(p);
+      return rc;
+    }
+  }
   *plan_out = p;
   return SVIMX_OK;
   SVIMX_TRY_END
+}
+
+int svimx_nr_plan_set_method(void* plan, int method) {
+  SVIMX_TRY_BEGIN
+  NrPlan* p = static_cast<NrPlan*>(plan);
+  SVIMX_REQUIRE(p && (method == 1 || method == 2), SVIMX_EBADARG, "method must be 1 (direct) or 2 (FFT)");
+  if (method == 2 && !p->fft) {
+    SVIMX_REQUIRE(p->weights_host.size() == (size_t)(2 * p->m + 1) * (2 * p->m + 1) * (2 * p->m + 1), SVIMX_EBADARG,
+                  "plan has no weights");
+    int rc = fft_plan_create(p->fftp, p->weights_host.data(), p->m, p->d);
+    if (rc) return rc;
+    p->fft = 1;
+    p->nmarks = 0;
+  } else if (method == 1 && p->fft) {
+    SVIMX_REQUIRE(p->m <= 33, SVIMX_EBADARG, "ball radius of %d cells needs the FFT", p->m);
+    fft_plan_destroy(p->fftp);
+    p->fft = 0;
+    if (p->run_mode)
+      nr_plan_set_run_len(p, p->run_mode);
+    else
+      p->nmarks = p->ntiles;
+  }
+  return SVIMX_OK;
+  SVIMX_TRY_END
+}
+
+int svimx_nr_plan_method(void* plan) {
+  const NrPlan* p = static_cast<const NrPlan*>(plan);
+  return p ? (p->fft ? 2 : 1) : 0;
 }
 
 int svimx_nr_plan_destroy(void* plan) {
@@ -254,6 +293,7 @@ int svimx_nr_plan_destroy(void* plan) {
   hipFree(p->list);
   hipFree(p->count);
   hipFree(p->ccounts);
+  if (p->fft) fft_plan_destroy(p->fftp);
   delete p;
   return SVIMX_OK;
 }
@@ -261,6 +301,7 @@ int svimx_nr_plan_destroy(void* plan) {
 int svimx_nr_reset(void* stream, void* plan) {
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p, SVIMX_EBADARG, "null plan");
+  if (p->fft) return SVIMX_OK;
   hipLaunchKernelGGL(k_nr_clear_marks, dim3(blocks_for(p->nmarks, 256)), dim3(256), 0, as_stream(stream), p->marks,
                      p->nmarks);
   SVIMX_CHECK_LAUNCH();
@@ -270,7 +311,7 @@ int svimx_nr_reset(void* stream, void* plan) {
 int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t n) {
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p, SVIMX_EBADARG, "null plan");
-  if (n <= 0) return SVIMX_OK;
+  if (n <= 0 || p->fft) return SVIMX_OK;
   hipLaunchKernelGGL(k_nr_mark_nodes, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), p->marks,
                      p->run_mode, p->nty, p->ntz, p->nzr, p->d, nodes, n);
   SVIMX_CHECK_LAUNCH();
@@ -281,6 +322,7 @@ int svimx_nr_select(void* stream, void* plan) {
   SVIMX_TRY_BEGIN
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p, SVIMX_EBADARG, "null plan");
+  if (p->fft) return SVIMX_OK;  // the FFT evaluates every node
   hipStream_t st = as_stream(stream);
   if (p->run_mode) {
     hipLaunchKernelGGL(k_runs_dilate_y, dim3(blocks_for(p->nruns, 256)), dim3(256), 0, st, p->marks, p->dil,
@@ -316,6 +358,7 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
   if (!carve_workspace(ws, ws_bytes, p->d.size(), w)) return SVIMX_EBADARG;
   hipStream_t st = as_stream(stream);
   const OccLayout OL = OccLayout::of(p->d.nx, p->d.ny, p->d.nz);
+  if (p->fft) return fft_nr(st, p->fftp, occupancy, nr_out, p->scale, p->negbv);
   if (p->run_mode) {
     const int64_t* total = p->ccounts + p->nchunks;
     // grid: a multiple of the XCD count (the kernel's block remapping needs it)
@@ -357,7 +400,7 @@ int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t
   SVIMX_TRY_BEGIN
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p && phi && band && occupancy && nr_out, SVIMX_EBADARG, "null argument");
-  if (!p->run_mode) return SVIMX_OK;  // tile mode has a single shape
+  if (!p->run_mode || p->fft) return SVIMX_OK;  // tile mode and the FFT have a single shape
   hipStream_t st = as_stream(stream);
   const int cands[3] = {4, 8, 16};
   hipEvent_t ev[2];

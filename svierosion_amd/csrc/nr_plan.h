@@ -10,7 +10,10 @@
 //  * tiles (larger balls): 4x8x8 tiles streamed through LDS x-plane by x-plane.
 #pragma once
 
+#include <vector>
+
 #include "common.h"
+#include "fft.h"
 
 namespace svimx {
 
@@ -51,6 +54,9 @@ struct NrPlan {
   int32_t* count;     // device, 1 (tile mode)
   int64_t* ccounts;   // device, compaction chunk counts + total (run mode)
   int64_t nchunks;
+  int fft;            // 1: N_r by FFT over the whole grid (no marks, no selection)
+  FftPlan fftp;
+  std::vector<double> weights_host;  // the (2m+1)^3 weights, for switching to the FFT
 };
 
 // Run length of a run-mode plan (its arrays are sized for kRunLenMin).

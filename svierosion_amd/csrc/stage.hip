@@ -134,6 +134,7 @@ int svimx_fractions(void* stream, const float* phi, const int32_t* band, int64_t
   NrPlan* plan = static_cast<NrPlan*>(nr_plan);
   uint8_t* marks = nullptr;
   int nty = 0, ntz = 0, nzr = 0, run_mode = 0;
+  if (plan && plan->fft) plan = nullptr;  // the FFT needs no marks
   if (plan) {
     SVIMX_REQUIRE(plan->d.nx == nx && plan->d.ny == ny && plan->d.nz == nz, SVIMX_EBADARG,
                   "N_r plan grid does not match");

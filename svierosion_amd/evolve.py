@@ -62,7 +62,7 @@ class ErosionSimulation:
 
     def __init__(self, levelset: LevelSet, law: SpeedLaw, cfl: float = 0.5, feedback_cfl: float = 1.0,
                  parabolic_cfl: float = 0.25, reinit_distance: float = 1.5, reinit_iterations: int = 20,
-                 cutoff=(4.0, 5.5), newton_steps: int = 1, autotune: bool = True):
+                 cutoff=(4.0, 5.5), newton_steps: int = 1, autotune: bool = True, nr_method: str = "auto"):
         self.ls = levelset
         self.law = law
         self.cfl = float(cfl)
@@ -83,7 +83,7 @@ class ErosionSimulation:
         self.field = None
         self.k1 = self.k2 = None
         if law.requires_svi:
-            self.conv = BallConvolution(levelset.grid, law.radius, device=dev)
+            self.conv = BallConvolution(levelset.grid, law.radius, device=dev, method=nr_method)
             self._check_convolution_margin()
             self.field = torch.zeros(n, dtype=torch.float32, device=dev)
         else:

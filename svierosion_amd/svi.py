@@ -64,7 +64,7 @@ class BallConvolution:
     (4/3 pi r^3, exact) and ``quadrature_volume`` (h^3 sum w).
     """
 
-    def __init__(self, grid: Grid, radius: float, device=None, quad: int = 32):
+    def __init__(self, grid: Grid, radius: float, device=None, quad: int = 32, method: str = "auto"):
         self.grid = grid
         self.radius = float(radius)
         self.device = _abi.require_device(device if device is not None else "cuda")
@@ -82,6 +82,15 @@ class BallConvolution:
                    "N_r plan")
         self.plan = handle
         self._ws = None
+        if method not in ("auto", "direct", "fft"):
+            raise ValueError("method must be 'auto', 'direct' or 'fft'")
+        if method != "auto":
+            _abi.check(_abi.lib.svimx_nr_plan_set_method(handle, 1 if method == "direct" else 2), "N_r method")
+
+    @property
+    def method(self) -> str:
+        """'direct' (summation at the needed nodes) or 'fft' (whole grid)."""
+        return "fft" if _abi.lib.svimx_nr_plan_method(self.plan) == 2 else "direct"
 
     def __del__(self):
         plan = getattr(self, "plan", None)

@@ -21,6 +21,8 @@ def _case(name):
         return icosphere(3, 0.8), ErosionParams(law="mean", alpha=1.0, resolution=32), 0.96, 400
     if name == "gauss":
         return cube(1.0), ErosionParams(law="gauss", alpha=0.25, resolution=24), 0.97, 40
+    if name == "svi_fft":
+        return cube(1.0), ErosionParams(law="svi", radius=0.2, k=1, s=0, resolution=40), 0.95, 400
     raise KeyError(name)
 
 
@@ -41,7 +43,7 @@ def _sim(name):
     law = p.make_law(length_scale=float((hi - lo).max()))
     ls = LevelSet(_PHI[name].clone(), grid, width=p.band)
     sim = ErosionSimulation(ls, law, cfl=p.cfl, feedback_cfl=p.feedback_cfl, reinit_distance=p.reinit_distance,
-                            reinit_iterations=p.reinit_iterations)
+                            reinit_iterations=p.reinit_iterations, nr_method="fft" if name == "svi_fft" else "auto")
     return sim, target, max_steps
 
 
@@ -80,7 +82,7 @@ def _engine_run(names, graphs):
     return out
 
 
-@pytest.mark.parametrize("name", ["svi_cube", "svi_sphere_k2", "mean", "gauss"])
+@pytest.mark.parametrize("name", ["svi_cube", "svi_sphere_k2", "mean", "gauss", "svi_fft"])
 def test_engine_job_equals_single_job_path(device, name):
     sim, target, max_steps = _sim(name)
     r = sim.run(target_fraction=target, max_steps=max_steps)
@@ -95,7 +97,7 @@ def test_engine_job_equals_single_job_path(device, name):
 
 
 def test_mix_equals_alone_with_graphs(device):
-    names = ["svi_cube", "mean", "svi_sphere_k2", "gauss"]
+    names = ["svi_cube", "mean", "svi_sphere_k2", "gauss", "svi_fft"]
     mix = _engine_run(names, graphs=True)
     for nm in names:
         alone = _engine_run([nm], graphs=False)[nm]

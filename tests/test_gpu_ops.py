@@ -221,3 +221,43 @@ def test_nr_run_length_tuning_is_result_neutral(device):
     assert set(b.tuning) == {4, 8, 16} and all(v > 0 for v in b.tuning.values())
     assert [(r.volume, r.dt) for r in a.history] == [(r.volume, r.dt) for r in b.history]
     assert torch.equal(a.ls.flat, b.ls.flat)
+
+
+def test_nr_fft_matches_direct_and_oracle(device):
+    """The FFT method (the reference's circular convolution) against the oracle's
+    FFT (2e-5 abs) and the direct summation at the same nodes (2e-5 abs)."""
+    from svierosion_amd.svi import BallConvolution
+    ls, ref, g = _pair(distort=True, R=1.0, h=0.05, pad=0.6)
+    r = 0.3
+    fft = BallConvolution(g, r, device="cuda", method="fft")
+    direct = BallConvolution(g, r, device="cuda", method="direct")
+    assert fft.method == "fft" and direct.method == "direct"
+    occ = ls.dense_fractions()
+    Nf = fft.exterior_fraction(occ).cpu().numpy()
+    Nd = direct.exterior_fraction(occ).cpu().numpy()
+    f = ref.dense_fractions()
+    Nref = F.exterior_fraction_fft(f, F.ball_kernel_hat(g.shape, F.ball_cell_weights(r, g.spacing)), g.spacing, r)
+    m = fft.m
+    inner = tuple(slice(m + 1, n - m - 1) for n in g.shape)
+    assert np.abs(Nf[inner] - Nref[inner]).max() < 2e-5
+    assert np.abs(Nf[inner] - Nd[inner]).max() < 2e-5
+
+
+def test_nr_fft_wide_ball_sphere_closed_form(device):
+    """r/h = 40 (ball half-width 41 cells, beyond the direct kernels): the plan
+    uses the FFT; N_r at interface closest points vs 1/2 + 3r/(16R)."""
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.levelset import LevelSet, tricubic
+    from svierosion_amd.svi import BallConvolution
+    R, r = 1.0, 0.4
+    h = r / 40
+    phi, origin, h, shape = sphere_phi(R, h, r + 10 * h)
+    g = Grid(origin, h, shape)
+    ls = LevelSet(torch.from_numpy(phi.astype(np.float32)).cuda(), g)
+    conv = BallConvolution(g, r, device="cuda")
+    assert conv.m > 33 and conv.method == "fft"
+    pts = ls.closest_points(ls.interface_nodes())
+    N = conv.exterior_fraction(ls.dense_fractions())
+    vals = tricubic(N.view(-1), shape, pts)
+    err = (vals.double() - F.sphere_exterior_fraction(r, R)).abs().max().item()
+    assert err < 6e-5

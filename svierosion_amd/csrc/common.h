@@ -132,23 +132,53 @@ struct ReduceSpec {
 int launch_finalize(hipStream_t st, const double* partials, int64_t nblocks, double* results,
                     const ReduceSpec& spec);
 
-// Body of the finalisation (one block of 256 threads): slot by slot, thread t
-// folds partial blocks t, t+256, ... and the block reduction combines them.
+// Body of the finalisation (one block of up to 256 threads): every thread
+// folds whole 16-slot partial rows b = t, t+256, ... (contiguous 128-byte
+// reads), each wave reduces its slots with the xor butterfly, and thread s
+// combines the waves' values of slot s in wave order -- the same combination
+// order per slot as a slot-by-slot block reduction, in one pass.
+__device__ __forceinline__ double fold_op(int op, double a, double b) {
+  return op == 0 ? a + b : (op == 1 ? fmax(a, b) : fmin(a, b));
+}
+
 __device__ __forceinline__ void finalize_body(const double* __restrict__ partials, int64_t nblocks,
                                               double* __restrict__ results, const ReduceSpec& spec) {
-  __shared__ double red[4];
+  __shared__ double red[4][kPartialSlots];
+  double v[kPartialSlots];
+#pragma unroll
   for (int s = 0; s < kPartialSlots; ++s) {
-    int op = spec.op[s];
-    if (op < 0) continue;
-    double v = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
-    for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
-      double u = partials[b * kPartialSlots + s];
-      v = op == 0 ? v + u : (op == 1 ? fmax(v, u) : fmin(v, u));
-    }
-    double r = op == 0 ? block_reduce<0>(v, red) : (op == 1 ? block_reduce<1>(v, red) : block_reduce<2>(v, red));
-    if (threadIdx.x == 0) results[spec.dst[s]] = r;
-    __syncthreads();
+    const int op = spec.op[s];
+    v[s] = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
   }
+  for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    const double2* row = reinterpret_cast<const double2*>(partials + b * kPartialSlots);
+#pragma unroll
+    for (int q = 0; q < kPartialSlots / 2; ++q) {
+      const double2 u = row[q];
+      if (spec.op[2 * q] >= 0) v[2 * q] = fold_op(spec.op[2 * q], v[2 * q], u.x);
+      if (spec.op[2 * q + 1] >= 0) v[2 * q + 1] = fold_op(spec.op[2 * q + 1], v[2 * q + 1], u.y);
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int s = 0; s < kPartialSlots; ++s) {
+    const int op = spec.op[s];
+    if (op < 0) continue;
+    for (int o = 32; o > 0; o >>= 1) v[s] = fold_op(op, v[s], __shfl_xor(v[s], o, 64));
+  }
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int s = 0; s < kPartialSlots; ++s) red[wave][s] = v[s];
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  if (threadIdx.x < kPartialSlots && spec.op[threadIdx.x] >= 0) {
+    const int s = threadIdx.x, op = spec.op[s];
+    double r = red[0][s];
+    for (int w = 1; w < nw; ++w) r = fold_op(op, r, red[w][s]);
+    results[spec.dst[s]] = r;
+  }
+  __syncthreads();
 }
 
 }  // namespace svimx

@@ -153,17 +153,26 @@ int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t 
  * created with the FFT.  The FFT allocates 12 bytes per grid node. */
 int svimx_nr_plan_set_method(void* plan, int method);
 int svimx_nr_plan_method(void* plan);
-/* Autotuner: times the run-mode summation with z-runs of 4, 8 and 16 nodes on
- * this job's field (fractions and selection recomputed per candidate; the
- * N_r values at the nodes the velocity reads are identical for every run
- * length) and keeps the fastest.  ms_out (3 doubles, may be NULL) receives the
+/* Autotuner: times the run-mode summation with z-runs of 4, 8 and 16 nodes,
+ * each with per-lane windows read through the L1 and with the occupancy
+ * staged through LDS, on this job's field (fractions and selection recomputed
+ * per candidate; the N_r values at the nodes the velocity reads are identical
+ * for every candidate) and keeps the fastest.  ms_out (6 doubles: run lengths
+ * 4, 8, 16 without, then with LDS staging; may be NULL) receives the
  * per-launch times.  Leaves occupancy, marks and run list those of the chosen
  * length for phi.  Synchronises.  No counterpart in the reference. */
 int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
                        int nz, double h, double cut_hi, float* occupancy, float* nr_out, void* ws, int64_t ws_bytes,
                        double* ms_out);
-/* Run length of a run-mode plan (0: tile mode). */
+/* Run length of a run-mode plan (0: tile mode); the setter takes 4, 8 or 16
+ * and invalidates the marks (reset and mark again before the next sum). */
 int svimx_nr_plan_run_length(void* plan);
+int svimx_nr_plan_set_run_length(void* plan, int rl);
+/* Run mode: 1 stages the occupancy through LDS (clusters of runs per wave),
+ * 0 reads each run's windows through the L1.  Same sums in the same order,
+ * so the results are bit-identical.  No counterpart in the reference. */
+int svimx_nr_plan_set_staging(void* plan, int lds);
+int svimx_nr_plan_staging(void* plan);
 int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws,
                   int64_t ws_bytes);
 /* The two halves of svimx_nr_exec: compact the marked tiles into the plan's

@@ -36,6 +36,7 @@ namespace svimx {
 
 constexpr int kMaxEngineJobs = 256;
 constexpr int kEngGrid = 2048;  // persistent grid (multiple of the XCD count)
+constexpr int kEngGridWaves = 2560;  // one-wave blocks of the LDS N_r family: 10 per CU
 
 struct JobDev {
   float* buf[3];
@@ -64,8 +65,10 @@ struct JobDev {
   int64_t* ccounts;
   int64_t nruns, nchunks;
   const NrColumn* cols;
+  const int* col_start;
   const float* weights;
   float scale, negbv;
+  int m, lds;
   // dt control (FP64)
   double hd, cfl, feedback, parabolic, radius, time, max_time;
   int requires_svi, requires_curv, has_max_time, pad;
@@ -77,8 +80,9 @@ struct Seg {
   int32_t slot[kMaxEngineJobs];
 };
 
-// N_r tables: T_NR + 3 * (MP / 4 - 1) + (0, 1, 2 for run lengths 4, 8, 16)
-enum SegId { T_ALL, T_SVI, T_MARKS, T_RUNS, T_CHUNKS, T_JOBS, T_RUNJOBS, T_CURV, T_NR, T_COUNT = T_NR + 12 };
+// N_r tables: T_NR + 3 * (MP / 4 - 1) + (0, 1, 2 for run lengths 4, 8, 16),
+// then the same twelve for LDS staging from T_NR + 12
+enum SegId { T_ALL, T_SVI, T_MARKS, T_RUNS, T_CHUNKS, T_JOBS, T_RUNJOBS, T_CURV, T_NR, T_COUNT = T_NR + 24 };
 
 struct Tables {
   Seg seg[T_COUNT];
@@ -184,6 +188,20 @@ __global__ void __launch_bounds__(256) k_eng_nr_runs(const JobDev* __restrict__ 
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
     nr_runs_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.weights,
                      j.scale, j.negbv, const_cast<float*>(j.field), xcd_slice(lb, G), G);
+  }
+}
+
+// LDS-staged variant: one wave per block (the job grid counts waves)
+template <int MP, int RL>
+__global__ void __launch_bounds__(64) k_eng_nr_lds(const JobDev* __restrict__ J, const Seg* S) {
+  __shared__ float4 slab[kNrLdsCap4];
+  __shared__ NrLdsClu clu[kNrLdsClusters];
+  ENG_FOR(S) {
+    const JobDev& j = J[jslot];
+    const int64_t nrun = j.ccounts[j.nchunks];
+    if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
+    nr_lds_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.col_start, j.m,
+                        j.weights, j.scale, j.negbv, const_cast<float*>(j.field), slab, clu, xcd_slice(lb, G), G);
   }
 }
 
@@ -393,6 +411,9 @@ static void fill_jobdev(const EngJob& e, JobDev& j) {
     j.nruns = p->nruns;
     j.nchunks = p->nchunks;
     j.cols = p->cols;
+    j.col_start = p->col_start;
+    j.m = p->m;
+    j.lds = p->lds;
     j.weights = p->weights;
     j.scale = p->scale;
     j.negbv = p->negbv;
@@ -415,8 +436,9 @@ static int mp_of(const NrPlan* p) { return std::max(4, (p->m + 3) / 4 * 4); }
 static int64_t nr_grid(const EngJob& e) {
   // runs are not known on the host before the selection: size by the band
   const int64_t est = std::max<int64_t>(e.nb / 4, 256);
-  int64_t g = (est + 255) / 256;
-  g = std::min<int64_t>(std::max<int64_t>(g, kXcds), 2048);
+  const int threads = e.plan->lds ? 64 : 256;  // the LDS variant runs one wave per block
+  int64_t g = (est + threads - 1) / threads;
+  g = std::min<int64_t>(std::max<int64_t>(g, kXcds), e.plan->lds ? 2560 : 2048);
   return (g + kXcds - 1) / kXcds * kXcds;
 }
 
@@ -454,7 +476,7 @@ static int upload(Engine& E) {
       seg_add(T.seg[T_CHUNKS], slot, e.plan->nchunks);
       seg_add(T.seg[T_RUNJOBS], slot, 1);
       const int rl = e.plan->run_mode;
-      const int t = T_NR + 3 * (mp_of(e.plan) / 4 - 1) + (rl == 4 ? 0 : rl == 8 ? 1 : 2);
+      const int t = T_NR + 12 * (e.plan->lds ? 1 : 0) + 3 * (mp_of(e.plan) / 4 - 1) + (rl == 4 ? 0 : rl == 8 ? 1 : 2);
       seg_add(T.seg[t], slot, nr_grid(e));
     }
   }
@@ -473,7 +495,7 @@ static unsigned grid_of(const Engine& E, int t) {
     // persistent grid: a captured graph keeps its grid while the band sizes
     // change, so it gets headroom over the totals at capture time
     g = E.graphs ? sg.total + sg.total / 2 + 64 : sg.total;
-    g = std::min<int64_t>(g, kEngGrid);
+    g = std::min<int64_t>(g, t >= T_NR + 12 ? kEngGridWaves : kEngGrid);
   }
   g = std::max<int64_t>(g, 1);
   return (unsigned)((g + kXcds - 1) / kXcds * kXcds);
@@ -503,6 +525,12 @@ static int prepare(Engine& E, int rel, int stats, int only_needed) {
   ENG_NR(12, 4, 6); ENG_NR(12, 8, 7); ENG_NR(12, 16, 8);
   ENG_NR(16, 4, 9); ENG_NR(16, 8, 10); ENG_NR(16, 16, 11);
 #undef ENG_NR
+#define ENG_NRL(MPV, RLV, I) ENG_LAUNCH((k_eng_nr_lds<MPV, RLV>), T_NR + 12 + (I), 64)
+  ENG_NRL(4, 4, 0); ENG_NRL(4, 8, 1); ENG_NRL(4, 16, 2);
+  ENG_NRL(8, 4, 3); ENG_NRL(8, 8, 4); ENG_NRL(8, 16, 5);
+  ENG_NRL(12, 4, 6); ENG_NRL(12, 8, 7); ENG_NRL(12, 16, 8);
+  ENG_NRL(16, 4, 9); ENG_NRL(16, 8, 10); ENG_NRL(16, 16, 11);
+#undef ENG_NRL
   // tile-mode plans (balls wider than 16 cells) run their own launches
   for (int slot : E.set) {
     EngJob& e = E.jobs[slot];

@@ -77,6 +77,21 @@ __global__ void __launch_bounds__(256)
                    gridDim.x);
 }
 
+// The same sums with the occupancy staged through LDS: one wave per block.
+template <int MP, int RL>
+__global__ void __launch_bounds__(64)
+    k_nr_lds(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
+             const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
+             int m, const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
+             double* __restrict__ results) {
+  __shared__ float4 slab[kNrLdsCap4];
+  __shared__ NrLdsClu clu[kNrLdsClusters];
+  const int64_t nrun = *total;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
+  nr_lds_body<MP, RL>(occ, OL, nzr, list, nrun, cols, col_start, m, w, scale, negbv, out, slab, clu,
+                      xcd_slice(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 // ---------------------------------------------------------------- tile-mode summation
 // Row pitch of an LDS plane: >= 8+2m and = 8 (mod 32), so the 4 y-rows of a
 // half-wave fall in disjoint bank groups.
@@ -201,6 +216,8 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   p->radius = radius;
   p->d = Dims{nx, ny, nz};
   p->run_mode = 0;
+  p->lds = 0;
+  p->lds_ok = OccLayout::of(nx, ny, nz).size() / 4 < (int64_t)INT32_MAX ? 1 : 0;
   p->ntx = (nx + kTX - 1) / kTX;
   p->nty = (ny + kTY - 1) / kTY;
   p->ntz = (nz + kTZ - 1) / kTZ;
@@ -364,9 +381,15 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
     // grid: a multiple of the XCD count (the kernel's block remapping needs it)
     const int grid = (int)std::min<int64_t>((p->nruns + 255) / 256 + kXcds - 1, 256 * 8) / kXcds * kXcds;
     const int mp = (p->m + 3) / 4 * 4;
+    // LDS variant: one wave per block, at most 10 resident per CU (LDS-bound)
+    const int grid_lds = (int)std::min<int64_t>((p->nruns + 63) / 64 + kXcds - 1, 256 * 10) / kXcds * kXcds;
 #define SVIMX_RUNS(MPV, RLV)                                                                                       \
-  hipLaunchKernelGGL((k_nr_runs<MPV, RLV>), dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, total,   \
-                     p->cols, p->ncols, p->weights, p->scale, p->negbv, nr_out, w.results)
+  if (p->lds)                                                                                                      \
+    hipLaunchKernelGGL((k_nr_lds<MPV, RLV>), dim3(grid_lds), dim3(64), 0, st, occupancy, OL, p->nzr, p->list,      \
+                       total, p->cols, p->col_start, p->m, p->weights, p->scale, p->negbv, nr_out, w.results);    \
+  else                                                                                                             \
+    hipLaunchKernelGGL((k_nr_runs<MPV, RLV>), dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, total, \
+                       p->cols, p->ncols, p->weights, p->scale, p->negbv, nr_out, w.results)
 #define SVIMX_RUNS_MP(RLV)                  \
   switch (mp) {                             \
     case 0:                                 \
@@ -402,14 +425,18 @@ int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t
   SVIMX_REQUIRE(p && phi && band && occupancy && nr_out, SVIMX_EBADARG, "null argument");
   if (!p->run_mode || p->fft) return SVIMX_OK;  // tile mode and the FFT have a single shape
   hipStream_t st = as_stream(stream);
-  const int cands[3] = {4, 8, 16};
+  // run lengths 4, 8, 16 with global windows, then the same with LDS staging
+  const int cands[kNrTuneCands] = {4, 8, 16, 4, 8, 16};
   hipEvent_t ev[2];
   SVIMX_CHECK_HIP(hipEventCreate(&ev[0]));
   SVIMX_CHECK_HIP(hipEventCreate(&ev[1]));
-  int best_rl = p->run_mode, rc = SVIMX_OK;
+  int best_rl = p->run_mode, best_lds = p->lds, rc = SVIMX_OK;
   double best = INFINITY;
-  for (int i = 0; i < 3 && !rc; ++i) {
+  for (int i = 0; i < kNrTuneCands && !rc; ++i) {
+    if (ms_out) ms_out[i] = NAN;
+    if (i >= 3 && !p->lds_ok) continue;
     nr_plan_set_run_len(p, cands[i]);
+    p->lds = i >= 3 ? 1 : 0;
     rc = svimx_fractions(stream, phi, band, nb, nx, ny, nz, h, occupancy, p, cut_hi, ws, ws_bytes);
     if (!rc) rc = svimx_nr_select(stream, p);
     if (!rc) rc = svimx_nr_sum(stream, p, occupancy, nr_out, ws, ws_bytes);  // warm-up
@@ -424,12 +451,14 @@ int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t
     if (ms / 3.0 < best) {
       best = ms / 3.0;
       best_rl = cands[i];
+      best_lds = p->lds;
     }
   }
   hipEventDestroy(ev[0]);
   hipEventDestroy(ev[1]);
   // leave the plan's marks and run list those of the chosen run length
   nr_plan_set_run_len(p, best_rl);
+  p->lds = best_lds;
   if (!rc) rc = svimx_fractions(stream, phi, band, nb, nx, ny, nz, h, occupancy, p, cut_hi, ws, ws_bytes);
   if (!rc) rc = svimx_nr_select(stream, p);
   return rc;
@@ -439,6 +468,27 @@ int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t
 int svimx_nr_plan_run_length(void* plan) {
   const NrPlan* p = static_cast<const NrPlan*>(plan);
   return p ? p->run_mode : 0;
+}
+
+int svimx_nr_plan_set_run_length(void* plan, int rl) {
+  NrPlan* p = static_cast<NrPlan*>(plan);
+  SVIMX_REQUIRE(p && p->run_mode && !p->fft && (rl == 4 || rl == 8 || rl == 16), SVIMX_EBADARG,
+                "run length needs a run-mode plan and 4, 8 or 16 nodes");
+  nr_plan_set_run_len(p, rl);
+  return SVIMX_OK;
+}
+
+int svimx_nr_plan_set_staging(void* plan, int lds) {
+  NrPlan* p = static_cast<NrPlan*>(plan);
+  SVIMX_REQUIRE(p && (lds == 0 || lds == 1), SVIMX_EBADARG, "bad N_r staging argument");
+  SVIMX_REQUIRE(!lds || p->lds_ok, SVIMX_EBADARG, "grid too large for LDS staging");
+  p->lds = lds;
+  return SVIMX_OK;
+}
+
+int svimx_nr_plan_staging(void* plan) {
+  const NrPlan* p = static_cast<const NrPlan*>(plan);
+  return p ? p->lds : 0;
 }
 
 int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws, int64_t ws_bytes) {

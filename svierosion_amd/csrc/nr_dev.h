@@ -2,6 +2,8 @@
 // (nr.hip) and the job-table kernel of the batch engine (engine.hip).
 #pragma once
 
+#include <climits>
+
 #include "common.h"
 #include "nr_plan.h"
 
@@ -119,5 +121,246 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
 // its own L2) a contiguous slice of a G-block range: G must be a multiple of 8
 // and the range must start at a block index that is a multiple of 8.
 __device__ __forceinline__ int64_t xcd_slice(int64_t b, int64_t G) { return (b % kXcds) * (G / kXcds) + b / kXcds; }
+
+// ---------------------------------------------------------------- LDS-staged run summation
+// The same sums as nr_runs_body, in the same order (bit-identical results),
+// with the occupancy staged through LDS.  Per wave of 64 runs, the runs are
+// grouped into up to kNrLdsClusters clusters (same x, z-runs within +-Dz and
+// lines within +-reach of the first pending run; the first cluster of a pass
+// shrinks its reach until it fits, down to one line), each with a bounding
+// slab of rows (y - m .. y + m) x (z windows).  Every lane stages up to
+// kNrLdsSlots float4s of the slabs per dx plane: the next plane's loads are
+// issued into registers before the current plane is summed from LDS.  Within
+// a slab row, float4 chunk c sits at (c mod S) * half + c / S (S = float4s
+// per run), so the lanes of one line read consecutive chunks (no bank
+// conflicts).  Runs that do not fit in this pass's slabs are taken next.
+constexpr int kNrLdsCap4 = 1000;                     // float4 per wave (16 KB): 9 waves per CU
+constexpr int kNrLdsSlots = 16;                      // float4s each lane stages per plane
+static_assert(kNrLdsSlots * 64 >= kNrLdsCap4, "the slots must cover the slab");
+#define NR_LDS_SLOTS(F) F(0) F(1) F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13) F(14) F(15)
+constexpr int kNrLdsClusters = 4;
+constexpr int kNrLdsDy = 40;                   // y reach of a cluster from its first run
+constexpr int nr_lds_dz(int rl) { return rl >= 24 ? 1 : 24 / rl; }  // z reach in runs
+// LDS float4s of a slab row holding sz4 chunks in the swizzled layout
+__host__ __device__ constexpr int nr_lds_pitch(int sz4, int s) { return (s * ((sz4 + s - 1) / s)) | 1; }
+
+struct NrLdsClu {
+  int g4;    // float4 index of the slab origin (x, ylo - m, zlo * RL - MP) in the occupancy
+  int sz4;   // float4 chunks per row
+  int p4;    // LDS row pitch (float4)
+  int half;  // chunks per swizzle group
+  int base;  // LDS offset (float4)
+  int q0;    // first staging index of this cluster
+  int pad0, pad1;
+};
+
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Orders this wave's LDS accesses across lanes (a wavefront executes its LDS
+// instructions in order; the fences stop the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Taps of one column with half-height K from a swizzled slab row: only the
+// float4s that the taps touch are read.
+template <int K, int MP, int RL>
+__device__ __forceinline__ void lds_column(const float4* __restrict__ row, int half, const float* __restrict__ wc,
+                                           float (&part)[RL]) {
+  if constexpr (K <= MP) {
+    constexpr int WIN = RL + 2 * MP, S = RL / 4;
+    constexpr int q0 = (MP - K) / 4, q1 = (MP + RL + K - 1) / 4;
+    float win[WIN];
+#pragma unroll
+    for (int q = q0; q <= q1; ++q) {
+      const float4 v = row[(q % S) * half + q / S];
+      win[4 * q] = v.x;
+      win[4 * q + 1] = v.y;
+      win[4 * q + 2] = v.z;
+      win[4 * q + 3] = v.w;
+    }
+    column_taps<K, MP, RL>(wc, win, part);
+  }
+}
+
+// slab: this wave's kNrLdsCap4 float4s; clu: its kNrLdsClusters records.
+// wv / NW: this wave's index and the wave count of the launch.  The
+// occupancy's float4 count must be below 2^31.
+template <int MP, int RL>
+__device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
+                                            const int32_t* __restrict__ list, int64_t nrun,
+                                            const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
+                                            int m, const float* __restrict__ w, float scale, float negbv,
+                                            float* __restrict__ out, float4* __restrict__ slab,
+                                            NrLdsClu* __restrict__ clu, int64_t wv, int64_t NW) {
+  constexpr int DZ = nr_lds_dz(RL), S = RL / 4;
+  static_assert((2 * MP + 1) * nr_lds_pitch(((2 * DZ + 1) * RL + 2 * MP) / 4, S) <= kNrLdsCap4,
+                "a one-line cluster must fit the slab");
+  const int lane = threadIdx.x & 63;
+  const int gp4 = (int)(OL.pitch_z / 4);                  // float4s per occupancy row
+  const int plane4 = (int)(OL.pitch_y * OL.pitch_z / 4);  // float4s per occupancy x-plane
+  const float4* __restrict__ occ4 = reinterpret_cast<const float4*>(occ);
+  int gfirst = 0;
+  while (col_start[gfirst] == col_start[gfirst + 1]) ++gfirst;  // first plane with columns
+  for (int64_t first = wv * 64; first < nrun; first += NW * 64) {
+    const int64_t it = first + lane;
+    const bool live = it < nrun;
+    const int32_t r = list[live ? it : first];
+    const int zr = r % nzr;
+    const int64_t xy = r / nzr;
+    const int y = (int)(xy % OL.ny), x = (int)(xy / OL.ny);
+    bool pending = live;
+    while (__any(pending)) {
+      // ---- clusters of this pass
+      int cid = -1, used = 0, nclu = 0, qtot = 0;
+      int qs1 = INT_MAX, qs2 = INT_MAX, qs3 = INT_MAX;  // staging starts of clusters 1..3
+      int my_ylo = 0, my_zlo = 0, my_p4 = 1, my_base = 0, my_half = 0;
+      while (nclu < kNrLdsClusters) {
+        const bool fr = pending && cid < 0;
+        const unsigned long long fm = __ballot(fr);
+        if (fm == 0ull) break;
+        const int p0 = __ffsll(fm) - 1;
+        const int x0 = __shfl(x, p0, 64), y0 = __shfl(y, p0, 64), z0 = __shfl(zr, p0, 64);
+        int reach = kNrLdsDy;
+        bool cand;
+        int ylo, yhi, zlo, sz4, p4, sy;
+        for (;;) {
+          cand = fr && x == x0 && abs(zr - z0) <= DZ && abs(y - y0) <= reach;
+          ylo = wave_min_i(cand ? y : INT_MAX);
+          yhi = wave_max_i(cand ? y : INT_MIN);
+          zlo = wave_min_i(cand ? zr : INT_MAX);
+          const int zhi = wave_max_i(cand ? zr : INT_MIN);
+          sz4 = ((zhi - zlo + 1) * RL + 2 * MP) / 4;
+          p4 = nr_lds_pitch(sz4, S);
+          sy = yhi - ylo + 1 + 2 * m;
+          if (used + sy * p4 <= kNrLdsCap4 || nclu > 0 || reach == 0) break;
+          reach >>= 1;
+        }
+        if (used + sy * p4 > kNrLdsCap4) break;  // only after the first cluster
+        const int half = (sz4 + S - 1) / S;
+        if (cand) {
+          cid = nclu;
+          my_ylo = ylo;
+          my_zlo = zlo;
+          my_p4 = p4;
+          my_base = used;
+          my_half = half;
+        }
+        if (nclu == 1) qs1 = qtot;
+        if (nclu == 2) qs2 = qtot;
+        if (nclu == 3) qs3 = qtot;
+        if (lane == 0)
+          clu[nclu] = NrLdsClu{(int)(OL.at(x0, ylo - m, zlo * RL - MP) / 4), sz4, p4, half, used, qtot, 0, 0};
+        used += sy * p4;
+        qtot += sy * sz4;
+        ++nclu;
+      }
+      wave_lds_sync();  // the cluster records are visible to every lane
+      // ---- staging addresses of this pass (the plane offset is added per dx);
+      // slots past the slabs load cluster 0's origin, which is in bounds, and
+      // store nothing.  Named registers (not an array) keep the staged float4s
+      // out of scratch memory.
+      const int g4_0 = clu[0].g4;
+#define NR_SLOT_DECL(i) \
+  float4 stg##i;        \
+  int src##i, dst##i;
+      NR_LDS_SLOTS(NR_SLOT_DECL)
+#undef NR_SLOT_DECL
+#define NR_SLOT_ADDR(i)                                                                   \
+  {                                                                                       \
+    const int q = lane + 64 * i;                                                          \
+    const int c = (q >= qs1 ? 1 : 0) + (q >= qs2 ? 1 : 0) + (q >= qs3 ? 1 : 0);           \
+    const NrLdsClu C = clu[c];                                                            \
+    const int local = q - C.q0;                                                           \
+    const int row = (int)(((float)local + 0.5f) * (1.0f / (float)C.sz4));                 \
+    const int col = local - row * C.sz4;                                                  \
+    const bool valid = q < qtot;                                                          \
+    src##i = valid ? C.g4 + row * gp4 + col : g4_0;                                       \
+    dst##i = valid ? C.base + row * C.p4 + (col % S) * C.half + col / S : -1;             \
+  }
+      NR_LDS_SLOTS(NR_SLOT_ADDR)
+#undef NR_SLOT_ADDR
+#define NR_SLOT_ISSUE(i) stg##i = occ4[src##i + off];
+#define NR_SLOT_COMMIT(i) \
+  if (dst##i >= 0) slab[dst##i] = stg##i;
+      {
+        const int off = (gfirst - m) * plane4;
+        NR_LDS_SLOTS(NR_SLOT_ISSUE)
+      }
+      // ---- sums over the dx planes
+      const bool act = cid >= 0;
+      const float4* my_row = slab + my_base + (y - my_ylo + m) * my_p4 + (zr - my_zlo);
+      float acc[RL], part[RL];
+#pragma unroll
+      for (int j = 0; j < RL; ++j) acc[j] = part[j] = 0.0f;
+      for (int g = gfirst; g <= 2 * m;) {
+        const int c0 = col_start[g], c1 = col_start[g + 1];
+        int gn = g + 1;
+        while (gn <= 2 * m && col_start[gn] == col_start[gn + 1]) ++gn;
+        wave_lds_sync();  // the previous plane's reads are done
+        NR_LDS_SLOTS(NR_SLOT_COMMIT)
+        wave_lds_sync();
+        if (gn <= 2 * m) {  // next plane's loads fly while this one is summed
+          const int off = (gn - m) * plane4;
+          NR_LDS_SLOTS(NR_SLOT_ISSUE)
+        }
+        if (g != gfirst) {
+#pragma unroll
+          for (int j = 0; j < RL; ++j) {
+            acc[j] += part[j];
+            part[j] = 0.0f;
+          }
+        }
+        if (act) {
+          for (int c = c0; c < c1; ++c) {
+            const NrColumn col = cols[c];
+            const float4* rowp = my_row + col.dy * my_p4;
+            const float* wc = w + col.woff;
+            switch (-col.dz0) {
+              case 0: lds_column<0, MP, RL>(rowp, my_half, wc, part); break;
+              case 1: lds_column<1, MP, RL>(rowp, my_half, wc, part); break;
+              case 2: lds_column<2, MP, RL>(rowp, my_half, wc, part); break;
+              case 3: lds_column<3, MP, RL>(rowp, my_half, wc, part); break;
+              case 4: lds_column<4, MP, RL>(rowp, my_half, wc, part); break;
+              case 5: lds_column<5, MP, RL>(rowp, my_half, wc, part); break;
+              case 6: lds_column<6, MP, RL>(rowp, my_half, wc, part); break;
+              case 7: lds_column<7, MP, RL>(rowp, my_half, wc, part); break;
+              case 8: lds_column<8, MP, RL>(rowp, my_half, wc, part); break;
+              case 9: lds_column<9, MP, RL>(rowp, my_half, wc, part); break;
+              case 10: lds_column<10, MP, RL>(rowp, my_half, wc, part); break;
+              case 11: lds_column<11, MP, RL>(rowp, my_half, wc, part); break;
+              case 12: lds_column<12, MP, RL>(rowp, my_half, wc, part); break;
+              case 13: lds_column<13, MP, RL>(rowp, my_half, wc, part); break;
+              case 14: lds_column<14, MP, RL>(rowp, my_half, wc, part); break;
+              case 15: lds_column<15, MP, RL>(rowp, my_half, wc, part); break;
+              case 16: lds_column<16, MP, RL>(rowp, my_half, wc, part); break;
+              default: break;
+            }
+          }
+        }
+        g = gn;
+      }
+      if (act) {
+        const int z0 = zr * RL;
+        const int64_t base = xy * OL.nz + z0;
+#pragma unroll
+        for (int j = 0; j < RL; ++j)
+          if (z0 + j < OL.nz) out[base + j] = ((acc[j] + part[j]) * scale) * negbv + 1.0f;
+      }
+      pending = pending && !act;
+#undef NR_SLOT_ISSUE
+#undef NR_SLOT_COMMIT
+    }
+  }
+}
 
 }  // namespace svimx

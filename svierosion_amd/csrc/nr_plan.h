@@ -20,6 +20,7 @@ namespace svimx {
 constexpr int kTX = 4, kTY = 8, kTZ = 8;   // tile mode: one wave per x-layer of 8x8 nodes
 constexpr int kRunLen = 8;                 // run mode: default nodes per z-run
 constexpr int kRunLenMin = 4, kRunLenMax = 16;  // run lengths the tuner chooses from
+constexpr int kNrTuneCands = 6;            // run lengths 4, 8, 16, each with global or LDS windows
 constexpr int kRunMaxM = 16;               // largest ball half-width handled in run mode (= kOccPad)
 
 struct NrColumn {
@@ -34,6 +35,8 @@ struct NrPlan {
   double h, radius;
   Dims d;
   int run_mode;       // 0: tile mode; else the run length (4, 8 or 16)
+  int lds;            // run mode: 1 stages the occupancy through LDS (same sums, same order)
+  int lds_ok;         // the occupancy's float4 count fits the LDS kernel's int32 indexing
   // tile mode
   int ntx, nty, ntz;
   int64_t ntiles;

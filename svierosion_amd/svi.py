@@ -131,19 +131,37 @@ class BallConvolution:
         """Nodes per z-run of the direct summation (0: tile mode)."""
         return int(_abi.lib.svimx_nr_plan_run_length(self.plan))
 
+    @run_length.setter
+    def run_length(self, value: int) -> None:
+        _abi.check(_abi.lib.svimx_nr_plan_set_run_length(self.plan, int(value)), "N_r run length")
+
+    @property
+    def staging(self) -> str:
+        """How the direct summation reads the occupancy: "lds" (staged per wave
+        through LDS) or "l1" (per-run windows)."""
+        return "lds" if int(_abi.lib.svimx_nr_plan_staging(self.plan)) else "l1"
+
+    @staging.setter
+    def staging(self, value: str) -> None:
+        if value not in ("lds", "l1"):
+            raise ValueError(f"unknown N_r staging {value!r}")
+        _abi.check(_abi.lib.svimx_nr_plan_set_staging(self.plan, 1 if value == "lds" else 0), "N_r staging")
+
     def tune(self, ls, cut_hi: float) -> dict:
-        """Autotune the summation's z-run length (4, 8 or 16 nodes) on the level
-        set's current field; returns {run length: ms per launch}.  The N_r
-        values the velocity reads do not depend on the choice.  Leaves the
-        occupancy and N_r marks of ``ls``'s current field in place."""
-        ms = (C.c_double * 3)()
+        """Autotune the summation's z-run length (4, 8 or 16 nodes) and its
+        staging (windows through the L1, or the occupancy staged in LDS) on the
+        level set's current field; returns {"<run length>" or "<run length>/lds":
+        ms per launch}.  The N_r values the velocity reads do not depend on the
+        choice.  Leaves the occupancy and N_r marks of ``ls``'s current field in
+        place."""
+        ms = (C.c_double * 6)()
         field = torch.empty(ls.grid.size, dtype=torch.float32, device=self.device)
         _abi.check(_abi.lib.svimx_nr_plan_tune(_abi.stream(), self.plan, _abi.ptr(ls.flat), _abi.ptr(ls._band),
                                                ls.nb, *ls.dims, ls.h, float(cut_hi), _abi.ptr(ls.occupancy),
                                                _abi.ptr(field), _abi.ptr(ls.ws), ls.ws.numel(), ms), "N_r tuning")
         if self.run_length == 0:
             return {}
-        return {4: ms[0], 8: ms[1], 16: ms[2]}
+        return {"4": ms[0], "8": ms[1], "16": ms[2], "4/lds": ms[3], "8/lds": ms[4], "16/lds": ms[5]}
 
     def interior_volume(self, fractions: torch.Tensor, nodes: torch.Tensor | None = None) -> torch.Tensor:
         """Dense |B_r(x) cap Omega| (derived from N_r)."""

@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 def _case(name):
     from svierosion_amd.api import ErosionParams
     from svierosion_amd.shapes import cube, icosphere
+    name = name.removesuffix("_lds")
     if name == "svi_cube":
         return cube(1.0), ErosionParams(law="svi", radius=0.2, k=1, s=1, t=0.9, resolution=40), 0.93, 400
     if name == "svi_sphere_k2":
@@ -36,14 +37,17 @@ def _sim(name):
     from svierosion_amd.levelset import LevelSet
     from svierosion_amd.sdf import mesh_to_sdf
     mesh, p, target, max_steps = _case(name)
+    base = name.removesuffix("_lds")
     grid = make_grid(mesh, p)
-    if name not in _PHI:
-        _PHI[name] = mesh_to_sdf(mesh, grid, band=p.band)
+    if base not in _PHI:
+        _PHI[base] = mesh_to_sdf(mesh, grid, band=p.band)
     lo, hi = mesh.bounds
     law = p.make_law(length_scale=float((hi - lo).max()))
-    ls = LevelSet(_PHI[name].clone(), grid, width=p.band)
+    ls = LevelSet(_PHI[base].clone(), grid, width=p.band)
     sim = ErosionSimulation(ls, law, cfl=p.cfl, feedback_cfl=p.feedback_cfl, reinit_distance=p.reinit_distance,
                             reinit_iterations=p.reinit_iterations, nr_method="fft" if name == "svi_fft" else "auto")
+    if sim.conv is not None and sim.conv.method == "direct":
+        sim.conv.staging = "lds" if name.endswith("_lds") else "l1"
     return sim, target, max_steps
 
 
@@ -82,7 +86,7 @@ def _engine_run(names, graphs):
     return out
 
 
-@pytest.mark.parametrize("name", ["svi_cube", "svi_sphere_k2", "mean", "gauss", "svi_fft"])
+@pytest.mark.parametrize("name", ["svi_cube", "svi_sphere_k2", "mean", "gauss", "svi_fft", "svi_cube_lds"])
 def test_engine_job_equals_single_job_path(device, name):
     sim, target, max_steps = _sim(name)
     r = sim.run(target_fraction=target, max_steps=max_steps)
@@ -96,8 +100,20 @@ def test_engine_job_equals_single_job_path(device, name):
         assert any(rec.reinit for rec in sim.history)
 
 
+def test_lds_staging_is_result_neutral_in_the_engine(device):
+    """The engine's LDS-staged N_r family reproduces the single-job path that
+    reads its windows through the L1, bit for bit."""
+    sim, target, max_steps = _sim("svi_sphere_k2")
+    assert sim.conv.staging == "l1"
+    r = sim.run(target_fraction=target, max_steps=max_steps)
+    got = _engine_run(["svi_sphere_k2_lds"], graphs=True)["svi_sphere_k2_lds"]
+    assert got[0].conv.staging == "lds"
+    assert got[1] == r.stop_reason
+    _assert_same_run(sim, got[0])
+
+
 def test_mix_equals_alone_with_graphs(device):
-    names = ["svi_cube", "mean", "svi_sphere_k2", "gauss", "svi_fft"]
+    names = ["svi_cube", "mean", "svi_sphere_k2_lds", "gauss", "svi_fft"]
     mix = _engine_run(names, graphs=True)
     for nm in names:
         alone = _engine_run([nm], graphs=False)[nm]

@@ -5,7 +5,8 @@
 // G * kBandThreads, block partials at partials[lb * kPartialSlots].  The
 // single-job kernels call it with (blockIdx.x, gridDim.x); the batch engine
 // with the job's own (lb, G), so a job's arithmetic and reduction order are
-// the same alone and in a batch.
+// the same alone and in a batch.  (Contiguous XCD-local chunks of the list
+// were measured slower in a batch of curvature jobs.)
 #pragma once
 
 #include "common.h"
@@ -131,7 +132,8 @@ __device__ __forceinline__ void fractions_body(const float* __restrict__ phi, co
                                                int64_t G) {
   __shared__ double red[4];
   const OccLayout OL = OccLayout::of(d.nx, d.ny, d.nz);
-  double fsum = 0.0, neg = 0.0, pos = 0.0;
+  double fsum = 0.0;
+  bool neg = false, pos = false;
   for (int64_t q = lb * kBandThreads + threadIdx.x; q < nb; q += G * kBandThreads) {
     const int64_t n = band[q];
     int i, j, k;
@@ -140,8 +142,8 @@ __device__ __forceinline__ void fractions_body(const float* __restrict__ phi, co
     float fr = cell_fraction(g);
     occ[OL.at(i, j, k)] = fr;
     fsum += fr;
-    neg = g.c < 0.0f ? 1.0 : neg;
-    pos = g.c > 0.0f ? 1.0 : pos;
+    neg = neg || g.c < 0.0f;
+    pos = pos || g.c > 0.0f;
     if (marks && fabsf(g.c) < cut) {
       float p[3];
       closest_point(g, i, j, k, h, hi.v, p);
@@ -152,15 +154,21 @@ __device__ __forceinline__ void fractions_body(const float* __restrict__ phi, co
         mark_stencil_tiles(marks, nty, ntz, bx, by, bz);
     }
   }
-  double s = block_reduce<0>(fsum, red);
-  double a = block_reduce<1>(neg, red);
-  double b = block_reduce<1>(pos, red);
+  // the sign flags are block-wide ORs: ballots, folded into the one LDS exchange of the sum
+  __shared__ int flags[4];
+  const int wave = threadIdx.x >> 6;
+  const unsigned long long bn = __ballot(neg), bp = __ballot(pos);
+  if ((threadIdx.x & 63) == 0) flags[wave] = (bn ? 1 : 0) | (bp ? 2 : 0);
+  double s = block_reduce<0>(fsum, red);  // its barriers publish flags[]
   if (threadIdx.x == 0) {
+    int f = 0;
+    for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) f |= flags[w];
     double* o = partials + lb * kPartialSlots;
     o[0] = s;
-    o[1] = a;
-    o[2] = b;
+    o[1] = (f & 1) ? 1.0 : 0.0;
+    o[2] = (f & 2) ? 1.0 : 0.0;
   }
+  __syncthreads();  // flags[] is rewritten by the block's next segment
 }
 
 // ---------------------------------------------------------------- curvature scatter
@@ -186,11 +194,11 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
                                               const float* __restrict__ k2d, float cut_lo, float cut_hi, Hi3 hi,
                                               int stats, float* __restrict__ V, double* __restrict__ partials,
                                               int64_t lb, int64_t G) {
-  __shared__ double red[4];
   float vmax = 0.0f;
   float qmin = INFINITY, qmax = -INFINITY, dvmax = -INFINITY, dmax = -INFINITY;
   float qmin_a = INFINITY, qmax_a = -INFINITY, dvmax_a = -INFINITY, dmax_a = -INFINITY;
-  double capped = 0, vsum = 0, nnear = 0, capped_a = 0, vsum_a = 0, nall = 0;
+  double vsum = 0, vsum_a = 0;
+  uint32_t capped = 0, nnear = 0, capped_a = 0, nall = 0;
   for (int64_t q = lb * kBandThreads + threadIdx.x; q < nb; q += G * kBandThreads) {
     const int64_t n = band[q];
     Grad g = central(phi, n, d);
@@ -208,7 +216,7 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
         v = vraw * cutoff_factor(x, cut_lo, cut_hi);
         if (stats) {
           float dv = svi_dspeed(L, N);
-          double cp = N > L.n_cap ? 1.0 : 0.0;
+          const uint32_t cp = N > L.n_cap ? 1u : 0u;
           qmin_a = fminf(qmin_a, N); qmax_a = fmaxf(qmax_a, N); dvmax_a = fmaxf(dvmax_a, dv);
           capped_a += cp; vsum_a += vraw; nall += 1;
           if (near) {
@@ -240,26 +248,65 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
     vmax = fmaxf(vmax, v);
   }
   if (!stats) return;
-  double r[15];
-  r[0] = block_reduce<1>(vmax, red);
-  r[1] = block_reduce<2>(qmin, red);
-  r[2] = block_reduce<1>(qmax, red);
-  r[3] = block_reduce<1>(dvmax, red);
-  r[4] = block_reduce<0>(capped, red);
-  r[5] = block_reduce<0>(vsum, red);
-  r[6] = block_reduce<0>(nnear, red);
-  r[7] = block_reduce<1>(dmax, red);
-  r[8] = block_reduce<2>(qmin_a, red);
-  r[9] = block_reduce<1>(qmax_a, red);
-  r[10] = block_reduce<1>(dvmax_a, red);
-  r[11] = block_reduce<0>(capped_a, red);
-  r[12] = block_reduce<0>(vsum_a, red);
-  r[13] = block_reduce<0>(nall, red);
-  r[14] = block_reduce<1>(dmax_a, red);
-  if (threadIdx.x == 0) {
-    double* o = partials + lb * kPartialSlots;
-    for (int s = 0; s < 15; ++s) o[s] = r[s];
+  // One block reduction of all statistics: max/min in float and counts in
+  // uint32 (both exact in any order), the two sums in double with the same
+  // xor-shuffle tree and wave order as block_reduce; a single LDS exchange.
+  float f[9] = {vmax, qmin, qmax, dvmax, dmax, qmin_a, qmax_a, dvmax_a, dmax_a};
+  constexpr bool fmx[9] = {true, false, true, true, true, false, true, true, true};  // max, else min
+  uint32_t c[4] = {capped, nnear, capped_a, nall};
+  double sm[2] = {vsum, vsum_a};
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const float u = __shfl_xor(f[t], o, 64);
+      f[t] = fmx[t] ? fmaxf(f[t], u) : fminf(f[t], u);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) c[t] += __shfl_xor(c[t], o, 64);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) sm[t] += __shfl_xor(sm[t], o, 64);
   }
+  __shared__ float wf[4][9];
+  __shared__ uint32_t wc[4][4];
+  __shared__ double wd[4][2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) wf[wave][t] = f[t];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) wc[wave][t] = c[t];
+    wd[wave][0] = sm[0];
+    wd[wave][1] = sm[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) {
+#pragma unroll
+      for (int t = 0; t < 9; ++t) f[t] = fmx[t] ? fmaxf(f[t], wf[w][t]) : fminf(f[t], wf[w][t]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) c[t] += wc[w][t];
+      sm[0] += wd[w][0];
+      sm[1] += wd[w][1];
+    }
+    double* o = partials + lb * kPartialSlots;
+    o[0] = f[0];
+    o[1] = f[1];
+    o[2] = f[2];
+    o[3] = f[3];
+    o[4] = (double)c[0];
+    o[5] = sm[0];
+    o[6] = (double)c[1];
+    o[7] = f[4];
+    o[8] = f[5];
+    o[9] = f[6];
+    o[10] = f[7];
+    o[11] = (double)c[2];
+    o[12] = sm[1];
+    o[13] = (double)c[3];
+    o[14] = f[8];
+  }
+  __syncthreads();  // wf/wc/wd are reused by the block's next segment
 }
 
 // ---------------------------------------------------------------- RK stage

@@ -76,6 +76,11 @@ __host__ __device__ inline int reduce_grid(int64_t n) {
   int64_t b = (n + kBandThreads - 1) / kBandThreads;
   return (int)(b < 1 ? 1 : (b > kMaxReduceBlocks ? kMaxReduceBlocks : b));
 }
+
+// Block remap giving XCD x (blocks go round-robin over the 8 XCDs, each with
+// its own L2) a contiguous slice of a G-block range: G must be a multiple of 8
+// and the range must start at a block index that is a multiple of 8.
+__host__ __device__ inline int64_t xcd_slice(int64_t b, int64_t G) { return (b % kXcds) * (G / kXcds) + b / kXcds; }
 constexpr int kCompactThreads = 256;
 constexpr int kCompactPerThread = 16;
 constexpr int64_t kCompactChunk = (int64_t)kCompactThreads * kCompactPerThread;

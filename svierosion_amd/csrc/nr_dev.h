@@ -61,7 +61,7 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
     const int32_t r = list[live ? it : first];
     const int zr = r % nzr;
     const int64_t xy = r / nzr;
-    const int y = (int)(xy % OL.ny), x = (int)(xy / OL.ny);
+    const int x = (int)((uint32_t)xy / (uint32_t)OL.ny), y = (int)((uint32_t)xy - (uint32_t)x * (uint32_t)OL.ny);
     const int z0 = zr * RL;
     float acc[RL], part[RL];
 #pragma unroll
@@ -117,10 +117,6 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
   }
 }
 
-// Block remap giving XCD x (blocks go round-robin over the 8 XCDs, each with
-// its own L2) a contiguous slice of a G-block range: G must be a multiple of 8
-// and the range must start at a block index that is a multiple of 8.
-__device__ __forceinline__ int64_t xcd_slice(int64_t b, int64_t G) { return (b % kXcds) * (G / kXcds) + b / kXcds; }
 
 // ---------------------------------------------------------------- LDS-staged run summation
 // The same sums as nr_runs_body, in the same order (bit-identical results),
@@ -217,7 +213,7 @@ __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const
     const int32_t r = list[live ? it : first];
     const int zr = r % nzr;
     const int64_t xy = r / nzr;
-    const int y = (int)(xy % OL.ny), x = (int)(xy / OL.ny);
+    const int x = (int)((uint32_t)xy / (uint32_t)OL.ny), y = (int)((uint32_t)xy - (uint32_t)x * (uint32_t)OL.ny);
     bool pending = live;
     while (__any(pending)) {
       // ---- clusters of this pass

@@ -47,11 +47,16 @@ struct OccLayout {
 };
 
 __device__ __forceinline__ void unflat(const Dims& d, int64_t n, int& i, int& j, int& k) {
-  int64_t syz = (int64_t)d.ny * d.nz;
-  i = (int)(n / syz);
-  int64_t r = n - (int64_t)i * syz;
-  j = (int)(r / d.nz);
-  k = (int)(r - (int64_t)j * d.nz);
+  // node indices fit 31 bits (band lists are int32): 32-bit unsigned divisions,
+  // several times cheaper than 64-bit ones on the GPU
+  const uint32_t u = (uint32_t)n, nz = (uint32_t)d.nz;
+  const uint32_t syz = (uint32_t)d.ny * nz;
+  const uint32_t a = u / syz;
+  const uint32_t r = u - a * syz;
+  const uint32_t b = r / nz;
+  i = (int)a;
+  j = (int)b;
+  k = (int)(r - b * nz);
 }
 
 __device__ __forceinline__ float sq(float x) { return x * x; }

@@ -188,6 +188,10 @@ __device__ __forceinline__ void curvatures_body(const float* __restrict__ phi, c
 // Partial slots: 0 vmax, 1 qmin, 2 qmax, 3 dvmax, 4 capped, 5 vsum, 6 nnear,
 // 7 dmax, 8 qmin_all, 9 qmax_all, 10 dvmax_all, 11 capped_all, 12 vsum_all,
 // 13 nall, 14 dmax_all.
+// SVI selects the law family at compile time (true: SVI power law, false:
+// curvature laws), so each variant gets the registers it needs: the SVI
+// tricubic holds 64 values in flight, the curvature trilinear 16.
+template <bool SVI>
 __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, const int32_t* __restrict__ band,
                                               int64_t nb, Dims d, float h, const LawF& L,
                                               const float* __restrict__ field, const float* __restrict__ k1d,
@@ -205,7 +209,7 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
     const float x = fabsf(g.c);
     const bool near = x < h;
     float v = 0.0f;
-    if (L.kind == 0) {
+    if constexpr (SVI) {
       if (x < cut_hi) {
         int i, j, k;
         unflat(d, n, i, j, k);

@@ -212,11 +212,12 @@ __global__ void __launch_bounds__(kBandThreads) k_eng_curvatures(const JobDev* _
   }
 }
 
+template <bool SVI>
 __global__ void __launch_bounds__(kBandThreads) k_eng_velocity(const JobDev* __restrict__ J, const Seg* S, int rel,
                                                                int stats) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    velocity_body(jbuf(j, rel), j.band, j.nb, j.d, j.h, j.law, j.field, j.k1, j.k2, j.cut_lo, j.cut_hi, j.hi, stats,
+    velocity_body<SVI>(jbuf(j, rel), j.band, j.nb, j.d, j.h, j.law, j.field, j.k1, j.k2, j.cut_lo, j.cut_hi, j.hi, stats,
                   j.V, j.partials, lb, G);
   }
 }
@@ -540,7 +541,8 @@ static int prepare(Engine& E, int rel, int stats, int only_needed) {
     if (rc) return rc;
   }
   ENG_LAUNCH(k_eng_curvatures, T_CURV, kBandThreads, rel);
-  ENG_LAUNCH(k_eng_velocity, T_ALL, kBandThreads, rel, stats);
+  ENG_LAUNCH(k_eng_velocity<true>, T_SVI, kBandThreads, rel, stats);
+  ENG_LAUNCH(k_eng_velocity<false>, T_CURV, kBandThreads, rel, stats);
   if (stats) ENG_LAUNCH(k_eng_stats_dt, T_JOBS, 256, E.d_dts);
   return SVIMX_OK;
 }

@@ -24,13 +24,14 @@ __global__ void __launch_bounds__(kBandThreads)
   curvatures_body(phi, band, nb, d, h, k1d, k2d, blockIdx.x, gridDim.x);
 }
 
+template <bool SVI>
 __global__ void __launch_bounds__(kBandThreads)
     k_velocity(const float* __restrict__ phi, const int32_t* __restrict__ band, int64_t nb, Dims d, float h,
                LawF L, const float* __restrict__ field, const float* __restrict__ k1d,
                const float* __restrict__ k2d, float cut_lo, float cut_hi, Hi3 hi, int stats,
                float* __restrict__ V, double* __restrict__ partials) {
-  velocity_body(phi, band, nb, d, h, L, field, k1d, k2d, cut_lo, cut_hi, hi, stats, V, partials, blockIdx.x,
-                gridDim.x);
+  velocity_body<SVI>(phi, band, nb, d, h, L, field, k1d, k2d, cut_lo, cut_hi, hi, stats, V, partials, blockIdx.x,
+                     gridDim.x);
 }
 
 __global__ void __launch_bounds__(kBandThreads)
@@ -185,9 +186,14 @@ int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t 
   hipStream_t st = as_stream(stream);
   int nblk = reduce_grid(nb);
   if (nb > 0) {
-    hipLaunchKernelGGL(k_velocity, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
-                       law_to_f(*law), field, k1, k2, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
-                       V, w.partials);
+    if (law->kind == 0)
+      hipLaunchKernelGGL(k_velocity<true>, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
+                         law_to_f(*law), field, k1, k2, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
+                         V, w.partials);
+    else
+      hipLaunchKernelGGL(k_velocity<false>, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
+                         law_to_f(*law), field, k1, k2, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
+                         V, w.partials);
     SVIMX_CHECK_LAUNCH();
   }
   if (!stats) return SVIMX_OK;

@@ -115,3 +115,36 @@ def test_job_device_bytes_scales_with_grid():
     big = job_device_bytes(m, ErosionParams(law="svi", radius=0.1, resolution=128))
     assert 4.0 < big / small < 9.0  # ~ the node count ratio (160^3 / 90^3 = 5.6)
     assert 15 < small / 90 ** 3 < 80  # tens of bytes per node, SDF transients included
+
+
+def test_result_files_roundtrip():
+    """The result writers produce the reference's file set (svierosion/cli.py:91-128):
+    final and snapshot PLY with the quality scalar, history CSV, JSON record."""
+    import json
+    from pathlib import Path
+    from svierosion_amd.api import ErosionResult
+    from svierosion_amd.evolve import StepRecord
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.plyio import read_ply
+    from svierosion_amd.results import read_history, write_result
+    from svierosion_amd.shapes import cube
+    m = cube(1.0)
+    m.vertex_data["depth"] = np.linspace(0.0, 1.0, len(m.vertices)).astype(np.float32)
+    hist = [StepRecord(step=i + 1, time=0.1 * (i + 1), dt=0.1, volume=1.0 - 0.01 * i, fraction=1.0 - 0.01 * i,
+                       vmax=2.0, vmean=1.0, qmin=0.4, qmax=0.6, capped=i, reinit=(i == 1), wall=0.0)
+            for i in range(3)]
+    res = ErosionResult(mesh=m, snapshots=[(0.97, 0.2, m)], history=hist, grid=Grid((0.0, 0.0, 0.0), 0.1, (8, 8, 8)),
+                        stop_reason="target_volume", time=0.3, fraction=0.98, initial_volume_mesh=1.0,
+                        timings={"sdf": 0.1, "evolve": 0.2, "mesh": 0.3}, params={"law": "svi", "k": 1})
+    with tempfile.TemporaryDirectory() as d:
+        out = Path(d) / "run.ply"
+        written = write_result(res, out, quality="depth")
+        assert [p.name for p in written] == ["run_v0.970.ply", "run.ply"]
+        back = read_ply(out)
+        assert np.allclose(back.vertices, m.vertices) and np.array_equal(back.faces, m.faces)
+        assert np.allclose(back.vertex_data["quality"], m.vertex_data["depth"])
+        assert read_history(Path(d) / "run_history.csv") == hist
+        meta = json.loads((Path(d) / "run.json").read_text())
+        assert meta["stop_reason"] == "target_volume" and meta["grid"]["shape"] == [8, 8, 8]
+        assert meta["snapshots"] == [{"fraction": 0.97, "time": 0.2}]
+        assert abs(meta["final_mesh_volume"] - 1.0) < 1e-9

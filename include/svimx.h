@@ -108,20 +108,21 @@ int svimx_fractions(void* stream, const float* phi, const int32_t* band, int64_t
                     int nz, double h, float* occupancy, void* nr_plan, double cut_hi, void* ws,
                     int64_t ws_bytes);
 
-/* Principal curvatures of the band scattered into dense k1/k2 fields that are
- * zero elsewhere (svierosion/evolve.py:136-143, svierosion/kernels.py:274-299). */
+/* Principal curvatures of the band scattered into a dense field of (k1, k2)
+ * pairs (2 floats per grid node, interleaved so one 8-byte load fetches both
+ * at a trilinear corner) that is zero elsewhere (svierosion/evolve.py:136-143,
+ * svierosion/kernels.py:274-299). */
 int svimx_curvatures(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
-                     int nz, double h, float* k1_dense, float* k2_dense);
+                     int nz, double h, float* k12_dense);
 
 /* Velocity of every band node: closest point, interpolated N_r (tricubic) or
  * k1/k2 (trilinear), speed law, smoothstep cutoff cut_lo*h -> cut_hi*h; with
  * stats != 0 the SVIMX_R_VMAX..SVIMX_R_DMAX_ALL results.  `field` is the dense
- * N_r for SVI laws; k1/k2 the dense curvature fields otherwise.
+ * N_r for SVI laws; k12 the dense (k1, k2) pairs of svimx_curvatures otherwise.
  * Replaces ErosionSimulation._velocity (svierosion/evolve.py:108-155). */
 int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
-                   int nz, double h, const svimx_law_t* law, const float* field, const float* k1,
-                   const float* k2, double cut_lo, double cut_hi, int stats, float* V, void* ws,
-                   int64_t ws_bytes);
+                   int nz, double h, const svimx_law_t* law, const float* field, const float* k12,
+                   double cut_lo, double cut_hi, int stats, float* V, void* ws, int64_t ws_bytes);
 
 /* One TVD-RK3 stage over the band (svierosion/evolve.py:157-160, :172-179):
  *   L = V * |grad phi_in|_Godunov-WENO5
@@ -309,8 +310,8 @@ typedef struct svimx_job_desc {
   svimx_law_t law;
   void* nr_plan;         /* SVI laws: the N_r plan */
   float* field;          /* SVI laws: dense N_r */
-  float* k1;             /* curvature laws: dense principal curvatures */
-  float* k2;
+  float* k12;            /* curvature laws: dense (k1, k2) pairs, 2 floats per node */
+  void* reserved_ptr;
   double radius;         /* SVI ball radius (feedback CFL) */
   double cfl, feedback_cfl, parabolic_cfl, reinit_distance;
   double cutoff_lo, cutoff_hi;  /* velocity cutoff in cells */

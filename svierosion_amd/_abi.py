@@ -43,7 +43,7 @@ class JobDesc(C.Structure):
                 ("nz", C.c_int32), ("cur", C.c_int32), ("nb", C.c_int64), ("inside_outside", C.c_int64),
                 ("h", C.c_double), ("h3", C.c_double), ("width", C.c_double), ("margin", C.c_int32),
                 ("frac_valid", C.c_int32), ("law", Law), ("nr_plan", C.c_void_p), ("field", C.c_void_p),
-                ("k1", C.c_void_p), ("k2", C.c_void_p), ("radius", C.c_double), ("cfl", C.c_double),
+                ("k12", C.c_void_p), ("reserved_ptr", C.c_void_p), ("radius", C.c_double), ("cfl", C.c_double),
                 ("feedback_cfl", C.c_double), ("parabolic_cfl", C.c_double), ("reinit_distance", C.c_double),
                 ("cutoff_lo", C.c_double), ("cutoff_hi", C.c_double), ("reinit_iterations", C.c_int32),
                 ("reserved", C.c_int32), ("time", C.c_double), ("initial_volume", C.c_double),
@@ -74,8 +74,8 @@ _SIGNATURES = {
     "svimx_occupancy_elems": (_I64, [_I, _I, _I]),
     "svimx_band_rebuild": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _I, _P, _I64, _P, _I64, _P]),
     "svimx_fractions": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _P, _D, _P, _I64]),
-    "svimx_curvatures": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _P]),
-    "svimx_velocity": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _P, _D, _D, _I, _P,
+    "svimx_curvatures": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
+    "svimx_velocity": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _I, _P,
                             _P, _I64]),
     "svimx_rk_stage": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P]),
     "svimx_nr_plan_create": (_I, [_P, _I, _D, _D, _I, _I, _I, C.POINTER(C.c_void_p)]),

@@ -173,14 +173,13 @@ __device__ __forceinline__ void fractions_body(const float* __restrict__ phi, co
 
 // ---------------------------------------------------------------- curvature scatter
 __device__ __forceinline__ void curvatures_body(const float* __restrict__ phi, const int32_t* __restrict__ band,
-                                                int64_t nb, Dims d, float h, float* __restrict__ k1d,
-                                                float* __restrict__ k2d, int64_t lb, int64_t G) {
+                                                int64_t nb, Dims d, float h, float* __restrict__ k12,
+                                                int64_t lb, int64_t G) {
   for (int64_t q = lb * kBandThreads + threadIdx.x; q < nb; q += G * kBandThreads) {
     const int64_t n = band[q];
     float k1, k2;
     principal_curvatures(phi, n, d, h, k1, k2);
-    k1d[n] = k1;
-    k2d[n] = k2;
+    reinterpret_cast<float2*>(k12)[n] = make_float2(k1, k2);
   }
 }
 
@@ -194,8 +193,8 @@ __device__ __forceinline__ void curvatures_body(const float* __restrict__ phi, c
 template <bool SVI>
 __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, const int32_t* __restrict__ band,
                                               int64_t nb, Dims d, float h, const LawF& L,
-                                              const float* __restrict__ field, const float* __restrict__ k1d,
-                                              const float* __restrict__ k2d, float cut_lo, float cut_hi, Hi3 hi,
+                                              const float* __restrict__ field, const float* __restrict__ k12,
+                                              float cut_lo, float cut_hi, Hi3 hi,
                                               int stats, float* __restrict__ V, double* __restrict__ partials,
                                               int64_t lb, int64_t G) {
   float vmax = 0.0f;
@@ -234,7 +233,8 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
       unflat(d, n, i, j, k);
       float p[3];
       closest_point(g, i, j, k, h, hi.v, p);
-      float k1 = trilinear(k1d, d, p), k2 = trilinear(k2d, d, p);
+      const float2 kk = trilinear2(reinterpret_cast<const float2*>(k12), d, p);
+      const float k1 = kk.x, k2 = kk.y;
       float vraw, D;
       curv_speed(L, k1, k2, vraw, D);
       v = vraw * cutoff_factor(x, cut_lo, cut_hi);

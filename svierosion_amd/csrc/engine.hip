@@ -50,8 +50,7 @@ struct JobDev {
   Hi3 hi;
   LawF law;
   const float* field;
-  float* k1;
-  float* k2;
+  float* k12;  // curvature laws: dense (k1, k2) pairs
   float* V;
   float* L0;
   double* results;
@@ -208,7 +207,7 @@ __global__ void __launch_bounds__(64) k_eng_nr_lds(const JobDev* __restrict__ J,
 __global__ void __launch_bounds__(kBandThreads) k_eng_curvatures(const JobDev* __restrict__ J, const Seg* S, int rel) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    curvatures_body(jbuf(j, rel), j.band, j.nb, j.d, j.h, j.k1, j.k2, lb, G);
+    curvatures_body(jbuf(j, rel), j.band, j.nb, j.d, j.h, j.k12, lb, G);
   }
 }
 
@@ -217,7 +216,7 @@ __global__ void __launch_bounds__(kBandThreads) k_eng_velocity(const JobDev* __r
                                                                int stats) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    velocity_body<SVI>(jbuf(j, rel), j.band, j.nb, j.d, j.h, j.law, j.field, j.k1, j.k2, j.cut_lo, j.cut_hi, j.hi, stats,
+    velocity_body<SVI>(jbuf(j, rel), j.band, j.nb, j.d, j.h, j.law, j.field, j.k12, j.cut_lo, j.cut_hi, j.hi, stats,
                   j.V, j.partials, lb, G);
   }
 }
@@ -286,12 +285,10 @@ __global__ void __launch_bounds__(256) k_eng_gather(const JobDev* __restrict__ J
   }
 }
 
-__global__ void k_eng_zero_at_band(float* __restrict__ a, float* __restrict__ b, const int32_t* __restrict__ band,
-                                   int64_t nb) {
+__global__ void k_eng_zero_at_band(float2* __restrict__ k12, const int32_t* __restrict__ band, int64_t nb) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nb) return;
-  a[band[q]] = 0.0f;
-  b[band[q]] = 0.0f;
+  k12[band[q]] = make_float2(0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- host side
@@ -388,8 +385,7 @@ static void fill_jobdev(const EngJob& e, JobDev& j) {
   j.hi = hi_of(j.d);
   j.law = law_to_f(D.law);
   j.field = D.field;
-  j.k1 = D.k1;
-  j.k2 = D.k2;
+  j.k12 = D.k12;
   j.V = e.V;
   j.L0 = e.L0;
   Workspace w;
@@ -674,7 +670,7 @@ static int reinit_job(Engine& E, EngJob& e) {
   const int64_t n = (int64_t)D.nx * D.ny * D.nz;
   if (!e.svi() && e.nb > 0) {
     // the dense curvature fields read 0 off the band (svierosion/evolve.py:138)
-    hipLaunchKernelGGL(k_eng_zero_at_band, dim3(blocks_for(e.nb, 256)), dim3(256), 0, E.st, D.k1, D.k2, D.band,
+    hipLaunchKernelGGL(k_eng_zero_at_band, dim3(blocks_for(e.nb, 256)), dim3(256), 0, E.st, reinterpret_cast<float2*>(D.k12), D.band,
                        e.nb);
     SVIMX_CHECK_LAUNCH();
   }
@@ -985,7 +981,7 @@ int svimx_engine_add(void* engine, const svimx_job_desc_t* desc, int* job_id) {
   SVIMX_REQUIRE(workspace_bytes_for(n) <= D.ws_bytes, SVIMX_EBADARG, "job workspace too small");
   SVIMX_REQUIRE(D.law.kind >= 0 && D.law.kind <= 2, SVIMX_EBADARG, "bad law");
   SVIMX_REQUIRE(D.law.kind != 0 || (D.nr_plan && D.field), SVIMX_EBADARG, "SVI job needs an N_r plan and field");
-  SVIMX_REQUIRE(D.law.kind == 0 || (D.k1 && D.k2), SVIMX_EBADARG, "curvature job needs k1/k2 fields");
+  SVIMX_REQUIRE(D.law.kind == 0 || D.k12, SVIMX_EBADARG, "curvature job needs the (k1, k2) field");
   SVIMX_REQUIRE(D.h > 0 && D.initial_volume != 0, SVIMX_EBADARG, "bad spacing or initial volume");
   if (D.nr_plan) {
     const NrPlan* p = static_cast<const NrPlan*>(D.nr_plan);

@@ -264,6 +264,31 @@ __device__ __forceinline__ float trilinear(const float* __restrict__ f, const Di
   return out;
 }
 
+// trilinear() of two interleaved fields at once: per component the same
+// operations in the same order, so each equals trilinear() of that field.
+__device__ __forceinline__ float2 trilinear2(const float2* __restrict__ f, const Dims& d, const float p[3]) {
+  float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+  float t0 = p[0] - fx, t1 = p[1] - fy, t2 = p[2] - fz;
+  int64_t b0 = (int64_t)fx * d.sx() + (int64_t)fy * d.sy() + (int64_t)fz;
+  float o1 = 0.0f, o2 = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    float wa = a ? t0 : 1.0f - t0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      float wb = wa * (b ? t1 : 1.0f - t1);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float w = wb * (c ? t2 : 1.0f - t2);
+        const float2 v = f[b0 + a * d.sx() + b * d.sy() + c];
+        o1 = o1 + w * v.x;
+        o2 = o2 + w * v.y;
+      }
+    }
+  }
+  return make_float2(o1, o2);
+}
+
 // Principal curvatures k1 >= k2 of the level set through node n from second
 // central differences (svierosion/kernels.py:274-299).
 __device__ __forceinline__ void principal_curvatures(const float* __restrict__ f, int64_t n,

@@ -20,17 +20,17 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
 
 __global__ void __launch_bounds__(kBandThreads)
     k_curvatures(const float* __restrict__ phi, const int32_t* __restrict__ band, int64_t nb, Dims d, float h,
-                 float* __restrict__ k1d, float* __restrict__ k2d) {
-  curvatures_body(phi, band, nb, d, h, k1d, k2d, blockIdx.x, gridDim.x);
+                 float* __restrict__ k12) {
+  curvatures_body(phi, band, nb, d, h, k12, blockIdx.x, gridDim.x);
 }
 
 template <bool SVI>
 __global__ void __launch_bounds__(kBandThreads)
     k_velocity(const float* __restrict__ phi, const int32_t* __restrict__ band, int64_t nb, Dims d, float h,
-               LawF L, const float* __restrict__ field, const float* __restrict__ k1d,
-               const float* __restrict__ k2d, float cut_lo, float cut_hi, Hi3 hi, int stats,
+               LawF L, const float* __restrict__ field, const float* __restrict__ k12,
+               float cut_lo, float cut_hi, Hi3 hi, int stats,
                float* __restrict__ V, double* __restrict__ partials) {
-  velocity_body<SVI>(phi, band, nb, d, h, L, field, k1d, k2d, cut_lo, cut_hi, hi, stats, V, partials, blockIdx.x,
+  velocity_body<SVI>(phi, band, nb, d, h, L, field, k12, cut_lo, cut_hi, hi, stats, V, partials, blockIdx.x,
                      gridDim.x);
 }
 
@@ -164,35 +164,35 @@ int svimx_fractions(void* stream, const float* phi, const int32_t* band, int64_t
 }
 
 int svimx_curvatures(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny, int nz,
-                     double h, float* k1_dense, float* k2_dense) {
+                     double h, float* k12_dense) {
   SVIMX_TRY_BEGIN
   if (nb <= 0) return SVIMX_OK;
-  hipLaunchKernelGGL(k_curvatures, GRID1(nb), phi, band, nb, Dims{nx, ny, nz}, (float)h, k1_dense, k2_dense);
+  hipLaunchKernelGGL(k_curvatures, GRID1(nb), phi, band, nb, Dims{nx, ny, nz}, (float)h, k12_dense);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
   SVIMX_TRY_END
 }
 
 int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny, int nz,
-                   double h, const svimx_law_t* law, const float* field, const float* k1, const float* k2,
-                   double cut_lo, double cut_hi, int stats, float* V, void* ws, int64_t ws_bytes) {
+                   double h, const svimx_law_t* law, const float* field, const float* k12, double cut_lo,
+                   double cut_hi, int stats, float* V, void* ws, int64_t ws_bytes) {
   SVIMX_TRY_BEGIN
   Dims d{nx, ny, nz};
   Workspace w;
   if (!carve_workspace(ws, ws_bytes, d.size(), w)) return SVIMX_EBADARG;
   SVIMX_REQUIRE(law && law->kind >= 0 && law->kind <= 2, SVIMX_EBADARG, "bad law");
   SVIMX_REQUIRE(law->kind != 0 || field, SVIMX_EBADARG, "SVI law needs the N_r field");
-  SVIMX_REQUIRE(law->kind == 0 || (k1 && k2), SVIMX_EBADARG, "curvature law needs k1/k2 fields");
+  SVIMX_REQUIRE(law->kind == 0 || k12, SVIMX_EBADARG, "curvature law needs the (k1, k2) field");
   hipStream_t st = as_stream(stream);
   int nblk = reduce_grid(nb);
   if (nb > 0) {
     if (law->kind == 0)
       hipLaunchKernelGGL(k_velocity<true>, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
-                         law_to_f(*law), field, k1, k2, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
+                         law_to_f(*law), field, k12, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
                          V, w.partials);
     else
       hipLaunchKernelGGL(k_velocity<false>, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
-                         law_to_f(*law), field, k1, k2, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
+                         law_to_f(*law), field, k12, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
                          V, w.partials);
     SVIMX_CHECK_LAUNCH();
   }

@@ -59,8 +59,7 @@ def job_desc(sim: ErosionSimulation) -> JobDesc:
         d.field = sim.field.data_ptr()
         d.radius = sim.law.radius
     else:
-        d.k1 = sim.k1.data_ptr()
-        d.k2 = sim.k2.data_ptr()
+        d.k12 = sim.k12.data_ptr()
     d.cfl = sim.cfl
     d.feedback_cfl = sim.feedback_cfl
     d.parabolic_cfl = sim.parabolic_cfl

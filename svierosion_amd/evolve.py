@@ -81,14 +81,13 @@ class ErosionSimulation:
         n = levelset.grid.size
         self.conv = None
         self.field = None
-        self.k1 = self.k2 = None
+        self.k12 = None  # curvature laws: dense (k1, k2) pairs, 2 floats per node
         if law.requires_svi:
             self.conv = BallConvolution(levelset.grid, law.radius, device=dev, method=nr_method)
             self._check_convolution_margin()
             self.field = torch.zeros(n, dtype=torch.float32, device=dev)
         else:
-            self.k1 = torch.zeros(n, dtype=torch.float32, device=dev)
-            self.k2 = torch.zeros(n, dtype=torch.float32, device=dev)
+            self.k12 = torch.zeros(2 * n, dtype=torch.float32, device=dev)
         self._alloc_band_arrays()
         self._frac_valid = None     # buffer index whose fractions/marks are current
         self.tuning = {}
@@ -167,7 +166,7 @@ class ErosionSimulation:
             else:
                 with self._span("curvatures"):
                     _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims,
-                                                         ls.h, _abi.ptr(self.k1), _abi.ptr(self.k2)))
+                                                         ls.h, _abi.ptr(self.k12)))
             with self._span("velocity"):
                 self._velocity_call(st, phi, stats)
         finally:
@@ -176,8 +175,8 @@ class ErosionSimulation:
     def _velocity_call(self, st, phi, stats: bool) -> None:
         ls = self.ls
         _abi.check(_abi.lib.svimx_velocity(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
-                                               self._law_abi, _abi.ptr(self.field), _abi.ptr(self.k1),
-                                               _abi.ptr(self.k2), self.cutoff[0], self.cutoff[1], int(stats),
+                                               self._law_abi, _abi.ptr(self.field), _abi.ptr(self.k12),
+                                               self.cutoff[0], self.cutoff[1], int(stats),
                                                _abi.ptr(self.V), _abi.ptr(ls.ws), ls.ws.numel()), "velocity")
 
     def _stage(self, stage: int, src: int, u0: int, dst: int, dt: float) -> None:
@@ -311,12 +310,10 @@ class ErosionSimulation:
             did_reinit = False
             reached = target is not None and vnew <= target * (1 + tol)
             if self._displacement >= self.reinit_distance * ls.h and not reached:
-                if self.k1 is not None:
+                if self.k12 is not None:
                     # the dense curvature fields must read 0 off the band, like the
                     # reference's freshly zeroed array (svierosion/evolve.py:138)
-                    old = ls.band.long()
-                    self.k1.index_fill_(0, old, 0.0)
-                    self.k2.index_fill_(0, old, 0.0)
+                    self.k12.view(-1, 2).index_fill_(0, ls.band.long(), 0.0)
                 with self._span("reinit"):
                     ls.reinitialize(self.reinit_iterations)
                 self._alloc_band_arrays()

@@ -34,12 +34,24 @@ __global__ void k_bits_fill(uint32_t* m, int64_t nw) {
   if (i < nw) m[i] = 0u;
 }
 
+// Sets the bit of every listed node.  The list is sorted, so the lanes of a
+// wave that hit one word are adjacent: a segmented OR over the wave leaves one
+// atomic per word and wave instead of one per node.
 __global__ void k_bits_scatter(uint32_t* m, const int32_t* __restrict__ idx, int64_t n, int nz, int wpl) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t node = (uint32_t)idx[i];
+  const bool ok = i < n;
+  const uint32_t node = ok ? (uint32_t)idx[i] : 0u;
   const uint32_t line = node / (uint32_t)nz, z = node - line * (uint32_t)nz;
-  atomicOr(&m[(int64_t)line * wpl + (z >> 5)], 1u << (z & 31u));
+  const int w = ok ? (int)(line * (uint32_t)wpl + (z >> 5)) : -1;  // word count < 2^31
+  uint32_t bits = ok ? 1u << (z & 31u) : 0u;
+  const int lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t b2 = __shfl_down(bits, o, 64);
+    const int w2 = __shfl_down(w, o, 64);
+    if (lane + o < 64 && w2 == w) bits |= b2;
+  }
+  const int wprev = __shfl_up(w, 1, 64);
+  if (ok && (lane == 0 || wprev != w)) atomicOr(&m[w], bits);
 }
 
 // bits of word wi of a z-line that are nodes (z < nz)

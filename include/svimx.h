@@ -154,14 +154,17 @@ int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t 
  * created with the FFT.  The FFT allocates 12 bytes per grid node. */
 int svimx_nr_plan_set_method(void* plan, int method);
 int svimx_nr_plan_method(void* plan);
-/* Autotuner: times the run-mode summation with z-runs of 4, 8 and 16 nodes,
- * each with per-lane windows read through the L1 and with the occupancy
- * staged through LDS, on this job's field (fractions and selection recomputed
- * per candidate; the N_r values at the nodes the velocity reads are identical
- * for every candidate) and keeps the fastest.  ms_out (6 doubles: run lengths
- * 4, 8, 16 without, then with LDS staging; may be NULL) receives the
- * per-launch times.  Leaves occupancy, marks and run list those of the chosen
- * length for phi.  Synchronises.  No counterpart in the reference. */
+/* Autotuner: times the run-mode summation over SVIMX_NR_TUNE_SHAPES shapes
+ * (z-run length 4/8/16 x y-lines per unit 1/2/4, and the LDS-staged
+ * one-line kernel; svimx_nr_tune_shape names shape i) on this job's field
+ * (fractions and selection recomputed per candidate; the N_r values at the
+ * nodes the velocity reads are identical for every shape) and keeps the
+ * fastest.  ms_out (SVIMX_NR_TUNE_SHAPES doubles, NaN for shapes this plan
+ * cannot take; may be NULL) receives the per-launch times.  Leaves occupancy,
+ * marks and unit list those of the chosen shape for phi.  Synchronises.  No
+ * counterpart in the reference. */
+#define SVIMX_NR_TUNE_SHAPES 11
+int svimx_nr_tune_shape(int i, int* run_length, int* lines, int* lds);
 int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
                        int nz, double h, double cut_hi, float* occupancy, float* nr_out, void* ws, int64_t ws_bytes,
                        double* ms_out);
@@ -169,6 +172,12 @@ int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t
  * and invalidates the marks (reset and mark again before the next sum). */
 int svimx_nr_plan_run_length(void* plan);
 int svimx_nr_plan_set_run_length(void* plan, int rl);
+/* y-lines per unit of a run-mode plan (1, 2 or 4): the lines of a unit share
+ * every occupancy window they read, with each line's sums in the one-line
+ * order (bit-identical).  Needs m + lines - 1 <= 16 and, for 4 lines, runs of
+ * at most 8 nodes.  No counterpart in the reference. */
+int svimx_nr_plan_set_lines(void* plan, int lines);
+int svimx_nr_plan_lines(void* plan);
 /* Run mode: 1 stages the occupancy through LDS (clusters of runs per wave),
  * 0 reads each run's windows through the L1.  Same sums in the same order,
  * so the results are bit-identical.  No counterpart in the reference. */

@@ -88,6 +88,9 @@ _SIGNATURES = {
     "svimx_nr_plan_tune": (_I, [_P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P, _P, _I64, C.POINTER(_D)]),
     "svimx_nr_plan_run_length": (_I, [_P]),
     "svimx_nr_plan_set_run_length": (_I, [_P, _I]),
+    "svimx_nr_plan_set_lines": (_I, [_P, _I]),
+    "svimx_nr_plan_lines": (_I, [_P]),
+    "svimx_nr_tune_shape": (_I, [_I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "svimx_nr_plan_set_staging": (_I, [_P, _I]),
     "svimx_nr_plan_staging": (_I, [_P]),
     "svimx_nr_plan_set_method": (_I, [_P, _I]),

@@ -68,6 +68,8 @@ struct JobDev {
   const float* weights;
   float scale, negbv;
   int m, lds;
+  int ylines, nyu;  // run mode: y-lines per unit, units per x-plane column
+  int64_t nunits;
   // dt control (FP64)
   double hd, cfl, feedback, parabolic, radius, time, max_time;
   int requires_svi, requires_curv, has_max_time, pad;
@@ -79,9 +81,10 @@ struct Seg {
   int32_t slot[kMaxEngineJobs];
 };
 
-// N_r tables: T_NR + 3 * (MP / 4 - 1) + (0, 1, 2 for run lengths 4, 8, 16),
-// then the same twelve for LDS staging from T_NR + 12
-enum SegId { T_ALL, T_SVI, T_MARKS, T_RUNS, T_CHUNKS, T_JOBS, T_RUNJOBS, T_CURV, T_NR, T_COUNT = T_NR + 24 };
+// N_r tables: T_NR + 12 * variant + 3 * (MP / 4 - 1) + (0, 1, 2 for run lengths
+// 4, 8, 16); variants 0: one line, 1: LDS staging, 2: two lines, 3: four lines
+enum SegId { T_ALL, T_SVI, T_MARKS, T_RUNS, T_CHUNKS, T_JOBS, T_RUNJOBS, T_CURV, T_NR, T_COUNT = T_NR + 48 };
+constexpr int kNrLdsVariant = 1;
 
 struct Tables {
   Seg seg[T_COUNT];
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(256) k_eng_dilate(const JobDev* __restrict__ J
 __global__ void __launch_bounds__(kCompactThreads) k_eng_need_count(const JobDev* __restrict__ J, const Seg* S) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    compact_count_body(RunNeed{j.dil, (int64_t)j.d.ny * j.nzr}, j.nruns, j.ccounts, lb);
+    compact_count_body(UnitNeed{j.dil, j.d.ny, j.nzr, j.nyu, j.ylines}, j.nunits, j.ccounts, lb);
   }
 }
 
@@ -174,7 +177,7 @@ __global__ void __launch_bounds__(1024) k_eng_need_scan(const JobDev* __restrict
 __global__ void __launch_bounds__(kCompactThreads) k_eng_need_scatter(const JobDev* __restrict__ J, const Seg* S) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    compact_scatter_body(RunNeed{j.dil, (int64_t)j.d.ny * j.nzr}, j.nruns, j.ccounts, j.list, j.nruns,
+    compact_scatter_body(UnitNeed{j.dil, j.d.ny, j.nzr, j.nyu, j.ylines}, j.nunits, j.ccounts, j.list, j.nunits,
                          IdentityMap(), lb);
   }
 }
@@ -187,6 +190,18 @@ __global__ void __launch_bounds__(256) k_eng_nr_runs(const JobDev* __restrict__ 
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
     nr_runs_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.weights,
                      j.scale, j.negbv, const_cast<float*>(j.field), xcd_slice(lb, G), G);
+  }
+}
+
+// Units of Y adjacent y-lines
+template <int MP, int RL, int Y>
+__global__ void __launch_bounds__(256) k_eng_nr_lines(const JobDev* __restrict__ J, const Seg* S) {
+  ENG_FOR(S) {
+    const JobDev& j = J[jslot];
+    const int64_t nunit = j.ccounts[j.nchunks];
+    if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nunit;
+    nr_lines_body<MP, RL, Y>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nunit, j.cols, j.col_start,
+                             j.m, j.weights, j.scale, j.negbv, const_cast<float*>(j.field), xcd_slice(lb, G), G);
   }
 }
 
@@ -406,6 +421,9 @@ static void fill_jobdev(const EngJob& e, JobDev& j) {
     j.list = p->list;
     j.ccounts = p->ccounts;
     j.nruns = p->nruns;
+    j.ylines = p->ylines;
+    j.nyu = p->nyu;
+    j.nunits = p->nunits;
     j.nchunks = p->nchunks;
     j.cols = p->cols;
     j.col_start = p->col_start;
@@ -432,7 +450,7 @@ static int mp_of(const NrPlan* p) { return std::max(4, (p->m + 3) / 4 * 4); }
 // virtual-block counts of a job in each table
 static int64_t nr_grid(const EngJob& e) {
   // runs are not known on the host before the selection: size by the band
-  const int64_t est = std::max<int64_t>(e.nb / 4, 256);
+  const int64_t est = std::max<int64_t>(e.nb / 4 / e.plan->ylines, 256);
   const int threads = e.plan->lds ? 64 : 256;  // the LDS variant runs one wave per block
   int64_t g = (est + threads - 1) / threads;
   g = std::min<int64_t>(std::max<int64_t>(g, kXcds), e.plan->lds ? 2560 : 2048);
@@ -473,7 +491,8 @@ static int upload(Engine& E) {
       seg_add(T.seg[T_CHUNKS], slot, e.plan->nchunks);
       seg_add(T.seg[T_RUNJOBS], slot, 1);
       const int rl = e.plan->run_mode;
-      const int t = T_NR + 12 * (e.plan->lds ? 1 : 0) + 3 * (mp_of(e.plan) / 4 - 1) + (rl == 4 ? 0 : rl == 8 ? 1 : 2);
+      const int variant = e.plan->lds ? kNrLdsVariant : (e.plan->ylines == 2 ? 2 : e.plan->ylines == 4 ? 3 : 0);
+      const int t = T_NR + 12 * variant + 3 * (mp_of(e.plan) / 4 - 1) + (rl == 4 ? 0 : rl == 8 ? 1 : 2);
       seg_add(T.seg[t], slot, nr_grid(e));
     }
   }
@@ -492,7 +511,8 @@ static unsigned grid_of(const Engine& E, int t) {
     // persistent grid: a captured graph keeps its grid while the band sizes
     // change, so it gets headroom over the totals at capture time
     g = E.graphs ? sg.total + sg.total / 2 + 64 : sg.total;
-    g = std::min<int64_t>(g, t >= T_NR + 12 ? kEngGridWaves : kEngGrid);
+    g = std::min<int64_t>(g, t >= T_NR + 12 * kNrLdsVariant && t < T_NR + 12 * (kNrLdsVariant + 1) ? kEngGridWaves
+                                                                                                     : kEngGrid);
   }
   g = std::max<int64_t>(g, 1);
   return (unsigned)((g + kXcds - 1) / kXcds * kXcds);
@@ -528,6 +548,16 @@ static int prepare(Engine& E, int rel, int stats, int only_needed) {
   ENG_NRL(12, 4, 6); ENG_NRL(12, 8, 7); ENG_NRL(12, 16, 8);
   ENG_NRL(16, 4, 9); ENG_NRL(16, 8, 10); ENG_NRL(16, 16, 11);
 #undef ENG_NRL
+#define ENG_NRY(MPV, RLV, YV, I) ENG_LAUNCH((k_eng_nr_lines<MPV, RLV, YV>), T_NR + 12 * (YV == 2 ? 2 : 3) + (I), 256)
+  ENG_NRY(4, 4, 2, 0); ENG_NRY(4, 8, 2, 1); ENG_NRY(4, 16, 2, 2);
+  ENG_NRY(8, 4, 2, 3); ENG_NRY(8, 8, 2, 4); ENG_NRY(8, 16, 2, 5);
+  ENG_NRY(12, 4, 2, 6); ENG_NRY(12, 8, 2, 7); ENG_NRY(12, 16, 2, 8);
+  ENG_NRY(16, 4, 2, 9); ENG_NRY(16, 8, 2, 10); ENG_NRY(16, 16, 2, 11);
+  ENG_NRY(4, 4, 4, 0); ENG_NRY(4, 8, 4, 1);
+  ENG_NRY(8, 4, 4, 3); ENG_NRY(8, 8, 4, 4);
+  ENG_NRY(12, 4, 4, 6); ENG_NRY(12, 8, 4, 7);
+  ENG_NRY(16, 4, 4, 9); ENG_NRY(16, 8, 4, 10);
+#undef ENG_NRY
   // tile-mode plans (balls wider than 16 cells) run their own launches
   for (int slot : E.set) {
     EngJob& e = E.jobs[slot];

@@ -77,6 +77,19 @@ __global__ void __launch_bounds__(256)
                    gridDim.x);
 }
 
+// Units of Y adjacent y-lines sharing their occupancy windows.
+template <int MP, int RL, int Y>
+__global__ void __launch_bounds__(256)
+    k_nr_lines(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
+               const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
+               int m, const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
+               double* __restrict__ results) {
+  const int64_t nunit = *total;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nunit;
+  nr_lines_body<MP, RL, Y>(occ, OL, nzr, list, nunit, cols, col_start, m, w, scale, negbv, out,
+                           xcd_slice(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 // The same sums with the occupancy staged through LDS: one wave per block.
 template <int MP, int RL>
 __global__ void __launch_bounds__(64)
@@ -156,13 +169,32 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Run length of a run-mode plan (the arrays are sized for kRunLenMin).
-void nr_plan_set_run_len(NrPlan* p, int rl) {
+// Run length and lines per unit of a run-mode plan (the arrays are sized for
+// kRunLenMin and one line per unit).
+void nr_plan_set_shape(NrPlan* p, int rl, int ylines) {
   p->run_mode = rl;
+  p->ylines = ylines;
   p->nzr = (p->d.nz + rl - 1) / rl;
   p->nruns = (int64_t)p->d.nx * p->d.ny * p->nzr;
+  p->nyu = (p->d.ny + ylines - 1) / ylines;
+  p->nunits = (int64_t)p->d.nx * p->nyu * p->nzr;
   p->nmarks = p->nruns;
-  p->nchunks = (p->nruns + kCompactChunk - 1) / kCompactChunk;
+  p->nchunks = (p->nunits + kCompactChunk - 1) / kCompactChunk;
+}
+
+void nr_plan_set_run_len(NrPlan* p, int rl) { nr_plan_set_shape(p, rl, p->ylines > 0 ? p->ylines : 1); }
+
+// Shapes a run-mode plan can take: LDS staging needs one line and an
+// occupancy indexable in int32; several lines need consecutive-dy columns,
+// windows within the occupancy's y padding (m + lines - 1 <= kOccPad) and
+// runs of at most 8 nodes for four lines.
+bool nr_shape_allowed(const NrPlan* p, const NrShape& s) {
+  if (!p->run_mode || p->fft) return false;
+  if (s.rl != 4 && s.rl != 8 && s.rl != 16) return false;
+  if (s.lines != 1 && s.lines != 2 && s.lines != 4) return false;
+  if (s.lds) return s.lines == 1 && p->lds_ok;
+  if (s.lines > 1) return p->cols_contig && p->m + s.lines - 1 <= kOccPad && !(s.lines == 4 && s.rl > 8);
+  return true;
 }
 
 }  // namespace svimx
@@ -218,6 +250,11 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   p->run_mode = 0;
   p->lds = 0;
   p->lds_ok = OccLayout::of(nx, ny, nz).size() / 4 < (int64_t)INT32_MAX ? 1 : 0;
+  p->ylines = 1;
+  p->cols_contig = 1;  // every dx group's columns are a consecutive dy range
+  for (int g = 0; g < W; ++g)
+    for (int c = col_start[g] + 1; c < col_start[g + 1]; ++c)
+      if (cols[c].dy != cols[c - 1].dy + 1) p->cols_contig = 0;
   p->ntx = (nx + kTX - 1) / kTX;
   p->nty = (ny + kTY - 1) / kTY;
   p->ntz = (nz + kTZ - 1) / kTZ;
@@ -345,14 +382,14 @@ int svimx_nr_select(void* stream, void* plan) {
     hipLaunchKernelGGL(k_runs_dilate_y, dim3(blocks_for(p->nruns, 256)), dim3(256), 0, st, p->marks, p->dil,
                        p->d.ny, p->nzr, p->nruns);
     SVIMX_CHECK_LAUNCH();
-    RunNeed pred{p->dil, (int64_t)p->d.ny * p->nzr};
-    hipLaunchKernelGGL(k_compact_count<RunNeed>, dim3((unsigned)p->nchunks), dim3(kCompactThreads), 0, st, pred,
-                       p->nruns, p->ccounts);
+    UnitNeed pred{p->dil, p->d.ny, p->nzr, p->nyu, p->ylines};
+    hipLaunchKernelGGL(k_compact_count<UnitNeed>, dim3((unsigned)p->nchunks), dim3(kCompactThreads), 0, st, pred,
+                       p->nunits, p->ccounts);
     SVIMX_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, p->ccounts, p->nchunks);
     SVIMX_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_compact_scatter<RunNeed, IdentityMap>), dim3((unsigned)p->nchunks), dim3(kCompactThreads),
-                       0, st, pred, p->nruns, p->ccounts, p->list, p->nruns, IdentityMap());
+    hipLaunchKernelGGL((k_compact_scatter<UnitNeed, IdentityMap>), dim3((unsigned)p->nchunks),
+                       dim3(kCompactThreads), 0, st, pred, p->nunits, p->ccounts, p->list, p->nunits, IdentityMap());
     SVIMX_CHECK_LAUNCH();
     return SVIMX_OK;
   }
@@ -379,14 +416,21 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
   if (p->run_mode) {
     const int64_t* total = p->ccounts + p->nchunks;
     // grid: a multiple of the XCD count (the kernel's block remapping needs it)
-    const int grid = (int)std::min<int64_t>((p->nruns + 255) / 256 + kXcds - 1, 256 * 8) / kXcds * kXcds;
+    const int grid = (int)std::min<int64_t>((p->nunits + 255) / 256 + kXcds - 1, 256 * 8) / kXcds * kXcds;
     const int mp = (p->m + 3) / 4 * 4;
     // LDS variant: one wave per block, at most 10 resident per CU (LDS-bound)
     const int grid_lds = (int)std::min<int64_t>((p->nruns + 63) / 64 + kXcds - 1, 256 * 10) / kXcds * kXcds;
+#define SVIMX_LINES(MPV, RLV, YV)                                                                              \
+  hipLaunchKernelGGL((k_nr_lines<MPV, RLV, YV>), dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, \
+                     total, p->cols, p->col_start, p->m, p->weights, p->scale, p->negbv, nr_out, w.results)
 #define SVIMX_RUNS(MPV, RLV)                                                                                       \
   if (p->lds)                                                                                                      \
     hipLaunchKernelGGL((k_nr_lds<MPV, RLV>), dim3(grid_lds), dim3(64), 0, st, occupancy, OL, p->nzr, p->list,      \
                        total, p->cols, p->col_start, p->m, p->weights, p->scale, p->negbv, nr_out, w.results);    \
+  else if (p->ylines == 2)                                                                                         \
+    SVIMX_LINES(MPV, RLV, 2);                                                                                      \
+  else if (p->ylines == 4 && RLV < 16)                                                                             \
+    SVIMX_LINES(MPV, (RLV < 16 ? RLV : 8), 4);                                                                     \
   else                                                                                                             \
     hipLaunchKernelGGL((k_nr_runs<MPV, RLV>), dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, total, \
                        p->cols, p->ncols, p->weights, p->scale, p->negbv, nr_out, w.results)
@@ -405,6 +449,7 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
     }
 #undef SVIMX_RUNS_MP
 #undef SVIMX_RUNS
+#undef SVIMX_LINES
     SVIMX_CHECK_LAUNCH();
     return SVIMX_OK;
   }
@@ -425,18 +470,18 @@ int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t
   SVIMX_REQUIRE(p && phi && band && occupancy && nr_out, SVIMX_EBADARG, "null argument");
   if (!p->run_mode || p->fft) return SVIMX_OK;  // tile mode and the FFT have a single shape
   hipStream_t st = as_stream(stream);
-  // run lengths 4, 8, 16 with global windows, then the same with LDS staging
-  const int cands[kNrTuneCands] = {4, 8, 16, 4, 8, 16};
   hipEvent_t ev[2];
   SVIMX_CHECK_HIP(hipEventCreate(&ev[0]));
   SVIMX_CHECK_HIP(hipEventCreate(&ev[1]));
-  int best_rl = p->run_mode, best_lds = p->lds, rc = SVIMX_OK;
+  NrShape best_shape{p->run_mode, p->ylines, p->lds};
+  int rc = SVIMX_OK;
   double best = INFINITY;
   for (int i = 0; i < kNrTuneCands && !rc; ++i) {
     if (ms_out) ms_out[i] = NAN;
-    if (i >= 3 && !p->lds_ok) continue;
-    nr_plan_set_run_len(p, cands[i]);
-    p->lds = i >= 3 ? 1 : 0;
+    const NrShape sh = kNrTuneShapes[i];
+    if (!nr_shape_allowed(p, sh)) continue;
+    nr_plan_set_shape(p, sh.rl, sh.lines);
+    p->lds = sh.lds;
     rc = svimx_fractions(stream, phi, band, nb, nx, ny, nz, h, occupancy, p, cut_hi, ws, ws_bytes);
     if (!rc) rc = svimx_nr_select(stream, p);
     if (!rc) rc = svimx_nr_sum(stream, p, occupancy, nr_out, ws, ws_bytes);  // warm-up
@@ -450,15 +495,14 @@ int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t
     if (ms_out) ms_out[i] = ms / 3.0;
     if (ms / 3.0 < best) {
       best = ms / 3.0;
-      best_rl = cands[i];
-      best_lds = p->lds;
+      best_shape = sh;
     }
   }
   hipEventDestroy(ev[0]);
   hipEventDestroy(ev[1]);
-  // leave the plan's marks and run list those of the chosen run length
-  nr_plan_set_run_len(p, best_rl);
-  p->lds = best_lds;
+  // leave the plan's marks and run list those of the chosen shape
+  nr_plan_set_shape(p, best_shape.rl, best_shape.lines);
+  p->lds = best_shape.lds;
   if (!rc) rc = svimx_fractions(stream, phi, band, nb, nx, ny, nz, h, occupancy, p, cut_hi, ws, ws_bytes);
   if (!rc) rc = svimx_nr_select(stream, p);
   return rc;
@@ -472,16 +516,38 @@ int svimx_nr_plan_run_length(void* plan) {
 
 int svimx_nr_plan_set_run_length(void* plan, int rl) {
   NrPlan* p = static_cast<NrPlan*>(plan);
-  SVIMX_REQUIRE(p && p->run_mode && !p->fft && (rl == 4 || rl == 8 || rl == 16), SVIMX_EBADARG,
-                "run length needs a run-mode plan and 4, 8 or 16 nodes");
+  SVIMX_REQUIRE(p && nr_shape_allowed(p, NrShape{rl, p->ylines, p->lds}), SVIMX_EBADARG,
+                "run length %d is not available for this plan", rl);
   nr_plan_set_run_len(p, rl);
+  return SVIMX_OK;
+}
+
+int svimx_nr_plan_set_lines(void* plan, int lines) {
+  NrPlan* p = static_cast<NrPlan*>(plan);
+  SVIMX_REQUIRE(p && nr_shape_allowed(p, NrShape{p->run_mode, lines, p->lds}), SVIMX_EBADARG,
+                "%d lines per unit are not available for this plan", lines);
+  nr_plan_set_shape(p, p->run_mode, lines);
+  return SVIMX_OK;
+}
+
+int svimx_nr_plan_lines(void* plan) {
+  const NrPlan* p = static_cast<const NrPlan*>(plan);
+  return p && p->run_mode ? p->ylines : 0;
+}
+
+int svimx_nr_tune_shape(int i, int* run_length, int* lines, int* lds) {
+  SVIMX_REQUIRE(i >= 0 && i < kNrTuneCands && run_length && lines && lds, SVIMX_EBADARG, "bad tuning index");
+  *run_length = kNrTuneShapes[i].rl;
+  *lines = kNrTuneShapes[i].lines;
+  *lds = kNrTuneShapes[i].lds;
   return SVIMX_OK;
 }
 
 int svimx_nr_plan_set_staging(void* plan, int lds) {
   NrPlan* p = static_cast<NrPlan*>(plan);
   SVIMX_REQUIRE(p && (lds == 0 || lds == 1), SVIMX_EBADARG, "bad N_r staging argument");
-  SVIMX_REQUIRE(!lds || p->lds_ok, SVIMX_EBADARG, "grid too large for LDS staging");
+  SVIMX_REQUIRE(!p->run_mode || nr_shape_allowed(p, NrShape{p->run_mode, p->ylines, lds}), SVIMX_EBADARG,
+                "LDS staging needs one line per unit and a grid whose occupancy fits int32 indexing");
   p->lds = lds;
   return SVIMX_OK;
 }

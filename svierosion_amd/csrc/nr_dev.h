@@ -31,6 +31,28 @@ struct RunNeed {
   }
 };
 
+// need of unit u = (x, yu, zr): any of its lines y = yu * Y + 0..Y-1 (< ny)
+// has a needed run; with Y = 1 the unit is the run and this is RunNeed.
+struct UnitNeed {
+  const uint8_t* dil;
+  int ny, nzr, nyu, Y;
+  __device__ bool operator()(int64_t u) const {
+    const int zr = (int)(u % nzr);
+    const int64_t xyu = u / nzr;
+    const int yu = (int)(xyu % nyu);
+    const int64_t x = xyu / nyu;
+    const int64_t plane = (int64_t)ny * nzr;
+    uint8_t v = 0;
+    for (int yy = 0; yy < Y; ++yy) {
+      const int y = yu * Y + yy;
+      if (y >= ny) break;
+      const int64_t r = (x * ny + y) * nzr + zr;
+      for (int a = 0; a <= 3 && a <= x; ++a) v |= dil[r - a * plane];
+    }
+    return v != 0;
+  }
+};
+
 // ---------------------------------------------------------------- run-mode summation
 template <int K, int MP, int RL, int WIN>
 __device__ __forceinline__ void column_taps(const float* __restrict__ wc, const float (&win)[WIN],
@@ -41,6 +63,106 @@ __device__ __forceinline__ void column_taps(const float* __restrict__ wc, const 
       const float wt = wc[t];
 #pragma unroll
       for (int j = 0; j < RL; ++j) part[j] += wt * win[j + MP - K + t];
+    }
+  }
+}
+
+template <int MP, int RL>
+__device__ __forceinline__ void taps_of(int k, const float* __restrict__ wc, const float (&win)[RL + 2 * MP],
+                                        float (&part)[RL]) {
+  switch (k) {
+    case 0: column_taps<0, MP, RL>(wc, win, part); break;
+    case 1: column_taps<1, MP, RL>(wc, win, part); break;
+    case 2: column_taps<2, MP, RL>(wc, win, part); break;
+    case 3: column_taps<3, MP, RL>(wc, win, part); break;
+    case 4: column_taps<4, MP, RL>(wc, win, part); break;
+    case 5: column_taps<5, MP, RL>(wc, win, part); break;
+    case 6: column_taps<6, MP, RL>(wc, win, part); break;
+    case 7: column_taps<7, MP, RL>(wc, win, part); break;
+    case 8: column_taps<8, MP, RL>(wc, win, part); break;
+    case 9: column_taps<9, MP, RL>(wc, win, part); break;
+    case 10: column_taps<10, MP, RL>(wc, win, part); break;
+    case 11: column_taps<11, MP, RL>(wc, win, part); break;
+    case 12: column_taps<12, MP, RL>(wc, win, part); break;
+    case 13: column_taps<13, MP, RL>(wc, win, part); break;
+    case 14: column_taps<14, MP, RL>(wc, win, part); break;
+    case 15: column_taps<15, MP, RL>(wc, win, part); break;
+    case 16: column_taps<16, MP, RL>(wc, win, part); break;
+    default: break;
+  }
+}
+
+// Y lines per unit: one lane sums the RL-node z-runs of Y adjacent y-lines.
+// Window line y0 + dy' of plane x + dx feeds column (dx, dy' - yy) of line yy,
+// so every line's columns are applied in the order of the one-line kernel
+// (bit-identical sums) while each window is loaded once for the Y lines.
+// Needs each dx group's columns to have consecutive dy.
+template <int MP, int RL, int Y>
+__device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
+                                              const int32_t* __restrict__ list, int64_t nunit,
+                                              const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
+                                              int m, const float* __restrict__ w, float scale, float negbv,
+                                              float* __restrict__ out, int64_t lb, int64_t G) {
+  constexpr int WIN = RL + 2 * MP;
+  const uint32_t nyu = (uint32_t)((OL.ny + Y - 1) / Y);
+  const int64_t stride = G * blockDim.x;
+  for (int64_t first = lb * blockDim.x; first < nunit; first += stride) {
+    const int64_t it = first + threadIdx.x;
+    const bool live = it < nunit;
+    const uint32_t u = (uint32_t)list[live ? it : first];
+    const uint32_t xyu = u / (uint32_t)nzr;
+    const int zr = (int)(u - xyu * (uint32_t)nzr);
+    const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
+    const int z0 = zr * RL;
+    float acc[Y][RL], part[Y][RL];
+#pragma unroll
+    for (int yy = 0; yy < Y; ++yy)
+#pragma unroll
+      for (int j = 0; j < RL; ++j) acc[yy][j] = part[yy][j] = 0.0f;
+    bool first_group = true;
+    for (int g = 0; g <= 2 * m; ++g) {
+      const int c0 = col_start[g], ncol = col_start[g + 1] - c0;
+      if (ncol == 0) continue;
+      if (!first_group) {
+#pragma unroll
+        for (int yy = 0; yy < Y; ++yy)
+#pragma unroll
+          for (int j = 0; j < RL; ++j) {
+            acc[yy][j] += part[yy][j];
+            part[yy][j] = 0.0f;
+          }
+      }
+      first_group = false;
+      const int dx = cols[c0].dx, dyf = cols[c0].dy;
+      for (int t = 0; t < ncol + Y - 1; ++t) {
+        const float4* src = reinterpret_cast<const float4*>(occ + OL.at(x + dx, y0 + dyf + t, z0 - MP));
+        float win[WIN];
+#pragma unroll
+        for (int q = 0; q < WIN / 4; ++q) {
+          const float4 v = src[q];
+          win[4 * q] = v.x;
+          win[4 * q + 1] = v.y;
+          win[4 * q + 2] = v.z;
+          win[4 * q + 3] = v.w;
+        }
+#pragma unroll
+        for (int yy = 0; yy < Y; ++yy) {
+          const int ci = t - yy;
+          if (ci >= 0 && ci < ncol) {
+            const NrColumn col = cols[c0 + ci];
+            taps_of<MP, RL>(-col.dz0, w + col.woff, win, part[yy]);
+          }
+        }
+      }
+    }
+    if (!live) continue;
+#pragma unroll
+    for (int yy = 0; yy < Y; ++yy) {
+      if (y0 + yy >= OL.ny) break;
+      const int64_t base = ((int64_t)x * OL.ny + y0 + yy) * OL.nz + z0;
+#pragma unroll
+      for (int j = 0; j < RL; ++j)
+        if (z0 + j < OL.nz) out[base + j] = ((acc[yy][j] + part[yy][j]) * scale) * negbv + 1.0f;
     }
   }
 }

@@ -3,10 +3,12 @@
 //
 // Two evaluation modes:
 //  * runs  (m <= kRunMaxM): the unit of work is a z-run of run_len nodes
-//    (4, 8 or 16: `run_mode` holds the run length, 0 means tile mode); the
-//    fractions pass marks the run holding each closest point's tricubic
-//    stencil base, N_r selection dilates the marks by the stencil (+0..3 in x
-//    and y) and compacts them on the device, and one lane sums one run.
+//    (4, 8 or 16: `run_mode` holds the run length, 0 means tile mode) on
+//    `ylines` adjacent y-lines (1, 2 or 4: the lines share every occupancy
+//    window they read); the fractions pass marks the run holding each closest
+//    point's tricubic stencil base, N_r selection dilates the marks by the
+//    stencil (+0..3 in x and y) and compacts the units holding a needed run
+//    on the device, and one lane sums one unit.
 //  * tiles (larger balls): 4x8x8 tiles streamed through LDS x-plane by x-plane.
 #pragma once
 
@@ -20,7 +22,12 @@ namespace svimx {
 constexpr int kTX = 4, kTY = 8, kTZ = 8;   // tile mode: one wave per x-layer of 8x8 nodes
 constexpr int kRunLen = 8;                 // run mode: default nodes per z-run
 constexpr int kRunLenMin = 4, kRunLenMax = 16;  // run lengths the tuner chooses from
-constexpr int kNrTuneCands = 6;            // run lengths 4, 8, 16, each with global or LDS windows
+constexpr int kNrTuneCands = 11;           // (run length, lines, LDS) shapes the autotuner times
+struct NrShape {
+  int rl, lines, lds;
+};
+constexpr NrShape kNrTuneShapes[kNrTuneCands] = {{4, 1, 0}, {8, 1, 0}, {16, 1, 0}, {4, 2, 0}, {8, 2, 0}, {16, 2, 0},
+                                                 {4, 4, 0}, {8, 4, 0}, {4, 1, 1}, {8, 1, 1}, {16, 1, 1}};
 constexpr int kRunMaxM = 16;               // largest ball half-width handled in run mode (= kOccPad)
 
 struct NrColumn {
@@ -37,6 +44,10 @@ struct NrPlan {
   int run_mode;       // 0: tile mode; else the run length (4, 8 or 16)
   int lds;            // run mode: 1 stages the occupancy through LDS (same sums, same order)
   int lds_ok;         // the occupancy's float4 count fits the LDS kernel's int32 indexing
+  int ylines;         // run mode: y-lines per unit (1, 2, 4)
+  int nyu;            // units per x-plane column: ceil(ny / ylines)
+  int64_t nunits;     // nx * nyu * nzr (the compacted list holds unit indices)
+  int cols_contig;    // each dx group's columns have consecutive dy (needed for ylines > 1)
   // tile mode
   int ntx, nty, ntz;
   int64_t ntiles;
@@ -62,8 +73,11 @@ struct NrPlan {
   std::vector<double> weights_host;  // the (2m+1)^3 weights, for switching to the FFT
 };
 
-// Run length of a run-mode plan (its arrays are sized for kRunLenMin).
+// Run length and lines per unit of a run-mode plan (its arrays are sized for
+// kRunLenMin and one line); nr_plan_set_run_len keeps the lines.
+void nr_plan_set_shape(NrPlan* p, int rl, int ylines);
 void nr_plan_set_run_len(NrPlan* p, int rl);
+bool nr_shape_allowed(const NrPlan* p, const NrShape& s);
 
 // Tile mode: mark the tiles covering the tricubic stencil [base, base+3]^3.
 __device__ __forceinline__ void mark_stencil_tiles(uint8_t* marks, int nty, int ntz, int bx, int by, int bz) {

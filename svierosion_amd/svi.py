@@ -57,6 +57,20 @@ def sphere_exterior_fraction(radius_ball: float, radius_sphere: float) -> float:
     return 0.5 + 3.0 * r / (16.0 * R)
 
 
+_TUNE_SHAPES = 11  # SVIMX_NR_TUNE_SHAPES
+
+
+def tune_shape(i: int) -> tuple:
+    """(run length, lines, lds) of the autotuner's shape i."""
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    _abi.check(_abi.lib.svimx_nr_tune_shape(int(i), C.byref(a), C.byref(b), C.byref(c)), "tuning shape")
+    return a.value, b.value, bool(c.value)
+
+
+def _shape_name(rl: int, lines: int, lds: bool) -> str:
+    return f"{rl}/lds" if lds else (f"{rl}" if lines <= 1 else f"{rl}x{lines}")
+
+
 class BallConvolution:
     """Ball weights of a grid as a libsvimx plan; evaluates N_r on the device.
 
@@ -136,6 +150,20 @@ class BallConvolution:
         _abi.check(_abi.lib.svimx_nr_plan_set_run_length(self.plan, int(value)), "N_r run length")
 
     @property
+    def lines(self) -> int:
+        """y-lines per unit of the direct summation (1, 2 or 4; 0 in tile mode)."""
+        return int(_abi.lib.svimx_nr_plan_lines(self.plan))
+
+    @lines.setter
+    def lines(self, value: int) -> None:
+        _abi.check(_abi.lib.svimx_nr_plan_set_lines(self.plan, int(value)), "N_r lines per unit")
+
+    @property
+    def shape_name(self) -> str:
+        """The summation shape as the tuner names it ("8", "8x2", "8/lds")."""
+        return _shape_name(self.run_length, self.lines, self.staging == "lds")
+
+    @property
     def staging(self) -> str:
         """How the direct summation reads the occupancy: "lds" (staged per wave
         through LDS) or "l1" (per-run windows)."""
@@ -148,20 +176,20 @@ class BallConvolution:
         _abi.check(_abi.lib.svimx_nr_plan_set_staging(self.plan, 1 if value == "lds" else 0), "N_r staging")
 
     def tune(self, ls, cut_hi: float) -> dict:
-        """Autotune the summation's z-run length (4, 8 or 16 nodes) and its
-        staging (windows through the L1, or the occupancy staged in LDS) on the
-        level set's current field; returns {"<run length>" or "<run length>/lds":
-        ms per launch}.  The N_r values the velocity reads do not depend on the
-        choice.  Leaves the occupancy and N_r marks of ``ls``'s current field in
-        place."""
-        ms = (C.c_double * 6)()
+        """Autotune the summation's shape (z-run length 4/8/16, y-lines per unit
+        1/2/4, or the LDS-staged one-line kernel) on the level set's current
+        field; returns {shape name ("8", "8x2", "8/lds"): ms per launch} for
+        the shapes this plan can take.  The N_r values the velocity reads do
+        not depend on the choice.  Leaves the occupancy and N_r marks of
+        ``ls``'s current field in place."""
+        ms = (C.c_double * _TUNE_SHAPES)()
         field = torch.empty(ls.grid.size, dtype=torch.float32, device=self.device)
         _abi.check(_abi.lib.svimx_nr_plan_tune(_abi.stream(), self.plan, _abi.ptr(ls.flat), _abi.ptr(ls._band),
                                                ls.nb, *ls.dims, ls.h, float(cut_hi), _abi.ptr(ls.occupancy),
                                                _abi.ptr(field), _abi.ptr(ls.ws), ls.ws.numel(), ms), "N_r tuning")
         if self.run_length == 0:
             return {}
-        return {"4": ms[0], "8": ms[1], "16": ms[2], "4/lds": ms[3], "8/lds": ms[4], "16/lds": ms[5]}
+        return {_shape_name(*tune_shape(i)): ms[i] for i in range(_TUNE_SHAPES) if not math.isnan(ms[i])}
 
     def interior_volume(self, fractions: torch.Tensor, nodes: torch.Tensor | None = None) -> torch.Tensor:
         """Dense |B_r(x) cap Omega| (derived from N_r)."""

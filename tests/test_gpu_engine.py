@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 def _case(name):
     from svierosion_amd.api import ErosionParams
     from svierosion_amd.shapes import cube, icosphere
-    name = name.removesuffix("_lds")
+    name = name.removesuffix("_lds").removesuffix("_y2").removesuffix("_y4")
     if name == "svi_cube":
         return cube(1.0), ErosionParams(law="svi", radius=0.2, k=1, s=1, t=0.9, resolution=40), 0.93, 400
     if name == "svi_sphere_k2":
@@ -37,7 +37,7 @@ def _sim(name):
     from svierosion_amd.levelset import LevelSet
     from svierosion_amd.sdf import mesh_to_sdf
     mesh, p, target, max_steps = _case(name)
-    base = name.removesuffix("_lds")
+    base = name.removesuffix("_lds").removesuffix("_y2").removesuffix("_y4")
     grid = make_grid(mesh, p)
     if base not in _PHI:
         _PHI[base] = mesh_to_sdf(mesh, grid, band=p.band)
@@ -47,7 +47,11 @@ def _sim(name):
     sim = ErosionSimulation(ls, law, cfl=p.cfl, feedback_cfl=p.feedback_cfl, reinit_distance=p.reinit_distance,
                             reinit_iterations=p.reinit_iterations, nr_method="fft" if name == "svi_fft" else "auto")
     if sim.conv is not None and sim.conv.method == "direct":
-        sim.conv.staging = "lds" if name.endswith("_lds") else "l1"
+        sim.conv.staging, sim.conv.lines = "l1", 1
+        if name.endswith("_lds"):
+            sim.conv.staging = "lds"
+        elif name.endswith("_y2") or name.endswith("_y4"):
+            sim.conv.lines = int(name[-1])
     return sim, target, max_steps
 
 
@@ -86,7 +90,8 @@ def _engine_run(names, graphs):
     return out
 
 
-@pytest.mark.parametrize("name", ["svi_cube", "svi_sphere_k2", "mean", "gauss", "svi_fft", "svi_cube_lds"])
+@pytest.mark.parametrize("name", ["svi_cube", "svi_sphere_k2", "mean", "gauss", "svi_fft", "svi_cube_lds",
+                                  "svi_cube_y2", "svi_sphere_k2_y4"])
 def test_engine_job_equals_single_job_path(device, name):
     sim, target, max_steps = _sim(name)
     r = sim.run(target_fraction=target, max_steps=max_steps)
@@ -100,14 +105,16 @@ def test_engine_job_equals_single_job_path(device, name):
         assert any(rec.reinit for rec in sim.history)
 
 
-def test_lds_staging_is_result_neutral_in_the_engine(device):
-    """The engine's LDS-staged N_r family reproduces the single-job path that
-    reads its windows through the L1, bit for bit."""
+@pytest.mark.parametrize("shape", ["lds", "y2", "y4"])
+def test_nr_shapes_are_result_neutral_in_the_engine(device, shape):
+    """The engine's LDS-staged and multi-line N_r families reproduce the
+    single-job path with one-line windows read through the L1, bit for bit."""
     sim, target, max_steps = _sim("svi_sphere_k2")
-    assert sim.conv.staging == "l1"
+    assert sim.conv.staging == "l1" and sim.conv.lines == 1
     r = sim.run(target_fraction=target, max_steps=max_steps)
-    got = _engine_run(["svi_sphere_k2_lds"], graphs=True)["svi_sphere_k2_lds"]
-    assert got[0].conv.staging == "lds"
+    name = "svi_sphere_k2_" + shape
+    got = _engine_run([name], graphs=True)[name]
+    assert got[0].conv.shape_name.endswith({"lds": "/lds", "y2": "x2", "y4": "x4"}[shape])
     assert got[1] == r.stop_reason
     _assert_same_run(sim, got[0])
 

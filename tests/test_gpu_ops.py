@@ -218,18 +218,20 @@ def test_nr_run_length_tuning_is_result_neutral(device):
         sims.append(sim)
     a, b = sims
     assert a.conv.run_length == 8 and b.conv.run_length in (4, 8, 16)
-    assert set(b.tuning) == {"4", "8", "16", "4/lds", "8/lds", "16/lds"}
+    shapes = {"4", "8", "16", "4x2", "8x2", "16x2", "4x4", "8x4", "4/lds", "8/lds", "16/lds"}
+    assert {"4", "8", "16"} <= set(b.tuning) <= shapes
     assert all(v > 0 for v in b.tuning.values())
     assert [(r.volume, r.dt) for r in a.history] == [(r.volume, r.dt) for r in b.history]
     assert torch.equal(a.ls.flat, b.ls.flat)
 
 
 @pytest.mark.parametrize("r", [0.12, 0.3, 0.45, 0.7])
-def test_nr_lds_staging_bit_identical(device, r):
-    """Staging the occupancy through LDS changes how the windows are read, not
-    the sums: N_r at every evaluated node is bit-identical to the per-run
-    windows, for each run length, for balls of 2 to 14 cells half-width, and
-    within 2e-5 of the oracle's FFT at the nodes it was asked for."""
+def test_nr_shapes_bit_identical(device, r):
+    """The summation shapes (LDS staging; 2 or 4 y-lines per unit sharing their
+    windows) change how the windows are read, not the sums: N_r at every node
+    the one-line kernel evaluates is bit-identical for each run length, for
+    balls of 2 to 15 cells half-width (shapes a ball cannot take are refused),
+    and within 2e-5 of the oracle's FFT at the nodes it was asked for."""
     from svierosion_amd.svi import BallConvolution
     ls, ref, g = _pair(distort=True, R=1.0, h=0.05, pad=1.0)
     conv = BallConvolution(g, r, device="cuda", method="direct")
@@ -239,18 +241,30 @@ def test_nr_lds_staging_bit_identical(device, r):
     f = ref.dense_fractions()
     Nref = F.exterior_fraction_fft(f, F.ball_kernel_hat(g.shape, F.ball_cell_weights(r, g.spacing)), g.spacing, r)
     idx = nodes.long().cpu().numpy()
+    tried = 0
     for rl in (4, 8, 16):
+        conv.staging, conv.lines = "l1", 1
         conv.run_length = rl
-        outs = []
-        for staging in ("l1", "lds"):
-            conv.staging = staging
-            outs.append(conv.exterior_fraction(occ, nodes))
-        a, b = outs
-        assert torch.equal(torch.isnan(a), torch.isnan(b)), (rl, "node sets differ")
+        a = conv.exterior_fraction(occ, nodes)
         ok = ~torch.isnan(a)
-        assert torch.equal(a[ok], b[ok]), (rl, float((a[ok] - b[ok]).abs().max()))
-        got = b.flatten().cpu().numpy()[idx]
-        assert np.abs(got - Nref.ravel()[idx]).max() < 2e-5, rl
+        for staging, lines in (("lds", 1), ("l1", 2), ("l1", 4)):
+            conv.staging, conv.lines = "l1", 1
+            try:
+                conv.lines = lines
+                conv.staging = staging
+            except ValueError:
+                assert lines == 4 and (rl == 16 or conv.m + 3 > 16) or lines == 2 and conv.m + 1 > 16
+                continue
+            tried += 1
+            b = conv.exterior_fraction(occ, nodes)
+            assert not torch.isnan(b[ok]).any(), (rl, staging, lines, "node missing")
+            if lines == 1:
+                assert torch.equal(torch.isnan(a), torch.isnan(b)), (rl, staging, "node sets differ")
+            assert torch.equal(a[ok], b[ok]), (rl, staging, lines, float((a[ok] - b[ok]).abs().max()))
+            got = b.flatten().cpu().numpy()[idx]
+            assert np.abs(got - Nref.ravel()[idx]).max() < 2e-5, (rl, staging, lines)
+        conv.staging, conv.lines = "l1", 1
+    assert tried >= 6
 
 
 def test_nr_fft_matches_direct_and_oracle(device):

@@ -55,7 +55,7 @@ def build_workload(args):
         raise SystemExit(f"workload invalid: max N_r {nmax:.4f} >= t {law.t}")
     info = {"triangles": int(len(mesh.faces)), "grid": list(grid.shape), "nodes": int(grid.size),
             "band_nodes": int(ls.nb), "mesh_s": round(t1 - t0, 2), "sdf_setup_s": round(t2 - t1, 2),
-            "nmax0": round(nmax, 5), "m": int(sim.conv.m), "nr_run_length": sim.conv.run_length,
+            "nmax0": round(nmax, 5), "m": int(sim.conv.m), "nr_shape": sim.conv.shape_name,
             "nr_tuning_ms": {str(k): round(v, 4) for k, v in sim.tuning.items()}}
     return mesh, grid, ls, sim, law, info
 
@@ -227,12 +227,19 @@ def main(argv=None):
     taps = float((sim.conv.weights > 0).sum())
     flops = 2.0 * taps * n_out
     achieved = alg_bytes / (nr_ms * 1e-3) / 1e9 if nr_ms > 0 else 0.0
+    conv = sim.conv
+    if conv.method == "fft":
+        kname = "k_fft_lines"
+    elif conv.run_length == 0:
+        kname = "k_nr_planes"
+    else:
+        kname = "k_nr_lds" if conv.staging == "lds" else ("k_nr_lines" if conv.lines > 1 else "k_nr_runs")
     traffic = None
-    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_nr_runs_traffic.json")
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{kname}_traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))
-        traffic = t["read_bytes_per_dispatch"] + t["write_bytes_per_dispatch"]
-    kname = "k_nr_runs" if sim.conv is not None and sim.conv.m <= 16 else "k_nr_planes"
+        if t.get("shape", conv.shape_name) == conv.shape_name:
+            traffic = t["read_bytes_per_dispatch"] + t["write_bytes_per_dispatch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)" if traffic else None,
@@ -262,7 +269,7 @@ def main(argv=None):
             "config": {"workload": "1.3M-triangle noisy ellipsoid scan, SVI k=2 s=1 t=0.95 r=2.0, res 400",
                        "triangles": info["triangles"], "grid": info["grid"], "band_nodes": info["band_nodes"],
                        "reinit_in_window": reinits, "stage_ms": {k: round(v, 3) for k, v in stage_ms.items()},
-                       "nr_run_length": info["nr_run_length"], "nr_tuning_ms": info["nr_tuning_ms"],
+                       "nr_shape": info["nr_shape"], "nr_tuning_ms": info["nr_tuning_ms"],
                        "launches": launches},
             "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "served": served}
     print(json.dumps(line), flush=True)

@@ -386,6 +386,10 @@ int svimx_engine_use_graphs(void* engine, int on);
 /* Ids (up to cap) of jobs that stopped since the last call, oldest first. */
 int svimx_engine_stopped(void* engine, int* ids, int cap, int* n);
 /* Milliseconds spent in ticks / secant retries / reinitialisations so far. */
+/* Ticks whose graph capture was invalidated (by a device-wide operation in
+ * another thread, e.g. a synchronous free) and that ran their launches eagerly
+ * instead; results are unaffected. */
+int64_t svimx_engine_capture_fallbacks(void* engine);
 int svimx_engine_timers(void* engine, double* out3);
 
 #ifdef __cplusplus

@@ -89,6 +89,7 @@ _SIGNATURES = {
     "svimx_nr_plan_run_length": (_I, [_P]),
     "svimx_nr_plan_set_run_length": (_I, [_P, _I]),
     "svimx_nr_plan_set_lines": (_I, [_P, _I]),
+    "svimx_engine_capture_fallbacks": (_I64, [_P]),
     "svimx_nr_plan_lines": (_I, [_P]),
     "svimx_nr_tune_shape": (_I, [_I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "svimx_nr_plan_set_staging": (_I, [_P, _I]),

@@ -113,7 +113,8 @@ class ServeReport:
         return {"completed": len(self.done), "failed": len(self.completions) - len(self.done),
                 "makespan_s": round(self.makespan, 3), "erosions_per_hour": round(self.erosions_per_hour(), 1),
                 "latency_p50_s": round(self.latency(0.5), 3), "latency_p99_s": round(self.latency(0.99), 3),
-                "peak_device_gb": round(self.peak_device_bytes / 1e9, 3), "ticks": self.ticks}
+                "peak_device_gb": round(self.peak_device_bytes / 1e9, 3), "ticks": self.ticks,
+                "capture_fallbacks": int(self.engine_ms.get("capture_fallbacks", 0))}
 
 
 class _Worker:

@@ -372,6 +372,7 @@ struct Engine {
   bool graphs = false;
   hipGraphExec_t graph = nullptr;
   std::vector<int> graph_set;  // job slots the captured graph was built for
+  int64_t capture_fallbacks = 0;  // ticks run eagerly because their capture was invalidated
   bool graph_has_tile = false;
   double ms_tick = 0, ms_secant = 0, ms_reinit = 0;
   std::vector<int> set;      // job slots of the current phase
@@ -633,15 +634,26 @@ static int launch_main(Engine& E) {
   if (!E.graphs || tile_jobs_in(E)) return main_sequence(E);
   if (!E.graph || E.graph_set != E.set) {
     drop_graph(E);
-    SVIMX_CHECK_HIP(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
+    // Relaxed: other threads of the process (set-up and meshing workers) keep
+    // allocating and freeing while this thread captures.
+    SVIMX_CHECK_HIP(hipStreamBeginCapture(E.st, hipStreamCaptureModeRelaxed));
     int rc = main_sequence(E);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(E.st, &g);
-    if (rc) {
+    if (rc || e != hipSuccess || !g) {
+      // The capture was invalidated (by a launch error, or by a device-wide
+      // operation such as a synchronous free in another thread of the
+      // process): none of the recorded work ran.  Clear the error and run
+      // this tick eagerly; the next tick captures again.  A genuine launch
+      // error fails the eager sequence as well and is returned from there.
       if (g) hipGraphDestroy(g);
-      return rc;
+      (void)hipGetLastError();
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      SVIMX_CHECK_HIP(hipStreamIsCapturing(E.st, &cs));
+      SVIMX_REQUIRE(cs == hipStreamCaptureStatusNone, SVIMX_EDEVICE, "engine stream left in capture state");
+      ++E.capture_fallbacks;
+      return main_sequence(E);
     }
-    SVIMX_CHECK_HIP(e);
     e = hipGraphInstantiate(&E.graph, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     SVIMX_CHECK_HIP(e);
@@ -1168,6 +1180,11 @@ int svimx_engine_timers(void* engine, double* out3) {
   out3[1] = E->ms_secant;
   out3[2] = E->ms_reinit;
   return SVIMX_OK;
+}
+
+int64_t svimx_engine_capture_fallbacks(void* engine) {
+  const Engine* E = static_cast<const Engine*>(engine);
+  return E ? E->capture_fallbacks : 0;
 }
 
 }  // extern "C"

@@ -186,7 +186,8 @@ class BatchEngine:
     def timers_ms(self) -> dict:
         out = (C.c_double * 3)()
         _abi.check(_abi.lib.svimx_engine_timers(self._h, out))
-        return {"tick": out[0], "secant": out[1], "reinit": out[2]}
+        return {"tick": out[0], "secant": out[1], "reinit": out[2],
+                "capture_fallbacks": int(_abi.lib.svimx_engine_capture_fallbacks(self._h))}
 
 
 def run_result(engine: BatchEngine, job: int):

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 from pathlib import Path
 
 import torch
@@ -144,6 +145,13 @@ def _load():
 
 
 lib = _load()
+
+# Held around the calls that allocate or free device memory synchronously
+# (N_r plan creation and destruction, which run on set-up threads or in the
+# garbage collector) and around the batch engine's ticks, which may capture a
+# HIP graph: a synchronous free in another thread would otherwise invalidate
+# a capture in flight.
+device_lock = threading.RLock()
 
 
 class SvimxError(RuntimeError):

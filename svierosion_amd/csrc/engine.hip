@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) k_eng_nr_runs(const JobDev* __restrict__ 
     const int64_t nrun = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
     nr_runs_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.weights,
-                     j.scale, j.negbv, const_cast<float*>(j.field), xcd_slice(lb, G), G);
+                     j.scale, j.negbv, const_cast<float*>(j.field), lb, G);
   }
 }
 
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) k_eng_nr_lines(const JobDev* __restrict__
     const int64_t nunit = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nunit;
     nr_lines_body<MP, RL, Y>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nunit, j.cols, j.col_start,
-                             j.m, j.weights, j.scale, j.negbv, const_cast<float*>(j.field), xcd_slice(lb, G), G);
+                             j.m, j.weights, j.scale, j.negbv, const_cast<float*>(j.field), lb, G);
   }
 }
 
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(64) k_eng_nr_lds(const JobDev* __restrict__ J,
     const int64_t nrun = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
     nr_lds_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.col_start, j.m,
-                        j.weights, j.scale, j.negbv, const_cast<float*>(j.field), slab, clu, xcd_slice(lb, G), G);
+                        j.weights, j.scale, j.negbv, const_cast<float*>(j.field), slab, clu, lb, G);
   }
 }
 

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256)
   const int64_t nrun = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
   // neighbouring runs read the same occupancy columns: keep them on one XCD's L2
-  nr_runs_body<MP, RL>(occ, OL, nzr, list, nrun, cols, ncols, w, scale, negbv, out, xcd_slice(blockIdx.x, gridDim.x),
+  nr_runs_body<MP, RL>(occ, OL, nzr, list, nrun, cols, ncols, w, scale, negbv, out, blockIdx.x,
                    gridDim.x);
 }
 
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256)
   const int64_t nunit = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nunit;
   nr_lines_body<MP, RL, Y>(occ, OL, nzr, list, nunit, cols, col_start, m, w, scale, negbv, out,
-                           xcd_slice(blockIdx.x, gridDim.x), gridDim.x);
+                           blockIdx.x, gridDim.x);
 }
 
 // The same sums with the occupancy staged through LDS: one wave per block.
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(64)
   const int64_t nrun = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
   nr_lds_body<MP, RL>(occ, OL, nzr, list, nrun, cols, col_start, m, w, scale, negbv, out, slab, clu,
-                      xcd_slice(blockIdx.x, gridDim.x), gridDim.x);
+                      blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- tile-mode summation

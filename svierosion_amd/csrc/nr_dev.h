@@ -53,6 +53,26 @@ struct UnitNeed {
   }
 };
 
+// Wave-level split of an n-entry work list over a launch of G blocks: the
+// work is cut into 64-entry wave items, each XCD (block lb runs on XCD lb mod
+// 8; G is a multiple of 8) takes a contiguous eighth of the items actually
+// present (balanced whatever the launch was sized for, and neighbouring
+// items share that XCD's L2), and the XCD's waves stride over its share.
+struct WaveSplit {
+  int64_t first, end, step;  // item range [first, end) of this wave, stride step
+};
+__device__ __forceinline__ WaveSplit wave_split(int64_t n, int64_t lb, int64_t G) {
+  const int64_t wpb = blockDim.x >> 6;              // waves per block
+  const int64_t items = (n + 63) >> 6;
+  const int64_t per = (items + kXcds - 1) / kXcds;  // items per XCD
+  const int64_t x = lb % kXcds, lbx = lb / kXcds;
+  WaveSplit s;
+  s.first = x * per + lbx * wpb + (threadIdx.x >> 6);
+  s.end = min(x * per + per, items);
+  s.step = (G / kXcds) * wpb;
+  return s;
+}
+
 // ---------------------------------------------------------------- run-mode summation
 template <int K, int MP, int RL, int WIN>
 __device__ __forceinline__ void column_taps(const float* __restrict__ wc, const float (&win)[WIN],
@@ -105,11 +125,11 @@ __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, con
                                               float* __restrict__ out, int64_t lb, int64_t G) {
   constexpr int WIN = RL + 2 * MP;
   const uint32_t nyu = (uint32_t)((OL.ny + Y - 1) / Y);
-  const int64_t stride = G * blockDim.x;
-  for (int64_t first = lb * blockDim.x; first < nunit; first += stride) {
-    const int64_t it = first + threadIdx.x;
+  const WaveSplit ws = wave_split(nunit, lb, G);
+  for (int64_t item = ws.first; item < ws.end; item += ws.step) {
+    const int64_t it = item * 64 + (threadIdx.x & 63);
     const bool live = it < nunit;
-    const uint32_t u = (uint32_t)list[live ? it : first];
+    const uint32_t u = (uint32_t)list[live ? it : item * 64];
     const uint32_t xyu = u / (uint32_t)nzr;
     const int zr = (int)(u - xyu * (uint32_t)nzr);
     const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
@@ -167,8 +187,8 @@ __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, con
   }
 }
 
-// One lane per z-run (RL nodes) of the run list; lb is the (XCD-remapped)
-// block index among G blocks of blockDim.x threads.
+// One lane per z-run (RL nodes) of the run list; lb is the block index among
+// G blocks of blockDim.x threads (work split by wave_split).
 template <int MP, int RL>
 __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
                                              const int32_t* __restrict__ list, int64_t nrun,
@@ -176,11 +196,11 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
                                              const float* __restrict__ w, float scale, float negbv,
                                              float* __restrict__ out, int64_t lb, int64_t G) {
   constexpr int WIN = RL + 2 * MP;
-  const int64_t stride = G * blockDim.x;
-  for (int64_t first = lb * blockDim.x; first < nrun; first += stride) {
-    const int64_t it = first + threadIdx.x;
+  const WaveSplit ws = wave_split(nrun, lb, G);
+  for (int64_t item = ws.first; item < ws.end; item += ws.step) {
+    const int64_t it = item * 64 + (threadIdx.x & 63);
     const bool live = it < nrun;
-    const int32_t r = list[live ? it : first];
+    const int32_t r = list[live ? it : item * 64];
     const int zr = r % nzr;
     const int64_t xy = r / nzr;
     const int x = (int)((uint32_t)xy / (uint32_t)OL.ny), y = (int)((uint32_t)xy - (uint32_t)x * (uint32_t)OL.ny);
@@ -311,7 +331,8 @@ __device__ __forceinline__ void lds_column(const float4* __restrict__ row, int h
 }
 
 // slab: this wave's kNrLdsCap4 float4s; clu: its kNrLdsClusters records.
-// wv / NW: this wave's index and the wave count of the launch.  The
+// wv / NW: this one-wave block's index and the block count of the launch
+// (work split by wave_split).  The
 // occupancy's float4 count must be below 2^31.
 template <int MP, int RL>
 __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
@@ -329,10 +350,11 @@ __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const
   const float4* __restrict__ occ4 = reinterpret_cast<const float4*>(occ);
   int gfirst = 0;
   while (col_start[gfirst] == col_start[gfirst + 1]) ++gfirst;  // first plane with columns
-  for (int64_t first = wv * 64; first < nrun; first += NW * 64) {
-    const int64_t it = first + lane;
+  const WaveSplit ws = wave_split(nrun, wv, NW);
+  for (int64_t item = ws.first; item < ws.end; item += ws.step) {
+    const int64_t it = item * 64 + lane;
     const bool live = it < nrun;
-    const int32_t r = list[live ? it : first];
+    const int32_t r = list[live ? it : item * 64];
     const int zr = r % nzr;
     const int64_t xy = r / nzr;
     const int x = (int)((uint32_t)xy / (uint32_t)OL.ny), y = (int)((uint32_t)xy - (uint32_t)x * (uint32_t)OL.ny);

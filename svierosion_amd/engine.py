@@ -121,7 +121,9 @@ class BatchEngine:
     def tick(self) -> int:
         """One step of every running job; returns how many are still running."""
         n = C.c_int()
-        _abi.check(_abi.lib.svimx_engine_tick(self._h, C.byref(n)), "engine tick")
+        with _abi.device_lock:  # no synchronous free elsewhere while a tick may capture
+            rc = _abi.lib.svimx_engine_tick(self._h, C.byref(n))
+        _abi.check(rc, "engine tick")
         return n.value
 
     def status(self, job: int) -> JobStatus:

@@ -91,15 +91,16 @@ class BallConvolution:
             raise ValueError("ball kernel larger than the grid")
         handle = C.c_void_p()
         wc = np.ascontiguousarray(w, dtype=np.float64)
-        _abi.check(_abi.lib.svimx_nr_plan_create(wc.ctypes.data_as(C.c_void_p), self.m, float(grid.spacing),
-                                                 self.radius, *map(int, grid.shape), C.byref(handle)),
-                   "N_r plan")
-        self.plan = handle
-        self._ws = None
         if method not in ("auto", "direct", "fft"):
             raise ValueError("method must be 'auto', 'direct' or 'fft'")
-        if method != "auto":
-            _abi.check(_abi.lib.svimx_nr_plan_set_method(handle, 1 if method == "direct" else 2), "N_r method")
+        with _abi.device_lock:
+            _abi.check(_abi.lib.svimx_nr_plan_create(wc.ctypes.data_as(C.c_void_p), self.m, float(grid.spacing),
+                                                     self.radius, *map(int, grid.shape), C.byref(handle)),
+                       "N_r plan")
+            self.plan = handle
+            self._ws = None
+            if method != "auto":
+                _abi.check(_abi.lib.svimx_nr_plan_set_method(handle, 1 if method == "direct" else 2), "N_r method")
 
     @property
     def method(self) -> str:
@@ -109,7 +110,8 @@ class BallConvolution:
     def __del__(self):
         plan = getattr(self, "plan", None)
         if plan is not None and _abi is not None and _abi.lib is not None:
-            _abi.lib.svimx_nr_plan_destroy(plan)
+            with _abi.device_lock:
+                _abi.lib.svimx_nr_plan_destroy(plan)
             self.plan = None
 
     def _workspace(self):

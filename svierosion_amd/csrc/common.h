@@ -146,9 +146,12 @@ __device__ __forceinline__ double fold_op(int op, double a, double b) {
   return op == 0 ? a + b : (op == 1 ? fmax(a, b) : fmin(a, b));
 }
 
+// Launched with kFinalizeThreads threads (one wave of rows per thread keeps
+// at most two partial rows in flight per thread for 2048 blocks).
+constexpr int kFinalizeThreads = 1024;
 __device__ __forceinline__ void finalize_body(const double* __restrict__ partials, int64_t nblocks,
                                               double* __restrict__ results, const ReduceSpec& spec) {
-  __shared__ double red[4][kPartialSlots];
+  __shared__ double red[kFinalizeThreads / 64][kPartialSlots];
   double v[kPartialSlots];
 #pragma unroll
   for (int s = 0; s < kPartialSlots; ++s) {

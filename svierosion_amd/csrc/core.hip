@@ -46,14 +46,14 @@ bool carve_workspace(void* ws, int64_t ws_bytes, int64_t n_nodes, Workspace& out
 }
 
 // ---------------------------------------------------------------- reductions
-__global__ void __launch_bounds__(256) k_finalize(const double* __restrict__ partials, int64_t nblocks,
+__global__ void __launch_bounds__(kFinalizeThreads) k_finalize(const double* __restrict__ partials, int64_t nblocks,
                                                   double* __restrict__ results, ReduceSpec spec) {
   finalize_body(partials, nblocks, results, spec);
 }
 
 int launch_finalize(hipStream_t st, const double* partials, int64_t nblocks, double* results,
                     const ReduceSpec& spec) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, partials, nblocks, results, spec);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, st, partials, nblocks, results, spec);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
 }

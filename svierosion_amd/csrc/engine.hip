@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(kBandThreads) k_eng_fractions(const JobDev* __
 }
 
 // volume partials -> result slots 0..2 (svimx_fractions' finalisation)
-__global__ void __launch_bounds__(256) k_eng_finalize_volume(const JobDev* __restrict__ J, const Seg* S) {
+__global__ void __launch_bounds__(kFinalizeThreads) k_eng_finalize_volume(const JobDev* __restrict__ J, const Seg* S) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
     ReduceSpec spec;
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(kBandThreads) k_eng_velocity(const JobDev* __r
 // Statistics finalisation (svimx_velocity's) and the stable dt of
 // svierosion/evolve.py:162-170 with the max_time clamp of :199-200, in FP64
 // with the operation order of the host formula.
-__global__ void __launch_bounds__(256) k_eng_stats_dt(const JobDev* __restrict__ J, const Seg* S,
+__global__ void __launch_bounds__(kFinalizeThreads) k_eng_stats_dt(const JobDev* __restrict__ J, const Seg* S,
                                                       double* __restrict__ dts) {
 #pragma clang fp contract(off)
   ENG_FOR(S) {
@@ -570,14 +570,14 @@ static int prepare(Engine& E, int rel, int stats, int only_needed) {
   ENG_LAUNCH(k_eng_curvatures, T_CURV, kBandThreads, rel);
   ENG_LAUNCH(k_eng_velocity<true>, T_SVI, kBandThreads, rel, stats);
   ENG_LAUNCH(k_eng_velocity<false>, T_CURV, kBandThreads, rel, stats);
-  if (stats) ENG_LAUNCH(k_eng_stats_dt, T_JOBS, 256, E.d_dts);
+  if (stats) ENG_LAUNCH(k_eng_stats_dt, T_JOBS, kFinalizeThreads, E.d_dts);
   return SVIMX_OK;
 }
 
 static int volume(Engine& E, int rel) {
   ENG_LAUNCH(k_eng_clear_marks, T_MARKS, 256, 0);
   ENG_LAUNCH(k_eng_fractions, T_ALL, kBandThreads, rel, 0);
-  ENG_LAUNCH(k_eng_finalize_volume, T_JOBS, 256);
+  ENG_LAUNCH(k_eng_finalize_volume, T_JOBS, kFinalizeThreads);
   ENG_LAUNCH(k_eng_gather, T_JOBS, 256, E.d_res);
   return SVIMX_OK;
 }

@@ -165,6 +165,39 @@ def served_leg(n_req: int, rate: float, workers: int):
             "identical_histories": bool(same), "peak_device_gb": round(rv.peak_device_bytes / 1e9, 3)}
 
 
+def nr_shape_sweep(sim) -> int:
+    """Time each N_r summation shape the plan can take on the current field:
+    one JSON line per shape (for per-shape profiler counters)."""
+    import torch
+    from svierosion_amd import _abi
+    from svierosion_amd.svi import _TUNE_SHAPES, tune_shape
+    conv, ls = sim.conv, sim.ls
+    st = _abi.stream()
+    out = torch.empty(ls.grid.size, dtype=torch.float32, device=ls.device)
+    for i in range(_TUNE_SHAPES):
+        rl, lines, lds = tune_shape(i)
+        try:
+            conv.staging, conv.lines = "l1", 1
+            conv.run_length = rl
+            conv.lines = lines
+            conv.staging = "lds" if lds else "l1"
+        except ValueError:
+            continue
+        ls.fractions(conv.plan, sim.cutoff[1])
+        _abi.check(_abi.lib.svimx_nr_select(st, conv.plan))
+        args = (st, conv.plan, _abi.ptr(ls.occupancy), _abi.ptr(out), _abi.ptr(ls.ws), ls.ws.numel())
+        for _ in range(2):
+            _abi.check(_abi.lib.svimx_nr_sum(*args))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(5):
+            _abi.check(_abi.lib.svimx_nr_sum(*args))
+        b.record()
+        torch.cuda.synchronize()
+        print(json.dumps({"shape": conv.shape_name, "ms": round(a.elapsed_time(b) / 5, 4)}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +215,9 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=2, help="set-up / meshing threads of the batched forms")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of this many MiB (profiler byte calibration)")
+    ap.add_argument("--nr-shape-sweep", action="store_true",
+                    help="diagnostic: time every N_r summation shape on the workload's field "
+                         "(5 launches each, in the autotuner's order) and exit")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this workload is a single erosion job on one MI355X "
@@ -202,6 +238,8 @@ def main(argv=None):
         print(f"[bench] calibration copy of {nbytes} bytes x3 done", file=sys.stderr, flush=True)
     mesh, grid, ls, sim, law, info = build_workload(args)
     print(f"[bench] workload ready: {info}", file=sys.stderr, flush=True)
+    if args.nr_shape_sweep:
+        return nr_shape_sweep(sim)
     sim.run(target_fraction=None, max_steps=args.warmup)
     torch.cuda.synchronize()
     if not args.no_events:

@@ -11,9 +11,10 @@ namespace svimx {
 
 // ---------------------------------------------------------------- run selection
 // dil(x, y, zr) = OR_{b=0..3} marks(x, y-b, zr)
+// (run and unit indices fit 31 bits: 32-bit divisions throughout)
 __device__ __forceinline__ void dilate_y_run(const uint8_t* __restrict__ marks, uint8_t* __restrict__ dil, int ny,
                                              int nzr, int64_t r) {
-  const int y = (int)((r / nzr) % ny);
+  const int y = (int)(((uint32_t)r / (uint32_t)nzr) % (uint32_t)ny);
   uint8_t v = 0;
   for (int b = 0; b <= 3 && b <= y; ++b) v |= marks[r - (int64_t)b * nzr];
   dil[r] = v;
@@ -24,7 +25,7 @@ struct RunNeed {
   const uint8_t* dil;
   int64_t plane;  // ny * nzr
   __device__ bool operator()(int64_t r) const {
-    const int64_t x = r / plane;
+    const int64_t x = (uint32_t)r / (uint32_t)plane;
     uint8_t v = 0;
     for (int a = 0; a <= 3 && a <= x; ++a) v |= dil[r - a * plane];
     return v != 0;
@@ -37,10 +38,11 @@ struct UnitNeed {
   const uint8_t* dil;
   int ny, nzr, nyu, Y;
   __device__ bool operator()(int64_t u) const {
-    const int zr = (int)(u % nzr);
-    const int64_t xyu = u / nzr;
-    const int yu = (int)(xyu % nyu);
-    const int64_t x = xyu / nyu;
+    const uint32_t xyu = (uint32_t)u / (uint32_t)nzr;
+    const int zr = (int)((uint32_t)u - xyu * (uint32_t)nzr);
+    const uint32_t xu = xyu / (uint32_t)nyu;
+    const int yu = (int)(xyu - xu * (uint32_t)nyu);
+    const int64_t x = xu;
     const int64_t plane = (int64_t)ny * nzr;
     uint8_t v = 0;
     for (int yy = 0; yy < Y; ++yy) {

@@ -63,6 +63,12 @@ __device__ __forceinline__ void scan_counts_body(int64_t* counts, int64_t n) {
 // Exclusive scan of `n` counts in place into offs[0..n), total into offs[n].
 __global__ void __launch_bounds__(1024) k_scan_counts(int64_t* counts, int64_t n);
 
+// Selects the positions whose byte flag is nonzero.
+struct FlagPred {
+  const uint8_t* f;
+  __device__ bool operator()(int64_t e) const { return f[e] != 0; }
+};
+
 // Output value of a selected position: the position itself, or map[position].
 struct IdentityMap {
   __device__ int32_t operator()(int64_t e) const { return (int32_t)e; }

@@ -153,17 +153,18 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_eng_finalize_volume(const 
   }
 }
 
-__global__ void __launch_bounds__(256) k_eng_dilate(const JobDev* __restrict__ J, const Seg* S) {
+__global__ void __launch_bounds__(256) k_eng_unit_need(const JobDev* __restrict__ J, const Seg* S) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    for (int64_t r = lb * 256 + threadIdx.x; r < j.nruns; r += G * 256) dilate_y_run(j.marks, j.dil, j.d.ny, j.nzr, r);
+    for (int64_t u = lb * 256 + threadIdx.x; u < j.nunits; u += G * 256)
+      unit_need(j.marks, j.dil, j.d.ny, j.nzr, j.nyu, j.ylines, u);
   }
 }
 
 __global__ void __launch_bounds__(kCompactThreads) k_eng_need_count(const JobDev* __restrict__ J, const Seg* S) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    compact_count_body(UnitNeed{j.dil, j.d.ny, j.nzr, j.nyu, j.ylines}, j.nunits, j.ccounts, lb);
+    compact_count_body(FlagPred{j.dil}, j.nunits, j.ccounts, lb);
   }
 }
 
@@ -177,7 +178,7 @@ __global__ void __launch_bounds__(1024) k_eng_need_scan(const JobDev* __restrict
 __global__ void __launch_bounds__(kCompactThreads) k_eng_need_scatter(const JobDev* __restrict__ J, const Seg* S) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    compact_scatter_body(UnitNeed{j.dil, j.d.ny, j.nzr, j.nyu, j.ylines}, j.nunits, j.ccounts, j.list, j.nunits,
+    compact_scatter_body(FlagPred{j.dil}, j.nunits, j.ccounts, j.list, j.nunits,
                          IdentityMap(), lb);
   }
 }
@@ -488,7 +489,7 @@ static int upload(Engine& E) {
     else seg_add(T.seg[T_CURV], slot, reduce_grid(e.nb));
     if (e.plan && !e.plan->fft) seg_add(T.seg[T_MARKS], slot, (e.plan->nmarks + 255) / 256);
     if (e.run_plan()) {
-      seg_add(T.seg[T_RUNS], slot, (e.plan->nruns + 255) / 256);
+      seg_add(T.seg[T_RUNS], slot, (e.plan->nunits + 255) / 256);
       seg_add(T.seg[T_CHUNKS], slot, e.plan->nchunks);
       seg_add(T.seg[T_RUNJOBS], slot, 1);
       const int rl = e.plan->run_mode;
@@ -533,7 +534,7 @@ static unsigned grid_of(const Engine& E, int t) {
 static int prepare(Engine& E, int rel, int stats, int only_needed) {
   ENG_LAUNCH(k_eng_clear_marks, T_MARKS, 256, only_needed);
   ENG_LAUNCH(k_eng_fractions, T_SVI, kBandThreads, rel, only_needed);
-  ENG_LAUNCH(k_eng_dilate, T_RUNS, 256);
+  ENG_LAUNCH(k_eng_unit_need, T_RUNS, 256);
   ENG_LAUNCH(k_eng_need_count, T_CHUNKS, kCompactThreads);
   ENG_LAUNCH(k_eng_need_scan, T_RUNJOBS, 1024);
   ENG_LAUNCH(k_eng_need_scatter, T_CHUNKS, kCompactThreads);

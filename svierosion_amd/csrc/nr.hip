@@ -57,10 +57,10 @@ __global__ void k_nr_mark_nodes(uint8_t* marks, int run_mode, int nty, int ntz, 
     marks[((int64_t)(i / kTX) * nty + j / kTY) * ntz + k / kTZ] = 1;
 }
 
-__global__ void k_runs_dilate_y(const uint8_t* __restrict__ marks, uint8_t* __restrict__ dil, int ny, int nzr,
-                                int64_t nruns) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < nruns) dilate_y_run(marks, dil, ny, nzr, r);
+__global__ void k_nr_unit_need(const uint8_t* __restrict__ marks, uint8_t* __restrict__ need, int ny, int nzr,
+                               int nyu, int Y, int64_t nunits) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < nunits) unit_need(marks, need, ny, nzr, nyu, Y, u);
 }
 
 // ---------------------------------------------------------------- run-mode summation
@@ -379,16 +379,16 @@ int svimx_nr_select(void* stream, void* plan) {
   if (p->fft) return SVIMX_OK;  // the FFT evaluates every node
   hipStream_t st = as_stream(stream);
   if (p->run_mode) {
-    hipLaunchKernelGGL(k_runs_dilate_y, dim3(blocks_for(p->nruns, 256)), dim3(256), 0, st, p->marks, p->dil,
-                       p->d.ny, p->nzr, p->nruns);
+    hipLaunchKernelGGL(k_nr_unit_need, dim3(blocks_for(p->nunits, 256)), dim3(256), 0, st, p->marks, p->dil,
+                       p->d.ny, p->nzr, p->nyu, p->ylines, p->nunits);
     SVIMX_CHECK_LAUNCH();
-    UnitNeed pred{p->dil, p->d.ny, p->nzr, p->nyu, p->ylines};
-    hipLaunchKernelGGL(k_compact_count<UnitNeed>, dim3((unsigned)p->nchunks), dim3(kCompactThreads), 0, st, pred,
+    FlagPred pred{p->dil};
+    hipLaunchKernelGGL(k_compact_count<FlagPred>, dim3((unsigned)p->nchunks), dim3(kCompactThreads), 0, st, pred,
                        p->nunits, p->ccounts);
     SVIMX_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, p->ccounts, p->nchunks);
     SVIMX_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_compact_scatter<UnitNeed, IdentityMap>), dim3((unsigned)p->nchunks),
+    hipLaunchKernelGGL((k_compact_scatter<FlagPred, IdentityMap>), dim3((unsigned)p->nchunks),
                        dim3(kCompactThreads), 0, st, pred, p->nunits, p->ccounts, p->list, p->nunits, IdentityMap());
     SVIMX_CHECK_LAUNCH();
     return SVIMX_OK;

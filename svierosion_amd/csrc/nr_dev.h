@@ -10,50 +10,24 @@
 namespace svimx {
 
 // ---------------------------------------------------------------- run selection
-// dil(x, y, zr) = OR_{b=0..3} marks(x, y-b, zr)
-// (run and unit indices fit 31 bits: 32-bit divisions throughout)
-__device__ __forceinline__ void dilate_y_run(const uint8_t* __restrict__ marks, uint8_t* __restrict__ dil, int ny,
-                                             int nzr, int64_t r) {
-  const int y = (int)(((uint32_t)r / (uint32_t)nzr) % (uint32_t)ny);
+// Need flag of unit u = (x, yu, zr): some run (x - a, y', zr) with a in [0, 3]
+// and y' in [yu*Y - 3, yu*Y + Y - 1] is marked, i.e. a marked tricubic
+// stencil base (+0..3 in x and y) reaches one of the unit's Y lines.  Unit
+// and run indices fit 31 bits: 32-bit divisions.
+__device__ __forceinline__ void unit_need(const uint8_t* __restrict__ marks, uint8_t* __restrict__ need, int ny,
+                                          int nzr, int nyu, int Y, int64_t u) {
+  const uint32_t xyu = (uint32_t)u / (uint32_t)nzr;
+  const int zr = (int)((uint32_t)u - xyu * (uint32_t)nzr);
+  const int x = (int)(xyu / (uint32_t)nyu);
+  const int yu = (int)(xyu - (uint32_t)x * (uint32_t)nyu);
+  const int ylo = max(yu * Y - 3, 0), yhi = min(yu * Y + Y - 1, ny - 1);
   uint8_t v = 0;
-  for (int b = 0; b <= 3 && b <= y; ++b) v |= marks[r - (int64_t)b * nzr];
-  dil[r] = v;
+  for (int a = 0; a <= 3 && a <= x; ++a) {
+    const uint8_t* row = marks + (int64_t)(x - a) * ny * nzr + zr;
+    for (int y = ylo; y <= yhi; ++y) v |= row[(int64_t)y * nzr];
+  }
+  need[u] = v ? 1 : 0;
 }
-
-// need(x, y, zr) = OR_{a=0..3} dil(x-a, y, zr)
-struct RunNeed {
-  const uint8_t* dil;
-  int64_t plane;  // ny * nzr
-  __device__ bool operator()(int64_t r) const {
-    const int64_t x = (uint32_t)r / (uint32_t)plane;
-    uint8_t v = 0;
-    for (int a = 0; a <= 3 && a <= x; ++a) v |= dil[r - a * plane];
-    return v != 0;
-  }
-};
-
-// need of unit u = (x, yu, zr): any of its lines y = yu * Y + 0..Y-1 (< ny)
-// has a needed run; with Y = 1 the unit is the run and this is RunNeed.
-struct UnitNeed {
-  const uint8_t* dil;
-  int ny, nzr, nyu, Y;
-  __device__ bool operator()(int64_t u) const {
-    const uint32_t xyu = (uint32_t)u / (uint32_t)nzr;
-    const int zr = (int)((uint32_t)u - xyu * (uint32_t)nzr);
-    const uint32_t xu = xyu / (uint32_t)nyu;
-    const int yu = (int)(xyu - xu * (uint32_t)nyu);
-    const int64_t x = xu;
-    const int64_t plane = (int64_t)ny * nzr;
-    uint8_t v = 0;
-    for (int yy = 0; yy < Y; ++yy) {
-      const int y = yu * Y + yy;
-      if (y >= ny) break;
-      const int64_t r = (x * ny + y) * nzr + zr;
-      for (int a = 0; a <= 3 && a <= x; ++a) v |= dil[r - a * plane];
-    }
-    return v != 0;
-  }
-};
 
 // Wave-level split of an n-entry work list over a launch of G blocks: the
 // work is cut into 64-entry wave items, each XCD (block lb runs on XCD lb mod

@@ -63,7 +63,7 @@ struct NrPlan {
   int* col_start;     // device, 2m+2 entries: columns of dx = g - m are [col_start[g], col_start[g+1])
   float* weights;     // device; run mode: per column the taps dz = -k..k
   uint8_t* marks;     // device, nmarks
-  uint8_t* dil;       // device, nruns (run mode: marks dilated in y)
+  uint8_t* dil;       // device, nruns bytes (run mode: need flag of each unit)
   int32_t* list;      // device, nmarks
   int32_t* count;     // device, 1 (tile mode)
   int64_t* ccounts;   // device, compaction chunk counts + total (run mode)

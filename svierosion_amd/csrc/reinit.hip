@@ -10,8 +10,8 @@
 // Geometry (subcell crossings and local dtau, svierosion/kernels.py:335-359)
 // is computed once from phi0.  Only nodes next to the interface have a
 // crossing; they keep dtau and the six subcell distances in their slot of
-// `geo` (slot = work index), every other node only its 4-byte info word (sign,
-// no crossing, dtau = cfl*h), so a sweep streams ~12 bytes for most nodes.
+// `geo` (slot = work index), every other node only its 1-byte info word (sign,
+// no crossing, dtau = cfl*h), so a sweep streams ~9 bytes for most nodes.
 //
 // Each TVD-RK2 iteration is two Jacobi sweeps ping-ponging between phi and a
 // scratch field:  scratch = phi + dtau R(phi);  phi = (phi + scratch +
@@ -213,7 +213,7 @@ __global__ void k_copy_list(const float* __restrict__ src, float* __restrict__ d
 // info word: bits 0-5 crossing flags, bits 6-7 sign code.  Nodes with a
 // crossing store dtau and the six distances at geo[8 i].
 __global__ void k_reinit_geo(const float* __restrict__ phi, const int32_t* __restrict__ idx, int64_t n, Dims d,
-                             float h, float cfl, uint32_t* __restrict__ info, float* __restrict__ geo) {
+                             float h, float cfl, uint8_t* __restrict__ info, float* __restrict__ geo) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   ReinitGeo g = reinit_geometry(phi, idx[i], d, h, cfl);
@@ -222,32 +222,45 @@ __global__ void k_reinit_geo(const float* __restrict__ phi, const int32_t* __res
     o[0] = g.dtau;
     for (int q = 0; q < 6; ++q) o[1 + q] = g.x[q];
   }
-  info[i] = g.flags & 0xFFu;
+  info[i] = (uint8_t)(g.flags & 0xFFu);
 }
 
 // mode 0: out = in + dtau R(in);  mode 1: out = (out + in + dtau R(in)) / 2.
+// Each thread takes kSweepNodes work nodes kBandThreads apart (coalesced), so
+// a wave keeps that many nodes' stencil loads in flight.
+constexpr int kSweepNodes = 2;
 __global__ void __launch_bounds__(256) k_reinit_sweep(int mode, const float* __restrict__ in, float* out,
                                                       const int32_t* __restrict__ idx, int64_t n, Dims d, float h,
-                                                      float dtau0, const uint32_t* __restrict__ info,
+                                                      float dtau0, const uint8_t* __restrict__ info,
                                                       const float* __restrict__ geo) {
   // XCD-contiguous slices of the (sorted) work list keep stencil neighbours in one L2
   const int lb = (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
-  const int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t node = idx[i];
-  const uint32_t word = info[i];
-  float dtau = dtau0;
-  float x[6] = {0, 0, 0, 0, 0, 0};
-  if (word & 63u) {
-    const float* g = geo + 8 * i;
-    dtau = g[0];
-    for (int q = 0; q < 6; ++q) x[q] = g[1 + q];
+  const int64_t base = (int64_t)lb * blockDim.x * kSweepNodes + threadIdx.x;
+  int64_t node[kSweepNodes];
+  uint32_t word[kSweepNodes];
+#pragma unroll
+  for (int k = 0; k < kSweepNodes; ++k) {
+    const int64_t i = base + (int64_t)k * blockDim.x;
+    node[k] = i < n ? idx[i] : -1;
+    word[k] = i < n ? info[i] : 0u;
   }
-  const float r = reinit_rhs(in, node, d, h, word, x);
-  if (mode == 0) {
-    out[node] = in[node] + dtau * r;
-  } else {
-    out[node] = 0.5f * (out[node] + in[node] + dtau * r);
+#pragma unroll
+  for (int k = 0; k < kSweepNodes; ++k) {
+    if (node[k] < 0) continue;
+    const int64_t i = base + (int64_t)k * blockDim.x;
+    float dtau = dtau0;
+    float x[6] = {0, 0, 0, 0, 0, 0};
+    if (word[k] & 63u) {
+      const float* g = geo + 8 * i;
+      dtau = g[0];
+      for (int q = 0; q < 6; ++q) x[q] = g[1 + q];
+    }
+    const float r = reinit_rhs(in, node[k], d, h, word[k], x);
+    if (mode == 0) {
+      out[node[k]] = in[node[k]] + dtau * r;
+    } else {
+      out[node[k]] = 0.5f * (out[node[k]] + in[node[k]] + dtau * r);
+    }
   }
 }
 
@@ -368,20 +381,20 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   rc = launch_finalize(st, w.partials, nblk, w.results, spec);
   if (rc) return rc;
   if (nw > 0) {
-    const int gwb = (int)((blocks_for(nw, 256) + kXcds - 1) / kXcds * kXcds);
+    const int gwb = (int)((blocks_for(nw, 256 * kSweepNodes) + kXcds - 1) / kXcds * kXcds);
     const dim3 gw(blocks_for(nw, 256));
     hipLaunchKernelGGL(k_copy_list, gw, tb, 0, st, phi, scratch, work_idx, nw);
     SVIMX_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_reinit_geo, gw, tb, 0, st, phi, work_idx, nw, d, (float)h, (float)cfl,
-                       (uint32_t*)work_info, geo);
+                       reinterpret_cast<uint8_t*>(work_info), geo);
     SVIMX_CHECK_LAUNCH();
     const float dtau0 = (float)cfl * fmaxf((float)h, 1e-3f * (float)h);
     for (int it = 0; it < iterations; ++it) {
       hipLaunchKernelGGL(k_reinit_sweep, dim3(gwb), tb, 0, st, 0, phi, scratch, work_idx, nw, d, (float)h, dtau0,
-                         (const uint32_t*)work_info, geo);
+                         reinterpret_cast<const uint8_t*>(work_info), geo);
       SVIMX_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_reinit_sweep, dim3(gwb), tb, 0, st, 1, scratch, phi, work_idx, nw, d, (float)h, dtau0,
-                         (const uint32_t*)work_info, geo);
+                         reinterpret_cast<const uint8_t*>(work_info), geo);
       SVIMX_CHECK_LAUNCH();
     }
   }

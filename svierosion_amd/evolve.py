@@ -159,7 +159,8 @@ class ErosionSimulation:
                 self._frac_valid = buf
             phi = ls.flat
             if self.conv is not None:
-                _abi.check(_abi.lib.svimx_nr_select(st, self.conv.plan), "N_r select")
+                with self._span("nr_select"):
+                    _abi.check(_abi.lib.svimx_nr_select(st, self.conv.plan), "N_r select")
                 with self._span("nr_sum"):
                     _abi.check(_abi.lib.svimx_nr_sum(st, self.conv.plan, _abi.ptr(ls.occupancy), _abi.ptr(self.field),
                                                      _abi.ptr(ls.ws), ls.ws.numel()), "N_r")

@@ -146,44 +146,51 @@ __device__ __forceinline__ double fold_op(int op, double a, double b) {
   return op == 0 ? a + b : (op == 1 ? fmax(a, b) : fmin(a, b));
 }
 
-// Launched with kFinalizeThreads threads (one wave of rows per thread keeps
-// at most two partial rows in flight per thread for 2048 blocks).
+// The fold order is that of kFinalizeThreads "virtual" threads whatever the
+// block size (a divisor of it): virtual thread t folds rows t, t + 1024, ...,
+// each virtual wave of 64 reduces with an xor tree, and the 16 virtual waves
+// combine in order; a block of 256 threads runs the virtual waves in 4 passes.
+// So a finalisation run by a block of any such size gives identical results.
+// (Folding the rows in the producer's last block instead of a separate launch
+// was measured slower: every block's agent-scope release writes back its
+// XCD's L2.)
 constexpr int kFinalizeThreads = 1024;
 __device__ __forceinline__ void finalize_body(const double* __restrict__ partials, int64_t nblocks,
                                               double* __restrict__ results, const ReduceSpec& spec) {
-  __shared__ double red[kFinalizeThreads / 64][kPartialSlots];
-  double v[kPartialSlots];
+  constexpr int kVWaves = kFinalizeThreads / 64;
+  __shared__ double red[kVWaves][kPartialSlots];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwr = (int)(blockDim.x >> 6);
+  for (int vw = wave; vw < kVWaves; vw += nwr) {
+    double v[kPartialSlots];
 #pragma unroll
-  for (int s = 0; s < kPartialSlots; ++s) {
-    const int op = spec.op[s];
-    v[s] = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
-  }
-  for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
-    const double2* row = reinterpret_cast<const double2*>(partials + b * kPartialSlots);
-#pragma unroll
-    for (int q = 0; q < kPartialSlots / 2; ++q) {
-      const double2 u = row[q];
-      if (spec.op[2 * q] >= 0) v[2 * q] = fold_op(spec.op[2 * q], v[2 * q], u.x);
-      if (spec.op[2 * q + 1] >= 0) v[2 * q + 1] = fold_op(spec.op[2 * q + 1], v[2 * q + 1], u.y);
+    for (int s = 0; s < kPartialSlots; ++s) {
+      const int op = spec.op[s];
+      v[s] = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
     }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t b = vw * 64 + lane; b < nblocks; b += kFinalizeThreads) {
+      const double2* row = reinterpret_cast<const double2*>(partials + b * kPartialSlots);
 #pragma unroll
-  for (int s = 0; s < kPartialSlots; ++s) {
-    const int op = spec.op[s];
-    if (op < 0) continue;
-    for (int o = 32; o > 0; o >>= 1) v[s] = fold_op(op, v[s], __shfl_xor(v[s], o, 64));
+      for (int q = 0; q < kPartialSlots / 2; ++q) {
+        const double2 u = row[q];
+        if (spec.op[2 * q] >= 0) v[2 * q] = fold_op(spec.op[2 * q], v[2 * q], u.x);
+        if (spec.op[2 * q + 1] >= 0) v[2 * q + 1] = fold_op(spec.op[2 * q + 1], v[2 * q + 1], u.y);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kPartialSlots; ++s) {
+      const int op = spec.op[s];
+      if (op < 0) continue;
+      for (int o = 32; o > 0; o >>= 1) v[s] = fold_op(op, v[s], __shfl_xor(v[s], o, 64));
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int s = 0; s < kPartialSlots; ++s) red[vw][s] = v[s];
   }
   __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int s = 0; s < kPartialSlots; ++s) red[wave][s] = v[s];
-  __syncthreads();
-  const int nw = (blockDim.x + 63) >> 6;
   if (threadIdx.x < kPartialSlots && spec.op[threadIdx.x] >= 0) {
     const int s = threadIdx.x, op = spec.op[s];
     double r = red[0][s];
-    for (int w = 1; w < nw; ++w) r = fold_op(op, r, red[w][s]);
+    for (int w = 1; w < kVWaves; ++w) r = fold_op(op, r, red[w][s]);
     results[spec.dst[s]] = r;
   }
   __syncthreads();

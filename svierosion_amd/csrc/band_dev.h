@@ -2,7 +2,7 @@
 // kernels (stage.hip) and the job-table kernels of the batch engine
 // (engine.hip).  A body processes one "virtual block" lb of a grid of G
 // blocks of kBandThreads threads: grid-stride over the band list with stride
-// G * kBandThreads, block partials at partials[lb * kPartialSlots].  The
+// G * kBandThreads, block partials at partials[slot * G + lb].  The
 // single-job kernels call it with (blockIdx.x, gridDim.x); the batch engine
 // with the job's own (lb, G), so a job's arithmetic and reduction order are
 // the same alone and in a batch.  (Contiguous XCD-local chunks of the list
@@ -163,10 +163,10 @@ __device__ __forceinline__ void fractions_body(const float* __restrict__ phi, co
   if (threadIdx.x == 0) {
     int f = 0;
     for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) f |= flags[w];
-    double* o = partials + lb * kPartialSlots;
+    double* o = partials + lb;
     o[0] = s;
-    o[1] = (f & 1) ? 1.0 : 0.0;
-    o[2] = (f & 2) ? 1.0 : 0.0;
+    o[G] = (f & 1) ? 1.0 : 0.0;
+    o[2 * G] = (f & 2) ? 1.0 : 0.0;
   }
   __syncthreads();  // flags[] is rewritten by the block's next segment
 }
@@ -293,22 +293,22 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
       sm[0] += wd[w][0];
       sm[1] += wd[w][1];
     }
-    double* o = partials + lb * kPartialSlots;
+    double* o = partials + lb;
     o[0] = f[0];
-    o[1] = f[1];
-    o[2] = f[2];
-    o[3] = f[3];
-    o[4] = (double)c[0];
-    o[5] = sm[0];
-    o[6] = (double)c[1];
-    o[7] = f[4];
-    o[8] = f[5];
-    o[9] = f[6];
-    o[10] = f[7];
-    o[11] = (double)c[2];
-    o[12] = sm[1];
-    o[13] = (double)c[3];
-    o[14] = f[8];
+    o[G] = f[1];
+    o[2 * G] = f[2];
+    o[3 * G] = f[3];
+    o[4 * G] = (double)c[0];
+    o[5 * G] = sm[0];
+    o[6 * G] = (double)c[1];
+    o[7 * G] = f[4];
+    o[8 * G] = f[5];
+    o[9 * G] = f[6];
+    o[10 * G] = f[7];
+    o[11 * G] = (double)c[2];
+    o[12 * G] = sm[1];
+    o[13 * G] = (double)c[3];
+    o[14 * G] = f[8];
   }
   __syncthreads();  // wf/wc/wd are reused by the block's next segment
 }

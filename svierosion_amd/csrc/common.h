@@ -54,7 +54,7 @@ inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) /
 // ---------------------------------------------------------------- workspace
 // One per job, caller-allocated (svimx_workspace_bytes).  Fixed carve-up:
 //   [0, 4 KiB)            results (doubles, SVIMX_R_* slots)
-//   partials              kPartialSlots doubles per block of kBandThreads
+//   partials              kPartialSlots doubles per block of kBandThreads (slot-major)
 //   compaction counts     one int64 per compaction chunk (+ scan)
 struct Workspace {
   double* results;
@@ -127,8 +127,8 @@ __device__ __forceinline__ double block_reduce(double v, double* lds4) {
   return r;
 }
 
-// Launch-wide finalisation: reduce nblocks x kPartialSlots partials in a fixed
-// order (deterministic) and store into results[dst + slot] with per-slot op.
+// Launch-wide finalisation: reduce kPartialSlots x nblocks partials in a fixed
+// order (deterministic) and store slot s into results[dst[s]] with op[s].
 // ops[slot]: 0 sum, 1 max, 2 min, -1 skip.
 struct ReduceSpec {
   int8_t op[kPartialSlots];
@@ -137,63 +137,55 @@ struct ReduceSpec {
 int launch_finalize(hipStream_t st, const double* partials, int64_t nblocks, double* results,
                     const ReduceSpec& spec);
 
-// Body of the finalisation (one block of up to 256 threads): every thread
-// folds whole 16-slot partial rows b = t, t+256, ... (contiguous 128-byte
-// reads), each wave reduces its slots with the xor butterfly, and thread s
-// combines the waves' values of slot s in wave order -- the same combination
-// order per slot as a slot-by-slot block reduction, in one pass.
 __device__ __forceinline__ double fold_op(int op, double a, double b) {
   return op == 0 ? a + b : (op == 1 ? fmax(a, b) : fmin(a, b));
 }
 
-// The fold order is that of kFinalizeThreads "virtual" threads whatever the
-// block size (a divisor of it): virtual thread t folds rows t, t + 1024, ...,
-// each virtual wave of 64 reduces with an xor tree, and the 16 virtual waves
-// combine in order; a block of 256 threads runs the virtual waves in 4 passes.
-// So a finalisation run by a block of any such size gives identical results.
-// (Folding the rows in the producer's last block instead of a separate launch
-// was measured slower: every block's agent-scope release writes back its
-// XCD's L2.)
+// Block partials are slot-major: the reducing kernel of a G-block grid stores
+// slot s of block b at partials[s * G + b], and the finalisation reads them
+// back with the same G (= its nblocks).  Slot s is folded by one wave: lane l
+// loads rows l, l + 64, ... (each load instruction reads 512 contiguous
+// bytes, 16 loads in flight before their sequential fold), then the wave
+// reduces with the xor tree 32, 16, ..., 1.  Unused slots cost nothing.  The
+// order depends on neither the block size nor the number of active slots, so
+// a job's results are the same from the single-job kernels and from the batch
+// engine.  (Row-major 128-byte rows read by one lane each, the earlier
+// layout, made every load instruction touch 64 cache lines: 13-19 us per
+// finalisation against the few us of its data.  Folding the rows in the
+// producer's last block instead of a separate launch was measured slower as
+// well: every block's agent-scope release writes back its XCD's L2.)
 constexpr int kFinalizeThreads = 1024;
+constexpr int kFinalizeRows = kMaxReduceBlocks / 64;  // rows per lane
+__device__ __forceinline__ int spec_byte(const int8_t (&a)[kPartialSlots], int s) {
+  uint64_t w[2];
+  __builtin_memcpy(w, a, sizeof(w));
+  return (int)(int8_t)((s < 8 ? w[0] : w[1]) >> (8 * (s & 7)));
+}
 __device__ __forceinline__ void finalize_body(const double* __restrict__ partials, int64_t nblocks,
                                               double* __restrict__ results, const ReduceSpec& spec) {
-  constexpr int kVWaves = kFinalizeThreads / 64;
-  __shared__ double red[kVWaves][kPartialSlots];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwr = (int)(blockDim.x >> 6);
-  for (int vw = wave; vw < kVWaves; vw += nwr) {
-    double v[kPartialSlots];
+  for (int s = wave; s < kPartialSlots; s += nwr) {
+    const int op = spec_byte(spec.op, s);
+    if (op < 0) continue;  // wave-uniform
+    const double id = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
+    const double* col = partials + (int64_t)s * nblocks;
+    double v = id;
+    // rows in batches of 16 per lane (all loads of a batch in flight); a
+    // batch past nblocks would fold only identities and is skipped
+    for (int i0 = 0; i0 < kFinalizeRows && 64 * i0 < nblocks; i0 += 16) {
+      double x[16];
 #pragma unroll
-    for (int s = 0; s < kPartialSlots; ++s) {
-      const int op = spec.op[s];
-      v[s] = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
-    }
-    for (int64_t b = vw * 64 + lane; b < nblocks; b += kFinalizeThreads) {
-      const double2* row = reinterpret_cast<const double2*>(partials + b * kPartialSlots);
-#pragma unroll
-      for (int q = 0; q < kPartialSlots / 2; ++q) {
-        const double2 u = row[q];
-        if (spec.op[2 * q] >= 0) v[2 * q] = fold_op(spec.op[2 * q], v[2 * q], u.x);
-        if (spec.op[2 * q + 1] >= 0) v[2 * q + 1] = fold_op(spec.op[2 * q + 1], v[2 * q + 1], u.y);
+      for (int i = 0; i < 16; ++i) {
+        const int64_t b = lane + 64 * (i0 + i);
+        x[i] = b < nblocks ? col[b] : id;
       }
-    }
 #pragma unroll
-    for (int s = 0; s < kPartialSlots; ++s) {
-      const int op = spec.op[s];
-      if (op < 0) continue;
-      for (int o = 32; o > 0; o >>= 1) v[s] = fold_op(op, v[s], __shfl_xor(v[s], o, 64));
+      for (int i = 0; i < 16; ++i) v = fold_op(op, v, x[i]);
     }
-    if (lane == 0)
-#pragma unroll
-      for (int s = 0; s < kPartialSlots; ++s) red[vw][s] = v[s];
+    for (int o = 32; o > 0; o >>= 1) v = fold_op(op, v, __shfl_xor(v, o, 64));
+    if (lane == 0) results[spec_byte(spec.dst, s)] = v;
   }
-  __syncthreads();
-  if (threadIdx.x < kPartialSlots && spec.op[threadIdx.x] >= 0) {
-    const int s = threadIdx.x, op = spec.op[s];
-    double r = red[0][s];
-    for (int w = 1; w < kVWaves; ++w) r = fold_op(op, r, red[w][s]);
-    results[spec.dst[s]] = r;
-  }
-  __syncthreads();
+  __syncthreads();  // callers read results[] from other threads of the block
 }
 
 }  // namespace svimx

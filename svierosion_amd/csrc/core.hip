@@ -53,6 +53,8 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_finalize(const double* __r
 
 int launch_finalize(hipStream_t st, const double* partials, int64_t nblocks, double* results,
                     const ReduceSpec& spec) {
+  SVIMX_REQUIRE(nblocks >= 1 && nblocks <= kMaxReduceBlocks, SVIMX_EBADARG, "finalisation of %lld partial blocks",
+                (long long)nblocks);
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalizeThreads), 0, st, partials, nblocks, results, spec);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
@@ -94,12 +96,12 @@ __global__ void __launch_bounds__(kBandThreads) k_band_classify(float* __restric
     }
   }
   double r = block_reduce<0>(inside_out, red);
-  if (threadIdx.x == 0) partials[blockIdx.x * kPartialSlots + 0] = r;
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
   for (int a = 0; a < 3; ++a) {
     double lo = block_reduce<2>(mn[a], red);
-    if (threadIdx.x == 0) partials[blockIdx.x * kPartialSlots + 1 + a] = lo;
+    if (threadIdx.x == 0) partials[(1 + a) * gridDim.x + blockIdx.x] = lo;
     double hi = block_reduce<1>(mx[a], red);
-    if (threadIdx.x == 0) partials[blockIdx.x * kPartialSlots + 4 + a] = hi;
+    if (threadIdx.x == 0) partials[(4 + a) * gridDim.x + blockIdx.x] = hi;
   }
 }
 

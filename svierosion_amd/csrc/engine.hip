@@ -126,7 +126,11 @@ __global__ void __launch_bounds__(256) k_eng_clear_marks(const JobDev* __restric
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
     if (only_needed && !j.need_frac) continue;
-    for (int64_t i = lb * 256 + threadIdx.x; i < j.nmarks; i += G * 256) j.marks[i] = 0;
+    // 16 bytes per thread (the plan's marks are hipMalloc'd), byte tail
+    const int64_t n16 = j.nmarks >> 4;
+    for (int64_t i = lb * 256 + threadIdx.x; i < n16; i += G * 256)
+      reinterpret_cast<uint4*>(j.marks)[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int64_t i = (n16 << 4) + lb * 256 + threadIdx.x; i < j.nmarks; i += G * 256) j.marks[i] = 0;
   }
 }
 
@@ -487,7 +491,7 @@ static int upload(Engine& E) {
     seg_add(T.seg[T_JOBS], slot, 1);
     if (e.svi()) seg_add(T.seg[T_SVI], slot, reduce_grid(e.nb));
     else seg_add(T.seg[T_CURV], slot, reduce_grid(e.nb));
-    if (e.plan && !e.plan->fft) seg_add(T.seg[T_MARKS], slot, (e.plan->nmarks + 255) / 256);
+    if (e.plan && !e.plan->fft) seg_add(T.seg[T_MARKS], slot, (e.plan->nmarks + 4095) / 4096);
     if (e.run_plan()) {
       seg_add(T.seg[T_RUNS], slot, (e.plan->nunits + 255) / 256);
       seg_add(T.seg[T_CHUNKS], slot, e.plan->nchunks);

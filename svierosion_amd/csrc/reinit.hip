@@ -192,7 +192,7 @@ static int compact_bits(hipStream_t st, const uint32_t* m, int64_t nw, int wpl, 
   return SVIMX_OK;
 }
 
-// partials[b][slot] = #{i in block b's share : phi[list[i]] < 0}
+// partials[slot * G + b] = #{i in block b's share : phi[list[i]] < 0}
 __global__ void __launch_bounds__(kBandThreads)
     k_count_neg(const float* __restrict__ phi, const int32_t* __restrict__ list, int64_t n,
                 double* __restrict__ partials, int slot) {
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(kBandThreads)
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     c += phi[list[i]] < 0.0f ? 1.0 : 0.0;
   c = block_reduce<0>(c, red);
-  if (threadIdx.x == 0) partials[blockIdx.x * kPartialSlots + slot] = c;
+  if (threadIdx.x == 0) partials[slot * gridDim.x + blockIdx.x] = c;
 }
 
 __global__ void k_copy_list(const float* __restrict__ src, float* __restrict__ dst, const int32_t* __restrict__ idx,
@@ -293,12 +293,12 @@ __global__ void __launch_bounds__(kBandThreads)
     }
   }
   double r = block_reduce<0>(outneg, red);
-  if (threadIdx.x == 0) partials[blockIdx.x * kPartialSlots + 2] = r;
+  if (threadIdx.x == 0) partials[2 * gridDim.x + blockIdx.x] = r;
   for (int q = 0; q < 3; ++q) {
     double l = block_reduce<2>(lo[q], red);
-    if (threadIdx.x == 0) partials[blockIdx.x * kPartialSlots + 3 + q] = l;
+    if (threadIdx.x == 0) partials[(3 + q) * gridDim.x + blockIdx.x] = l;
     double u = block_reduce<1>(hi[q], red);
-    if (threadIdx.x == 0) partials[blockIdx.x * kPartialSlots + 6 + q] = u;
+    if (threadIdx.x == 0) partials[(6 + q) * gridDim.x + blockIdx.x] = u;
   }
 }
 

@@ -13,9 +13,15 @@ __global__ void __launch_bounds__(kBandThreads)
   fractions_body(phi, band, nb, d, h, occ, marks, run_mode, nty, ntz, nzr, cut, hi, partials, blockIdx.x, gridDim.x);
 }
 
+// 16 bytes per thread: p must be 16-byte aligned (the plan's hipMalloc'd marks)
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+  if (i + 16 <= n) {
+    const uint32_t w = v * 0x01010101u;
+    *reinterpret_cast<uint4*>(p + i) = make_uint4(w, w, w, w);
+  } else {
+    for (int64_t k = i; k < n; ++k) p[k] = v;
+  }
 }
 
 __global__ void __launch_bounds__(kBandThreads)
@@ -139,7 +145,8 @@ int svimx_fractions(void* stream, const float* phi, const int32_t* band, int64_t
   if (plan) {
     SVIMX_REQUIRE(plan->d.nx == nx && plan->d.ny == ny && plan->d.nz == nz, SVIMX_EBADARG,
                   "N_r plan grid does not match");
-    hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for(plan->nmarks, 256)), dim3(256), 0, st, plan->marks,
+    SVIMX_REQUIRE((reinterpret_cast<uintptr_t>(plan->marks) & 15) == 0, SVIMX_EBADARG, "misaligned N_r marks");
+    hipLaunchKernelGGL(k_fill_u8, dim3(blocks_for((plan->nmarks + 15) / 16, 256)), dim3(256), 0, st, plan->marks,
                        plan->nmarks, (uint8_t)0);
     SVIMX_CHECK_LAUNCH();
     marks = plan->marks;

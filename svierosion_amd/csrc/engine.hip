@@ -160,8 +160,7 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_eng_finalize_volume(const 
 __global__ void __launch_bounds__(256) k_eng_unit_need(const JobDev* __restrict__ J, const Seg* S) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    for (int64_t u = lb * 256 + threadIdx.x; u < j.nunits; u += G * 256)
-      unit_need(j.marks, j.dil, j.d.ny, j.nzr, j.nyu, j.ylines, u);
+    unit_need_range(j.marks, j.dil, j.d.ny, j.nzr, j.nyu, j.ylines, j.nunits, lb * 256 + threadIdx.x, G * 256);
   }
 }
 

@@ -59,8 +59,8 @@ __global__ void k_nr_mark_nodes(uint8_t* marks, int run_mode, int nty, int ntz, 
 
 __global__ void k_nr_unit_need(const uint8_t* __restrict__ marks, uint8_t* __restrict__ need, int ny, int nzr,
                                int nyu, int Y, int64_t nunits) {
-  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (u < nunits) unit_need(marks, need, ny, nzr, nyu, Y, u);
+  unit_need_range(marks, need, ny, nzr, nyu, Y, nunits, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                  (int64_t)gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------- run-mode summation
@@ -379,7 +379,10 @@ int svimx_nr_select(void* stream, void* plan) {
   if (p->fft) return SVIMX_OK;  // the FFT evaluates every node
   hipStream_t st = as_stream(stream);
   if (p->run_mode) {
-    hipLaunchKernelGGL(k_nr_unit_need, dim3(blocks_for(p->nunits, 256)), dim3(256), 0, st, p->marks, p->dil,
+    SVIMX_REQUIRE(((reinterpret_cast<uintptr_t>(p->marks) | reinterpret_cast<uintptr_t>(p->dil)) & 3) == 0,
+                  SVIMX_EBADARG, "misaligned N_r selection buffers");
+    const int64_t nthr = (p->nzr & 3) == 0 ? p->nunits / 4 : p->nunits;
+    hipLaunchKernelGGL(k_nr_unit_need, dim3(std::max(blocks_for(nthr, 256), 1)), dim3(256), 0, st, p->marks, p->dil,
                        p->d.ny, p->nzr, p->nyu, p->ylines, p->nunits);
     SVIMX_CHECK_LAUNCH();
     FlagPred pred{p->dil};

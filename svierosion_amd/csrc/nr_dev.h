@@ -29,6 +29,50 @@ __device__ __forceinline__ void unit_need(const uint8_t* __restrict__ marks, uin
   need[u] = v ? 1 : 0;
 }
 
+// The same flags for the four units 4q .. 4q+3 of one (x, yu) row, with
+// 32-bit loads (nzr a multiple of 4; marks and need 4-byte aligned): the 4·(Y+3)
+// stencil loads are fixed-trip and predicated, so all of them are in flight
+// at once; the word OR is the byte-wise OR, and each byte becomes 0/1.
+template <int Y>
+__device__ __forceinline__ void unit_need4(const uint8_t* __restrict__ marks, uint8_t* __restrict__ need, int ny,
+                                           int nzr, int nyu, int64_t q) {
+  const uint32_t u = (uint32_t)q * 4u;
+  const uint32_t xyu = u / (uint32_t)nzr;
+  const int zr = (int)(u - xyu * (uint32_t)nzr);
+  const int x = (int)(xyu / (uint32_t)nyu);
+  const int yu = (int)(xyu - (uint32_t)x * (uint32_t)nyu);
+  uint32_t v = 0;
+#pragma unroll
+  for (int a = 0; a <= 3; ++a) {
+    const uint8_t* row = marks + (int64_t)(x - a) * ny * nzr + zr;
+#pragma unroll
+    for (int k = 0; k < Y + 3; ++k) {
+      const int y = yu * Y - 3 + k;
+      if (a <= x && y >= 0 && y < ny) v |= *reinterpret_cast<const uint32_t*>(row + (int64_t)y * nzr);
+    }
+  }
+  const uint32_t t = ((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v;
+  *reinterpret_cast<uint32_t*>(need + u) = (t >> 7) & 0x01010101u;
+}
+
+// Need flags of units [4 q0, ...) for quads q = q0 + i·stride < nunits / 4
+// when nzr % 4 == 0, else of single units u = q0 + i·stride < nunits.
+__device__ __forceinline__ void unit_need_range(const uint8_t* __restrict__ marks, uint8_t* __restrict__ need,
+                                                int ny, int nzr, int nyu, int Y, int64_t nunits, int64_t q0,
+                                                int64_t stride) {
+  if ((nzr & 3) == 0) {
+    const int64_t nq = nunits >> 2;
+    if (Y == 4)
+      for (int64_t q = q0; q < nq; q += stride) unit_need4<4>(marks, need, ny, nzr, nyu, q);
+    else if (Y == 2)
+      for (int64_t q = q0; q < nq; q += stride) unit_need4<2>(marks, need, ny, nzr, nyu, q);
+    else
+      for (int64_t q = q0; q < nq; q += stride) unit_need4<1>(marks, need, ny, nzr, nyu, q);
+  } else {
+    for (int64_t u = q0; u < nunits; u += stride) unit_need(marks, need, ny, nzr, nyu, Y, u);
+  }
+}
+
 // Wave-level split of an n-entry work list over a launch of G blocks: the
 // work is cut into 64-entry wave items, each XCD (block lb runs on XCD lb mod
 // 8; G is a multiple of 8) takes a contiguous eighth of the items actually

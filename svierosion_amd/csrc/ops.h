@@ -382,19 +382,37 @@ __device__ __forceinline__ ReinitGeo reinit_geometry(const float* __restrict__ f
 
 // -sgn(phi0)(|grad phi| - 1) with ENO2 differences and the subcell fix
 // (svierosion/kernels.py:363-379).
-__device__ __forceinline__ float reinit_rhs(const float* __restrict__ f, int64_t n, const Dims& d,
-                                            float h, uint32_t flags, const float x[6]) {
+// The 13 values the right-hand side reads: the node and +-1, +-2 along each
+// axis, loaded apart from the arithmetic so a caller can put the loads of
+// several nodes in flight before computing any of them.
+struct Stencil13 {
+  float c, m2[3], m1[3], p1[3], p2[3];
+};
+__device__ __forceinline__ Stencil13 load_stencil13(const float* __restrict__ f, int64_t n, const Dims& d) {
   const int64_t st[3] = {d.sx(), d.sy(), 1};
+  Stencil13 S;
+  S.c = f[n];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t s = st[a];
+    S.m2[a] = f[n - 2 * s];
+    S.m1[a] = f[n - s];
+    S.p1[a] = f[n + s];
+    S.p2[a] = f[n + 2 * s];
+  }
+  return S;
+}
+
+__device__ __forceinline__ float reinit_rhs_of(const Stencil13& S, float h, uint32_t flags, const float x[6]) {
   const float ih = rcp(h), ihh = ih * ih;
-  float c = f[n];
+  const float c = S.c;
   uint32_t sb = (flags >> 6) & 3u;
   float sgn = sb == 1u ? 1.0f : (sb == 2u ? -1.0f : 0.0f);
   bool outside = sgn > 0.0f;
   float g2 = 0.0f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    int64_t s = st[a];
-    float m2 = f[n - 2 * s], m1 = f[n - s], p1 = f[n + s], p2 = f[n + 2 * s];
+    const float m2 = S.m2[a], m1 = S.m1[a], p1 = S.p1[a], p2 = S.p2[a];
     float mm_p = minmod(p1 - 2.0f * c + m1, p2 - 2.0f * p1 + c);
     float mm_m = minmod(p1 - 2.0f * c + m1, c - 2.0f * m1 + m2);
     bool cp = (flags >> (2 * a)) & 1u;
@@ -407,6 +425,11 @@ __device__ __forceinline__ float reinit_rhs(const float* __restrict__ f, int64_t
     g2 = g2 + (outside ? pos : neg);
   }
   return -sgn * (sqrtf(g2) - 1.0f);
+}
+
+__device__ __forceinline__ float reinit_rhs(const float* __restrict__ f, int64_t n, const Dims& d,
+                                            float h, uint32_t flags, const float x[6]) {
+  return reinit_rhs_of(load_stencil13(f, n, d), h, flags, x);
 }
 
 // ---------------------------------------------------------------- point-triangle distance

@@ -226,8 +226,9 @@ __global__ void k_reinit_geo(const float* __restrict__ phi, const int32_t* __res
 }
 
 // mode 0: out = in + dtau R(in);  mode 1: out = (out + in + dtau R(in)) / 2.
-// Each thread takes kSweepNodes work nodes kBandThreads apart (coalesced), so
-// a wave keeps that many nodes' stencil loads in flight.
+// Each thread takes kSweepNodes work nodes kBandThreads apart (coalesced) and
+// issues the stencil loads of all of them before computing any, so a wave
+// keeps kSweepNodes x 13 gathers in flight.
 constexpr int kSweepNodes = 2;
 __global__ void __launch_bounds__(256) k_reinit_sweep(int mode, const float* __restrict__ in, float* out,
                                                       const int32_t* __restrict__ idx, int64_t n, Dims d, float h,
@@ -244,6 +245,15 @@ __global__ void __launch_bounds__(256) k_reinit_sweep(int mode, const float* __r
     node[k] = i < n ? idx[i] : -1;
     word[k] = i < n ? info[i] : 0u;
   }
+  Stencil13 S[kSweepNodes];
+  float prev[kSweepNodes];
+#pragma unroll
+  for (int k = 0; k < kSweepNodes; ++k) {
+    if (node[k] >= 0) {
+      S[k] = load_stencil13(in, node[k], d);
+      prev[k] = mode == 0 ? 0.0f : out[node[k]];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < kSweepNodes; ++k) {
     if (node[k] < 0) continue;
@@ -255,11 +265,11 @@ __global__ void __launch_bounds__(256) k_reinit_sweep(int mode, const float* __r
       dtau = g[0];
       for (int q = 0; q < 6; ++q) x[q] = g[1 + q];
     }
-    const float r = reinit_rhs(in, node[k], d, h, word[k], x);
+    const float r = reinit_rhs_of(S[k], h, word[k], x);
     if (mode == 0) {
-      out[node[k]] = in[node[k]] + dtau * r;
+      out[node[k]] = S[k].c + dtau * r;
     } else {
-      out[node[k]] = 0.5f * (out[node[k]] + in[node[k]] + dtau * r);
+      out[node[k]] = 0.5f * (prev[k] + S[k].c + dtau * r);
     }
   }
 }

@@ -65,6 +65,7 @@ struct JobDev {
   int64_t nruns, nchunks;
   const NrColumn* cols;
   const int* col_start;
+  int gsplit, csplit;  // lo/hi split of the N_r sums (the plan's)
   const float* weights;
   float scale, negbv;
   int m, lds;
@@ -192,20 +193,22 @@ __global__ void __launch_bounds__(256) k_eng_nr_runs(const JobDev* __restrict__ 
     const JobDev& j = J[jslot];
     const int64_t nrun = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
-    nr_runs_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.weights,
-                     j.scale, j.negbv, const_cast<float*>(j.field), lb, G);
+    nr_runs_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.csplit,
+                         j.weights, j.scale, j.negbv, const_cast<float*>(j.field), lb, G);
   }
 }
 
 // Units of Y adjacent y-lines
 template <int MP, int RL, int Y>
 __global__ void __launch_bounds__(256) k_eng_nr_lines(const JobDev* __restrict__ J, const Seg* S) {
+  __shared__ float xch[nr_lines_xch<RL, Y, kNrLineLanes>()];
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
     const int64_t nunit = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nunit;
-    nr_lines_body<MP, RL, Y>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nunit, j.cols, j.col_start,
-                             j.m, j.weights, j.scale, j.negbv, const_cast<float*>(j.field), lb, G);
+    nr_lines_body<MP, RL, Y, kNrLineLanes>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nunit, j.cols,
+                                           j.col_start, j.m, j.gsplit, j.weights, j.scale, j.negbv,
+                                           const_cast<float*>(j.field), xch, lb, G);
   }
 }
 
@@ -219,7 +222,7 @@ __global__ void __launch_bounds__(64) k_eng_nr_lds(const JobDev* __restrict__ J,
     const int64_t nrun = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
     nr_lds_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.col_start, j.m,
-                        j.weights, j.scale, j.negbv, const_cast<float*>(j.field), slab, clu, lb, G);
+                        j.gsplit, j.weights, j.scale, j.negbv, const_cast<float*>(j.field), slab, clu, lb, G);
   }
 }
 
@@ -432,6 +435,8 @@ static void fill_jobdev(const EngJob& e, JobDev& j) {
     j.nchunks = p->nchunks;
     j.cols = p->cols;
     j.col_start = p->col_start;
+    j.gsplit = p->gsplit;
+    j.csplit = p->csplit;
     j.m = p->m;
     j.lds = p->lds;
     j.weights = p->weights;

@@ -132,77 +132,163 @@ __device__ __forceinline__ void taps_of(int k, const float* __restrict__ wc, con
   }
 }
 
-// Y lines per unit: one lane sums the RL-node z-runs of Y adjacent y-lines.
+constexpr int kNrLineLanes = 2;  // waves per unit group of the multi-line kernels (1 or 2)
+
+// Order of every run-mode sum (all shapes, so they agree bit for bit): the
+// columns are grouped by dx; a group's columns are summed into a fresh
+// partial, lo folds the partials of the groups g < gsplit in order starting
+// from 0.0, hi those of the groups g >= gsplit, and N_r comes from lo + hi.
+
+// Y lines per unit: a lane sums the RL-node z-runs of Y adjacent y-lines.
 // Window line y0 + dy' of plane x + dx feeds column (dx, dy' - yy) of line yy,
 // so every line's columns are applied in the order of the one-line kernel
 // (bit-identical sums) while each window is loaded once for the Y lines.
-// Needs each dx group's columns to have consecutive dy.
+// Needs each dx group's columns to have consecutive dy.  S = 2 gives each
+// unit two waves, one per half of the order (twice the waves for the same
+// loads, for lists that leave most SIMDs with 2-3 waves).
+// Sum of the dx groups [gb, ge) of the unit's Y lines, folded group by group
+// from 0.0 (one half of the canonical order).
 template <int MP, int RL, int Y>
-__device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
-                                              const int32_t* __restrict__ list, int64_t nunit,
+__device__ __forceinline__ void nr_lines_half(const float* __restrict__ occ, const OccLayout& OL,
                                               const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-                                              int m, const float* __restrict__ w, float scale, float negbv,
-                                              float* __restrict__ out, int64_t lb, int64_t G) {
+                                              const float* __restrict__ w, int x, int y0, int z0, int gb, int ge,
+                                              float (&sum)[Y][RL]) {
   constexpr int WIN = RL + 2 * MP;
-  const uint32_t nyu = (uint32_t)((OL.ny + Y - 1) / Y);
-  const WaveSplit ws = wave_split(nunit, lb, G);
-  for (int64_t item = ws.first; item < ws.end; item += ws.step) {
-    const int64_t it = item * 64 + (threadIdx.x & 63);
-    const bool live = it < nunit;
-    const uint32_t u = (uint32_t)list[live ? it : item * 64];
-    const uint32_t xyu = u / (uint32_t)nzr;
-    const int zr = (int)(u - xyu * (uint32_t)nzr);
-    const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
-    const int z0 = zr * RL;
-    float acc[Y][RL], part[Y][RL];
+  float acc[Y][RL], part[Y][RL];
 #pragma unroll
-    for (int yy = 0; yy < Y; ++yy)
+  for (int yy = 0; yy < Y; ++yy)
 #pragma unroll
-      for (int j = 0; j < RL; ++j) acc[yy][j] = part[yy][j] = 0.0f;
-    bool first_group = true;
-    for (int g = 0; g <= 2 * m; ++g) {
-      const int c0 = col_start[g], ncol = col_start[g + 1] - c0;
-      if (ncol == 0) continue;
-      if (!first_group) {
+    for (int j = 0; j < RL; ++j) acc[yy][j] = part[yy][j] = 0.0f;
+  bool first_group = true;
+  for (int g = gb; g < ge; ++g) {
+    const int c0 = col_start[g], ncol = col_start[g + 1] - c0;
+    if (ncol == 0) continue;
+    if (!first_group) {
 #pragma unroll
-        for (int yy = 0; yy < Y; ++yy)
+      for (int yy = 0; yy < Y; ++yy)
 #pragma unroll
-          for (int j = 0; j < RL; ++j) {
-            acc[yy][j] += part[yy][j];
-            part[yy][j] = 0.0f;
-          }
-      }
-      first_group = false;
-      const int dx = cols[c0].dx, dyf = cols[c0].dy;
-      for (int t = 0; t < ncol + Y - 1; ++t) {
-        const float4* src = reinterpret_cast<const float4*>(occ + OL.at(x + dx, y0 + dyf + t, z0 - MP));
-        float win[WIN];
-#pragma unroll
-        for (int q = 0; q < WIN / 4; ++q) {
-          const float4 v = src[q];
-          win[4 * q] = v.x;
-          win[4 * q + 1] = v.y;
-          win[4 * q + 2] = v.z;
-          win[4 * q + 3] = v.w;
+        for (int j = 0; j < RL; ++j) {
+          acc[yy][j] += part[yy][j];
+          part[yy][j] = 0.0f;
         }
+    }
+    first_group = false;
+    const int dx = cols[c0].dx, dyf = cols[c0].dy;
+    for (int t = 0; t < ncol + Y - 1; ++t) {
+      const float4* src = reinterpret_cast<const float4*>(occ + OL.at(x + dx, y0 + dyf + t, z0 - MP));
+      float win[WIN];
 #pragma unroll
-        for (int yy = 0; yy < Y; ++yy) {
-          const int ci = t - yy;
-          if (ci >= 0 && ci < ncol) {
-            const NrColumn col = cols[c0 + ci];
-            taps_of<MP, RL>(-col.dz0, w + col.woff, win, part[yy]);
-          }
+      for (int q = 0; q < WIN / 4; ++q) {
+        const float4 v = src[q];
+        win[4 * q] = v.x;
+        win[4 * q + 1] = v.y;
+        win[4 * q + 2] = v.z;
+        win[4 * q + 3] = v.w;
+      }
+#pragma unroll
+      for (int yy = 0; yy < Y; ++yy) {
+        const int ci = t - yy;
+        if (ci >= 0 && ci < ncol) {
+          const NrColumn col = cols[c0 + ci];
+          taps_of<MP, RL>(-col.dz0, w + col.woff, win, part[yy]);
         }
       }
     }
-    if (!live) continue;
+  }
 #pragma unroll
-    for (int yy = 0; yy < Y; ++yy) {
-      if (y0 + yy >= OL.ny) break;
-      const int64_t base = ((int64_t)x * OL.ny + y0 + yy) * OL.nz + z0;
+  for (int yy = 0; yy < Y; ++yy)
 #pragma unroll
-      for (int j = 0; j < RL; ++j)
-        if (z0 + j < OL.nz) out[base + j] = ((acc[yy][j] + part[yy][j]) * scale) * negbv + 1.0f;
+    for (int j = 0; j < RL; ++j) sum[yy][j] = acc[yy][j] + part[yy][j];
+}
+
+// LDS floats the multi-line body needs per block of 256 threads.
+template <int RL, int Y, int S>
+constexpr int nr_lines_xch() { return S == 2 ? Y * RL * 128 : 1; }
+
+template <int MP, int RL, int Y, int S>
+__device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
+                                              const int32_t* __restrict__ list, int64_t nunit,
+                                              const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
+                                              int m, int gsplit, const float* __restrict__ w, float scale,
+                                              float negbv, float* __restrict__ out, float* __restrict__ xch,
+                                              int64_t lb, int64_t G) {
+  static_assert(S == 1 || (S == 2 && Y % 2 == 0), "the two halves of a unit store half of its lines each");
+  const int lane = threadIdx.x & 63;
+  const uint32_t nyu = (uint32_t)((OL.ny + Y - 1) / Y);
+  if constexpr (S == 1) {
+    const WaveSplit ws = wave_split(nunit, lb, G);
+    for (int64_t item = ws.first; item < ws.end; item += ws.step) {
+      const int64_t it = item * 64 + lane;
+      const bool live = it < nunit;
+      const uint32_t u = (uint32_t)list[live ? it : item * 64];
+      const uint32_t xyu = u / (uint32_t)nzr;
+      const int zr = (int)(u - xyu * (uint32_t)nzr);
+      const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
+      const int z0 = zr * RL;
+      float lo[Y][RL], hi[Y][RL];
+      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, 0, gsplit, lo);
+      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, gsplit, 2 * m + 1, hi);
+      if (!live) continue;
+#pragma unroll
+      for (int yy = 0; yy < Y; ++yy) {
+        if (y0 + yy >= OL.ny) break;
+        const int64_t base = ((int64_t)x * OL.ny + y0 + yy) * OL.nz + z0;
+#pragma unroll
+        for (int j = 0; j < RL; ++j)
+          if (z0 + j < OL.nz) out[base + j] = ((lo[yy][j] + hi[yy][j]) * scale) * negbv + 1.0f;
+      }
+    }
+  } else {
+    // A block (4 waves) takes 128 units: waves 0-1 sum their lo halves, waves
+    // 2-3 the hi halves of the same units (each wave's loop bounds stay
+    // uniform).  The lo wave stores lines 0..YH-1, the hi wave lines YH..Y-1;
+    // each hands the other half's sums of the partner's lines over LDS, laid
+    // out [half][k][unit] (consecutive lanes, consecutive banks).
+    constexpr int YH = Y / 2, K = YH * RL;
+    // readfirstlane: the compiler cannot see that the wave index is uniform,
+    // and the half decides the (scalar) column loop
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int half = wave >> 1, slot = (wave & 1) * 64 + lane;
+    const int64_t items = (nunit + 127) >> 7;
+    const int64_t per = (items + kXcds - 1) / kXcds;  // contiguous items per XCD
+    const int64_t xcd = lb % kXcds;
+    const int64_t end = min(xcd * per + per, items);
+    for (int64_t item = xcd * per + lb / kXcds; item < end; item += G / kXcds) {  // block-uniform
+      const int64_t it = item * 128 + slot;
+      const bool live = it < nunit;
+      const uint32_t u = (uint32_t)list[live ? it : item * 128];
+      const uint32_t xyu = u / (uint32_t)nzr;
+      const int zr = (int)(u - xyu * (uint32_t)nzr);
+      const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
+      const int z0 = zr * RL;
+      float own[Y][RL];
+      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, half ? gsplit : 0,
+                               half ? 2 * m + 1 : gsplit, own);
+#pragma unroll
+      for (int yl = 0; yl < YH; ++yl)
+#pragma unroll
+        for (int j = 0; j < RL; ++j)
+          xch[(half * K + yl * RL + j) * 128 + slot] = half ? own[yl][j] : own[YH + yl][j];
+      __syncthreads();
+      float tot[YH][RL];
+#pragma unroll
+      for (int yl = 0; yl < YH; ++yl)
+#pragma unroll
+        for (int j = 0; j < RL; ++j) {
+          const float other = xch[((1 - half) * K + yl * RL + j) * 128 + slot];
+          tot[yl][j] = half ? other + own[YH + yl][j] : own[yl][j] + other;  // lo + hi
+        }
+      __syncthreads();  // the exchange buffer is rewritten by the next item
+      if (!live) continue;
+#pragma unroll
+      for (int yl = 0; yl < YH; ++yl) {
+        const int yy = half * YH + yl;
+        if (y0 + yy >= OL.ny) continue;
+        const int64_t base = ((int64_t)x * OL.ny + y0 + yy) * OL.nz + z0;
+#pragma unroll
+        for (int j = 0; j < RL; ++j)
+          if (z0 + j < OL.nz) out[base + j] = (tot[yl][j] * scale) * negbv + 1.0f;
+      }
     }
   }
 }
@@ -212,7 +298,7 @@ __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, con
 template <int MP, int RL>
 __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
                                              const int32_t* __restrict__ list, int64_t nrun,
-                                             const NrColumn* __restrict__ cols, int ncols,
+                                             const NrColumn* __restrict__ cols, int ncols, int csplit,
                                              const float* __restrict__ w, float scale, float negbv,
                                              float* __restrict__ out, int64_t lb, int64_t G) {
   constexpr int WIN = RL + 2 * MP;
@@ -225,13 +311,20 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
     const int64_t xy = r / nzr;
     const int x = (int)((uint32_t)xy / (uint32_t)OL.ny), y = (int)((uint32_t)xy - (uint32_t)x * (uint32_t)OL.ny);
     const int z0 = zr * RL;
-    float acc[RL], part[RL];
+    float lo[RL], acc[RL], part[RL];
 #pragma unroll
-    for (int j = 0; j < RL; ++j) acc[j] = part[j] = 0.0f;
+    for (int j = 0; j < RL; ++j) lo[j] = acc[j] = part[j] = 0.0f;
     int cur_dx = cols[0].dx;
     for (int c = 0; c < ncols; ++c) {
       const NrColumn col = cols[c];
-      if (col.dx != cur_dx) {
+      if (c == csplit) {  // first hi group: lo is complete
+#pragma unroll
+        for (int j = 0; j < RL; ++j) {
+          lo[j] = acc[j] + part[j];
+          acc[j] = part[j] = 0.0f;
+        }
+        cur_dx = col.dx;
+      } else if (col.dx != cur_dx) {
 #pragma unroll
         for (int j = 0; j < RL; ++j) {
           acc[j] += part[j];
@@ -275,7 +368,7 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
     const int64_t base = xy * OL.nz + z0;
 #pragma unroll
     for (int j = 0; j < RL; ++j)
-      if (z0 + j < OL.nz) out[base + j] = ((acc[j] + part[j]) * scale) * negbv + 1.0f;
+      if (z0 + j < OL.nz) out[base + j] = ((lo[j] + (acc[j] + part[j])) * scale) * negbv + 1.0f;
   }
 }
 
@@ -358,8 +451,8 @@ template <int MP, int RL>
 __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
                                             const int32_t* __restrict__ list, int64_t nrun,
                                             const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-                                            int m, const float* __restrict__ w, float scale, float negbv,
-                                            float* __restrict__ out, float4* __restrict__ slab,
+                                            int m, int gsplit, const float* __restrict__ w, float scale,
+                                            float negbv, float* __restrict__ out, float4* __restrict__ slab,
                                             NrLdsClu* __restrict__ clu, int64_t wv, int64_t NW) {
   constexpr int DZ = nr_lds_dz(RL), S = RL / 4;
   static_assert((2 * MP + 1) * nr_lds_pitch(((2 * DZ + 1) * RL + 2 * MP) / 4, S) <= kNrLdsCap4,
@@ -459,9 +552,10 @@ __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const
       // ---- sums over the dx planes
       const bool act = cid >= 0;
       const float4* my_row = slab + my_base + (y - my_ylo + m) * my_p4 + (zr - my_zlo);
-      float acc[RL], part[RL];
+      float lo[RL], acc[RL], part[RL];
 #pragma unroll
-      for (int j = 0; j < RL; ++j) acc[j] = part[j] = 0.0f;
+      for (int j = 0; j < RL; ++j) lo[j] = acc[j] = part[j] = 0.0f;
+      bool in_hi = false;
       for (int g = gfirst; g <= 2 * m;) {
         const int c0 = col_start[g], c1 = col_start[g + 1];
         int gn = g + 1;
@@ -473,7 +567,14 @@ __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const
           const int off = (gn - m) * plane4;
           NR_LDS_SLOTS(NR_SLOT_ISSUE)
         }
-        if (g != gfirst) {
+        if (!in_hi && g >= gsplit) {  // first hi group: lo is complete
+#pragma unroll
+          for (int j = 0; j < RL; ++j) {
+            lo[j] = acc[j] + part[j];
+            acc[j] = part[j] = 0.0f;
+          }
+          in_hi = true;
+        } else if (g != gfirst) {
 #pragma unroll
           for (int j = 0; j < RL; ++j) {
             acc[j] += part[j];
@@ -514,7 +615,7 @@ __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const
         const int64_t base = xy * OL.nz + z0;
 #pragma unroll
         for (int j = 0; j < RL; ++j)
-          if (z0 + j < OL.nz) out[base + j] = ((acc[j] + part[j]) * scale) * negbv + 1.0f;
+          if (z0 + j < OL.nz) out[base + j] = ((lo[j] + (acc[j] + part[j])) * scale) * negbv + 1.0f;
       }
       pending = pending && !act;
 #undef NR_SLOT_ISSUE

@@ -61,6 +61,8 @@ struct NrPlan {
   float negbv;        // float32(-1 / (4/3 pi r^3))
   NrColumn* cols;     // device, grouped by dx
   int* col_start;     // device, 2m+2 entries: columns of dx = g - m are [col_start[g], col_start[g+1])
+  int gsplit;         // run mode: a node's sum is lo + hi, lo over the dx groups g < gsplit,
+  int csplit;         //   hi over the rest (each folded group by group); csplit = col_start[gsplit]
   float* weights;     // device; run mode: per column the taps dz = -k..k
   uint8_t* marks;     // device, nmarks
   uint8_t* dil;       // device, nruns bytes (run mode: need flag of each unit)

@@ -154,15 +154,16 @@ int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t 
  * created with the FFT.  The FFT allocates 12 bytes per grid node. */
 int svimx_nr_plan_set_method(void* plan, int method);
 int svimx_nr_plan_method(void* plan);
-/* Autotuner: times the run-mode summation over SVIMX_NR_TUNE_SHAPES shapes
- * (z-run length 4/8/16 x y-lines per unit 1/2/4, and the LDS-staged
- * one-line kernel; svimx_nr_tune_shape names shape i) on this job's field
- * (fractions and selection recomputed per candidate; the N_r values at the
- * nodes the velocity reads are identical for every shape) and keeps the
- * fastest.  ms_out (SVIMX_NR_TUNE_SHAPES doubles, NaN for shapes this plan
- * cannot take; may be NULL) receives the per-launch times.  Leaves occupancy,
- * marks and unit list those of the chosen shape for phi.  Synchronises.  No
- * counterpart in the reference. */
+/* Autotuner: times the run-mode summation over the shapes read through the
+ * L1 (z-run length 4/8/16 x y-lines per unit 1/2/4; svimx_nr_tune_shape
+ * names shape i of SVIMX_NR_TUNE_SHAPES, which also lists the LDS-staged
+ * one-line kernels) on this job's field (fractions and selection recomputed
+ * per candidate; these shapes sum in one order, so the N_r values at the
+ * nodes the velocity reads are identical for every one) and keeps the
+ * fastest.  ms_out (SVIMX_NR_TUNE_SHAPES doubles, NaN for the LDS shapes and
+ * for shapes this plan cannot take; may be NULL) receives the per-launch
+ * times.  Leaves occupancy, marks and unit list those of the chosen shape for
+ * phi.  Synchronises.  No counterpart in the reference. */
 #define SVIMX_NR_TUNE_SHAPES 11
 int svimx_nr_tune_shape(int i, int* run_length, int* lines, int* lds);
 int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
@@ -178,9 +179,10 @@ int svimx_nr_plan_set_run_length(void* plan, int rl);
  * at most 8 nodes.  No counterpart in the reference. */
 int svimx_nr_plan_set_lines(void* plan, int lines);
 int svimx_nr_plan_lines(void* plan);
-/* Run mode: 1 stages the occupancy through LDS (clusters of runs per wave),
- * 0 reads each run's windows through the L1.  Same sums in the same order,
- * so the results are bit-identical.  No counterpart in the reference. */
+/* Run mode: 1 stages the occupancy through LDS (clusters of runs per wave,
+ * one dx plane at a time, so the planes are summed in their plain order:
+ * equal to the L1 shapes up to FP32 rounding), 0 reads each run's windows
+ * through the L1.  No counterpart in the reference. */
 int svimx_nr_plan_set_staging(void* plan, int lds);
 int svimx_nr_plan_staging(void* plan);
 int svimx_nr_exec(void* stream, void* plan, const float* occupancy, float* nr_out, void* ws,

@@ -65,7 +65,7 @@ struct JobDev {
   int64_t nruns, nchunks;
   const NrColumn* cols;
   const int* col_start;
-  int gsplit, csplit;  // lo/hi split of the N_r sums (the plan's)
+  int asplit, cfirst, csplit;  // the plan's N_r summation order
   const float* weights;
   float scale, negbv;
   int m, lds;
@@ -193,8 +193,8 @@ __global__ void __launch_bounds__(256) k_eng_nr_runs(const JobDev* __restrict__ 
     const JobDev& j = J[jslot];
     const int64_t nrun = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
-    nr_runs_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.csplit,
-                         j.weights, j.scale, j.negbv, const_cast<float*>(j.field), lb, G);
+    nr_runs_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.ncols, j.cfirst,
+                         j.csplit, j.weights, j.scale, j.negbv, const_cast<float*>(j.field), lb, G);
   }
 }
 
@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256) k_eng_nr_lines(const JobDev* __restrict__
     const int64_t nunit = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nunit;
     nr_lines_body<MP, RL, Y, kNrLineLanes>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nunit, j.cols,
-                                           j.col_start, j.m, j.gsplit, j.weights, j.scale, j.negbv,
+                                           j.col_start, j.m, j.asplit, j.weights, j.scale, j.negbv,
                                            const_cast<float*>(j.field), xch, lb, G);
   }
 }
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(64) k_eng_nr_lds(const JobDev* __restrict__ J,
     const int64_t nrun = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nrun;
     nr_lds_body<MP, RL>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nrun, j.cols, j.col_start, j.m,
-                        j.gsplit, j.weights, j.scale, j.negbv, const_cast<float*>(j.field), slab, clu, lb, G);
+                        j.weights, j.scale, j.negbv, const_cast<float*>(j.field), slab, clu, lb, G);
   }
 }
 
@@ -435,7 +435,8 @@ static void fill_jobdev(const EngJob& e, JobDev& j) {
     j.nchunks = p->nchunks;
     j.cols = p->cols;
     j.col_start = p->col_start;
-    j.gsplit = p->gsplit;
+    j.asplit = p->asplit;
+    j.cfirst = p->cfirst;
     j.csplit = p->csplit;
     j.m = p->m;
     j.lds = p->lds;

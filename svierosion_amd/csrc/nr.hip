@@ -67,13 +67,13 @@ __global__ void k_nr_unit_need(const uint8_t* __restrict__ marks, uint8_t* __res
 template <int MP, int RL>
 __global__ void __launch_bounds__(256)
     k_nr_runs(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
-              const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, int ncols, int csplit,
-              const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
+              const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, int ncols, int cfirst,
+              int csplit, const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
               double* __restrict__ results) {
   const int64_t nrun = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
   // neighbouring runs read the same occupancy columns: keep them on one XCD's L2
-  nr_runs_body<MP, RL>(occ, OL, nzr, list, nrun, cols, ncols, csplit, w, scale, negbv, out, blockIdx.x,
+  nr_runs_body<MP, RL>(occ, OL, nzr, list, nrun, cols, ncols, cfirst, csplit, w, scale, negbv, out, blockIdx.x,
                        gridDim.x);
 }
 
@@ -82,12 +82,12 @@ template <int MP, int RL, int Y>
 __global__ void __launch_bounds__(256)
     k_nr_lines(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
                const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-               int m, int gsplit, const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
+               int m, int asplit, const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
                double* __restrict__ results) {
   const int64_t nunit = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nunit;
   __shared__ float xch[nr_lines_xch<RL, Y, kNrLineLanes>()];
-  nr_lines_body<MP, RL, Y, kNrLineLanes>(occ, OL, nzr, list, nunit, cols, col_start, m, gsplit, w, scale, negbv,
+  nr_lines_body<MP, RL, Y, kNrLineLanes>(occ, OL, nzr, list, nunit, cols, col_start, m, asplit, w, scale, negbv,
                                          out, xch, blockIdx.x, gridDim.x);
 }
 
@@ -96,13 +96,13 @@ template <int MP, int RL>
 __global__ void __launch_bounds__(64)
     k_nr_lds(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
              const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-             int m, int gsplit, const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
+             int m, const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
              double* __restrict__ results) {
   __shared__ float4 slab[kNrLdsCap4];
   __shared__ NrLdsClu clu[kNrLdsClusters];
   const int64_t nrun = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nrun;
-  nr_lds_body<MP, RL>(occ, OL, nzr, list, nrun, cols, col_start, m, gsplit, w, scale, negbv, out, slab, clu,
+  nr_lds_body<MP, RL>(occ, OL, nzr, list, nrun, cols, col_start, m, w, scale, negbv, out, slab, clu,
                       blockIdx.x, gridDim.x);
 }
 
@@ -244,22 +244,27 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   col_start[W] = (int)cols.size();
   NrPlan* p = new NrPlan();
   p->m = m;
-  // the split into lo and hi groups with the most even tap counts (the two
-  // lanes that share a unit in the line kernels take one half each)
+  // the lo / hi split of the |dx| groups, as even as possible in a cost of
+  // taps plus window loads (the two waves that share a unit group in the
+  // multi-line kernels take one half each); it depends on the ball only, so
+  // every shape sums in the same order
   {
-    std::vector<int64_t> taps(W, 0);
+    std::vector<int64_t> cost(m + 1, 0);
     int64_t all = 0;
-    for (int g = 0; g < W; ++g)
-      for (int c = col_start[g]; c < col_start[g + 1]; ++c) taps[g] += 2 * (int64_t)(-cols[c].dz0) + 1;
-    for (int g = 0; g < W; ++g) all += taps[g];
-    int64_t lo = 0, best = INT64_MAX;
-    p->gsplit = W;
-    for (int g = 0; g <= W; ++g) {
-      const int64_t diff = std::llabs(all - 2 * lo);
-      if (diff < best) best = diff, p->gsplit = g;
-      if (g < W) lo += taps[g];
+    for (int a = 0; a <= m; ++a) {
+      for (int c = col_start[m + a]; c < col_start[m + a + 1]; ++c) cost[a] += 2 * (int64_t)(-cols[c].dz0) + 1;
+      cost[a] += 8 * (int64_t)(col_start[m + a + 1] - col_start[m + a]) * (a > 0 ? 2 : 1);
+      all += cost[a];
     }
-    p->csplit = col_start[p->gsplit];
+    int64_t lo = 0, best = INT64_MAX;
+    p->asplit = m + 1;
+    for (int a = 0; a <= m + 1; ++a) {
+      const int64_t diff = std::llabs(all - 2 * lo);
+      if (diff < best) best = diff, p->asplit = a;
+      if (a <= m) lo += cost[a];
+    }
+    p->cfirst = col_start[m];
+    p->csplit = col_start[m + p->asplit];
   }
   p->weights_host.assign(weights, weights + (size_t)W * W * W);
   p->h = h;
@@ -443,20 +448,19 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
     const int grid_lds = (int)std::min<int64_t>((p->nruns + 63) / 64 + kXcds - 1, 256 * 10) / kXcds * kXcds;
 #define SVIMX_LINES(MPV, RLV, YV)                                                                              \
   hipLaunchKernelGGL((k_nr_lines<MPV, RLV, YV>), dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, \
-                     total, p->cols, p->col_start, p->m, p->gsplit, p->weights, p->scale, p->negbv, nr_out,   \
+                     total, p->cols, p->col_start, p->m, p->asplit, p->weights, p->scale, p->negbv, nr_out,   \
                      w.results)
 #define SVIMX_RUNS(MPV, RLV)                                                                                       \
   if (p->lds)                                                                                                      \
     hipLaunchKernelGGL((k_nr_lds<MPV, RLV>), dim3(grid_lds), dim3(64), 0, st, occupancy, OL, p->nzr, p->list,      \
-                       total, p->cols, p->col_start, p->m, p->gsplit, p->weights, p->scale, p->negbv, nr_out,     \
-                       w.results);                                                                                 \
+                       total, p->cols, p->col_start, p->m, p->weights, p->scale, p->negbv, nr_out, w.results);    \
   else if (p->ylines == 2)                                                                                         \
     SVIMX_LINES(MPV, RLV, 2);                                                                                      \
   else if (p->ylines == 4 && RLV < 16)                                                                             \
     SVIMX_LINES(MPV, (RLV < 16 ? RLV : 8), 4);                                                                     \
   else                                                                                                             \
     hipLaunchKernelGGL((k_nr_runs<MPV, RLV>), dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, total, \
-                       p->cols, p->ncols, p->csplit, p->weights, p->scale, p->negbv, nr_out, w.results)
+                       p->cols, p->ncols, p->cfirst, p->csplit, p->weights, p->scale, p->negbv, nr_out, w.results)
 #define SVIMX_RUNS_MP(RLV)                  \
   switch (mp) {                             \
     case 0:                                 \
@@ -502,7 +506,10 @@ int svimx_nr_plan_tune(void* stream, void* plan, const float* phi, const int32_t
   for (int i = 0; i < kNrTuneCands && !rc; ++i) {
     if (ms_out) ms_out[i] = NAN;
     const NrShape sh = kNrTuneShapes[i];
-    if (!nr_shape_allowed(p, sh)) continue;
+    // the LDS staging keeps its own summation order: the tuner picks among
+    // the shapes whose sums agree bit for bit, so a job's results never
+    // depend on the timings
+    if (sh.lds || !nr_shape_allowed(p, sh)) continue;
     nr_plan_set_shape(p, sh.rl, sh.lines);
     p->lds = sh.lds;
     rc = svimx_fractions(stream, phi, band, nb, nx, ny, nz, h, occupancy, p, cut_hi, ws, ws_bytes);

@@ -134,10 +134,15 @@ __device__ __forceinline__ void taps_of(int k, const float* __restrict__ wc, con
 
 constexpr int kNrLineLanes = 2;  // waves per unit group of the multi-line kernels (1 or 2)
 
-// Order of every run-mode sum (all shapes, so they agree bit for bit): the
-// columns are grouped by dx; a group's columns are summed into a fresh
-// partial, lo folds the partials of the groups g < gsplit in order starting
-// from 0.0, hi those of the groups g >= gsplit, and N_r comes from lo + hi.
+// Order of the run-mode sums read through the L1 (every such shape, so they
+// agree bit for bit and the autotuner's pick never changes a result): the
+// columns of dx = +a (a = |dx| = 0..m) form group a; for a > 0 each window of
+// plane x + a is added element-wise to the same window of plane x - a before
+// the group's taps (the ball is mirror-symmetric in x, so the +a columns'
+// weights serve both planes: half the FMAs); a group's columns are summed
+// into a fresh partial, lo folds the partials of the groups a < asplit in
+// order starting from 0.0, hi those of the groups a >= asplit, and N_r comes
+// from lo + hi.
 
 // Y lines per unit: a lane sums the RL-node z-runs of Y adjacent y-lines.
 // Window line y0 + dy' of plane x + dx feeds column (dx, dy' - yy) of line yy,
@@ -146,13 +151,40 @@ constexpr int kNrLineLanes = 2;  // waves per unit group of the multi-line kerne
 // Needs each dx group's columns to have consecutive dy.  S = 2 gives each
 // unit two waves, one per half of the order (twice the waves for the same
 // loads, for lists that leave most SIMDs with 2-3 waves).
-// Sum of the dx groups [gb, ge) of the unit's Y lines, folded group by group
-// from 0.0 (one half of the canonical order).
+// The window z0 - MP .. z0 + RL + MP - 1 of line y in plane x + a, plus (for
+// a > 0) the same window of plane x - a, as aligned float4 loads.
+template <int MP, int RL>
+__device__ __forceinline__ void nr_window(const float* __restrict__ occ, const OccLayout& OL, int x, int a, int y,
+                                          int z0, float (&win)[RL + 2 * MP]) {
+  const float4* src = reinterpret_cast<const float4*>(occ + OL.at(x + a, y, z0 - MP));
+#pragma unroll
+  for (int q = 0; q < (RL + 2 * MP) / 4; ++q) {
+    const float4 v = src[q];
+    win[4 * q] = v.x;
+    win[4 * q + 1] = v.y;
+    win[4 * q + 2] = v.z;
+    win[4 * q + 3] = v.w;
+  }
+  if (a > 0) {
+    const float4* mir = reinterpret_cast<const float4*>(occ + OL.at(x - a, y, z0 - MP));
+#pragma unroll
+    for (int q = 0; q < (RL + 2 * MP) / 4; ++q) {
+      const float4 v = mir[q];
+      win[4 * q] += v.x;
+      win[4 * q + 1] += v.y;
+      win[4 * q + 2] += v.z;
+      win[4 * q + 3] += v.w;
+    }
+  }
+}
+
+// Sum of the |dx| groups [ab, ae) of the unit's Y lines, folded group by
+// group from 0.0 (one half of the canonical order).
 template <int MP, int RL, int Y>
 __device__ __forceinline__ void nr_lines_half(const float* __restrict__ occ, const OccLayout& OL,
                                               const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-                                              const float* __restrict__ w, int x, int y0, int z0, int gb, int ge,
-                                              float (&sum)[Y][RL]) {
+                                              const float* __restrict__ w, int x, int y0, int z0, int m, int ab,
+                                              int ae, float (&sum)[Y][RL]) {
   constexpr int WIN = RL + 2 * MP;
   float acc[Y][RL], part[Y][RL];
 #pragma unroll
@@ -160,8 +192,8 @@ __device__ __forceinline__ void nr_lines_half(const float* __restrict__ occ, con
 #pragma unroll
     for (int j = 0; j < RL; ++j) acc[yy][j] = part[yy][j] = 0.0f;
   bool first_group = true;
-  for (int g = gb; g < ge; ++g) {
-    const int c0 = col_start[g], ncol = col_start[g + 1] - c0;
+  for (int a = ab; a < ae; ++a) {
+    const int c0 = col_start[m + a], ncol = col_start[m + a + 1] - c0;
     if (ncol == 0) continue;
     if (!first_group) {
 #pragma unroll
@@ -173,18 +205,10 @@ __device__ __forceinline__ void nr_lines_half(const float* __restrict__ occ, con
         }
     }
     first_group = false;
-    const int dx = cols[c0].dx, dyf = cols[c0].dy;
+    const int dyf = cols[c0].dy;
     for (int t = 0; t < ncol + Y - 1; ++t) {
-      const float4* src = reinterpret_cast<const float4*>(occ + OL.at(x + dx, y0 + dyf + t, z0 - MP));
       float win[WIN];
-#pragma unroll
-      for (int q = 0; q < WIN / 4; ++q) {
-        const float4 v = src[q];
-        win[4 * q] = v.x;
-        win[4 * q + 1] = v.y;
-        win[4 * q + 2] = v.z;
-        win[4 * q + 3] = v.w;
-      }
+      nr_window<MP, RL>(occ, OL, x, a, y0 + dyf + t, z0, win);
 #pragma unroll
       for (int yy = 0; yy < Y; ++yy) {
         const int ci = t - yy;
@@ -209,7 +233,7 @@ template <int MP, int RL, int Y, int S>
 __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
                                               const int32_t* __restrict__ list, int64_t nunit,
                                               const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-                                              int m, int gsplit, const float* __restrict__ w, float scale,
+                                              int m, int asplit, const float* __restrict__ w, float scale,
                                               float negbv, float* __restrict__ out, float* __restrict__ xch,
                                               int64_t lb, int64_t G) {
   static_assert(S == 1 || (S == 2 && Y % 2 == 0), "the two halves of a unit store half of its lines each");
@@ -226,8 +250,8 @@ __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, con
       const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
       const int z0 = zr * RL;
       float lo[Y][RL], hi[Y][RL];
-      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, 0, gsplit, lo);
-      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, gsplit, 2 * m + 1, hi);
+      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, m, 0, asplit, lo);
+      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, m, asplit, m + 1, hi);
       if (!live) continue;
 #pragma unroll
       for (int yy = 0; yy < Y; ++yy) {
@@ -262,8 +286,8 @@ __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, con
       const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
       const int z0 = zr * RL;
       float own[Y][RL];
-      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, half ? gsplit : 0,
-                               half ? 2 * m + 1 : gsplit, own);
+      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, m, half ? asplit : 0,
+                               half ? m + 1 : asplit, own);
 #pragma unroll
       for (int yl = 0; yl < YH; ++yl)
 #pragma unroll
@@ -298,8 +322,8 @@ __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, con
 template <int MP, int RL>
 __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
                                              const int32_t* __restrict__ list, int64_t nrun,
-                                             const NrColumn* __restrict__ cols, int ncols, int csplit,
-                                             const float* __restrict__ w, float scale, float negbv,
+                                             const NrColumn* __restrict__ cols, int ncols, int cfirst,
+                                             int csplit, const float* __restrict__ w, float scale, float negbv,
                                              float* __restrict__ out, int64_t lb, int64_t G) {
   constexpr int WIN = RL + 2 * MP;
   const WaveSplit ws = wave_split(nrun, lb, G);
@@ -314,8 +338,8 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
     float lo[RL], acc[RL], part[RL];
 #pragma unroll
     for (int j = 0; j < RL; ++j) lo[j] = acc[j] = part[j] = 0.0f;
-    int cur_dx = cols[0].dx;
-    for (int c = 0; c < ncols; ++c) {
+    int cur_dx = cols[cfirst].dx;
+    for (int c = cfirst; c < ncols; ++c) {  // the columns of dx >= 0
       const NrColumn col = cols[c];
       if (c == csplit) {  // first hi group: lo is complete
 #pragma unroll
@@ -332,16 +356,8 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
         }
         cur_dx = col.dx;
       }
-      const float4* src = reinterpret_cast<const float4*>(occ + OL.at(x + col.dx, y + col.dy, z0 - MP));
       float win[WIN];
-#pragma unroll
-      for (int q = 0; q < WIN / 4; ++q) {
-        const float4 v = src[q];
-        win[4 * q] = v.x;
-        win[4 * q + 1] = v.y;
-        win[4 * q + 2] = v.z;
-        win[4 * q + 3] = v.w;
-      }
+      nr_window<MP, RL>(occ, OL, x, col.dx, y + col.dy, z0, win);
       const float* wc = w + col.woff;
       switch (-col.dz0) {
         case 0: column_taps<0, MP, RL>(wc, win, part); break;
@@ -374,8 +390,10 @@ __device__ __forceinline__ void nr_runs_body(const float* __restrict__ occ, cons
 
 
 // ---------------------------------------------------------------- LDS-staged run summation
-// The same sums as nr_runs_body, in the same order (bit-identical results),
-// with the occupancy staged through LDS.  Per wave of 64 runs, the runs are
+// The one-line run sums with the occupancy staged through LDS, one dx plane at
+// a time, so in the plain order: the dx groups -m..m folded in turn from 0.0
+// (the same value as the L1 shapes up to rounding; an explicit option that
+// the autotuner does not pick).  Per wave of 64 runs, the runs are
 // grouped into up to kNrLdsClusters clusters (same x, z-runs within +-Dz and
 // lines within +-reach of the first pending run; the first cluster of a pass
 // shrinks its reach until it fits, down to one line), each with a bounding
@@ -451,8 +469,8 @@ template <int MP, int RL>
 __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
                                             const int32_t* __restrict__ list, int64_t nrun,
                                             const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-                                            int m, int gsplit, const float* __restrict__ w, float scale,
-                                            float negbv, float* __restrict__ out, float4* __restrict__ slab,
+                                            int m, const float* __restrict__ w, float scale, float negbv,
+                                            float* __restrict__ out, float4* __restrict__ slab,
                                             NrLdsClu* __restrict__ clu, int64_t wv, int64_t NW) {
   constexpr int DZ = nr_lds_dz(RL), S = RL / 4;
   static_assert((2 * MP + 1) * nr_lds_pitch(((2 * DZ + 1) * RL + 2 * MP) / 4, S) <= kNrLdsCap4,
@@ -552,10 +570,9 @@ __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const
       // ---- sums over the dx planes
       const bool act = cid >= 0;
       const float4* my_row = slab + my_base + (y - my_ylo + m) * my_p4 + (zr - my_zlo);
-      float lo[RL], acc[RL], part[RL];
+      float acc[RL], part[RL];
 #pragma unroll
-      for (int j = 0; j < RL; ++j) lo[j] = acc[j] = part[j] = 0.0f;
-      bool in_hi = false;
+      for (int j = 0; j < RL; ++j) acc[j] = part[j] = 0.0f;
       for (int g = gfirst; g <= 2 * m;) {
         const int c0 = col_start[g], c1 = col_start[g + 1];
         int gn = g + 1;
@@ -567,14 +584,7 @@ __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const
           const int off = (gn - m) * plane4;
           NR_LDS_SLOTS(NR_SLOT_ISSUE)
         }
-        if (!in_hi && g >= gsplit) {  // first hi group: lo is complete
-#pragma unroll
-          for (int j = 0; j < RL; ++j) {
-            lo[j] = acc[j] + part[j];
-            acc[j] = part[j] = 0.0f;
-          }
-          in_hi = true;
-        } else if (g != gfirst) {
+        if (g != gfirst) {
 #pragma unroll
           for (int j = 0; j < RL; ++j) {
             acc[j] += part[j];
@@ -615,7 +625,7 @@ __device__ __forceinline__ void nr_lds_body(const float* __restrict__ occ, const
         const int64_t base = xy * OL.nz + z0;
 #pragma unroll
         for (int j = 0; j < RL; ++j)
-          if (z0 + j < OL.nz) out[base + j] = ((lo[j] + (acc[j] + part[j])) * scale) * negbv + 1.0f;
+          if (z0 + j < OL.nz) out[base + j] = ((acc[j] + part[j]) * scale) * negbv + 1.0f;
       }
       pending = pending && !act;
 #undef NR_SLOT_ISSUE

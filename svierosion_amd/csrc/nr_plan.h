@@ -61,8 +61,12 @@ struct NrPlan {
   float negbv;        // float32(-1 / (4/3 pi r^3))
   NrColumn* cols;     // device, grouped by dx
   int* col_start;     // device, 2m+2 entries: columns of dx = g - m are [col_start[g], col_start[g+1])
-  int gsplit;         // run mode: a node's sum is lo + hi, lo over the dx groups g < gsplit,
-  int csplit;         //   hi over the rest (each folded group by group); csplit = col_start[gsplit]
+  // run-mode sum order (every L1 shape): |dx| groups a = 0..m, group a > 0
+  // summing the windows of planes x + a and x - a before the taps of its
+  // columns (those of dx = +a); lo folds the groups a < asplit, hi the rest
+  int asplit;
+  int cfirst;         // col_start[m]: the first column of dx = 0
+  int csplit;         // col_start[m + asplit]: the first column of the hi groups
   float* weights;     // device; run mode: per column the taps dz = -k..k
   uint8_t* marks;     // device, nmarks
   uint8_t* dil;       // device, nruns bytes (run mode: need flag of each unit)

@@ -179,10 +179,11 @@ class BallConvolution:
 
     def tune(self, ls, cut_hi: float) -> dict:
         """Autotune the summation's shape (z-run length 4/8/16, y-lines per unit
-        1/2/4, or the LDS-staged one-line kernel) on the level set's current
-        field; returns {shape name ("8", "8x2", "8/lds"): ms per launch} for
-        the shapes this plan can take.  The N_r values the velocity reads do
-        not depend on the choice.  Leaves the occupancy and N_r marks of
+        1/2/4) on the level set's current field; returns {shape name ("8",
+        "8x2"): ms per launch} for the shapes this plan can take.  These
+        shapes sum in one order, so the N_r values the velocity reads do not
+        depend on the choice; the LDS-staged kernel (``staging = "lds"``) sums
+        in another and is never picked.  Leaves the occupancy and N_r marks of
         ``ls``'s current field in place."""
         ms = (C.c_double * _TUNE_SHAPES)()
         field = torch.empty(ls.grid.size, dtype=torch.float32, device=self.device)

@@ -107,10 +107,13 @@ def test_engine_job_equals_single_job_path(device, name):
 
 @pytest.mark.parametrize("shape", ["lds", "y2", "y4"])
 def test_nr_shapes_are_result_neutral_in_the_engine(device, shape):
-    """The engine's LDS-staged and multi-line N_r families reproduce the
-    single-job path with one-line windows read through the L1, bit for bit."""
+    """The engine's multi-line N_r families reproduce the single-job path with
+    one-line windows read through the L1, bit for bit; its LDS-staged family
+    (which sums in its own order) reproduces the single-job LDS kernel."""
     sim, target, max_steps = _sim("svi_sphere_k2")
     assert sim.conv.staging == "l1" and sim.conv.lines == 1
+    if shape == "lds":
+        sim.conv.staging = "lds"
     r = sim.run(target_fraction=target, max_steps=max_steps)
     name = "svi_sphere_k2_" + shape
     got = _engine_run([name], graphs=True)[name]

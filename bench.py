@@ -262,6 +262,9 @@ def main(argv=None):
     n_out, n_in, tiles = nr_read_bytes(sim)
     nr_ms = stage_ms.get("nr_sum", 0.0) / max(launches.get("nr_sum", 1), 1)
     alg_bytes = 4.0 * (n_out + n_in)
+    # FP32 rate in ball taps (2 flops per nonzero weight and output, the dense
+    # convolution's count); the kernels execute fewer FMAs than that, since
+    # the +dx and -dx windows share their columns' taps
     taps = float((sim.conv.weights > 0).sum())
     flops = 2.0 * taps * n_out
     achieved = alg_bytes / (nr_ms * 1e-3) / 1e9 if nr_ms > 0 else 0.0
@@ -283,8 +286,8 @@ def main(argv=None):
                 "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)" if traffic else None,
                 "kernel": kname, "avg_ms": round(nr_ms, 4),
                 "alg_bytes_per_launch": alg_bytes,
-                "valu_tflops": round(flops / (nr_ms * 1e-3) / 1e12, 2) if nr_ms > 0 else 0.0,
-                "valu_frac_packed_peak": round(flops / (nr_ms * 1e-3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)
+                "tap_tflops": round(flops / (nr_ms * 1e-3) / 1e12, 2) if nr_ms > 0 else 0.0,
+                "tap_frac_fp32_peak": round(flops / (nr_ms * 1e-3) / 1e12 / FP32_VALU_PEAK_TFLOPS, 4)
                 if nr_ms > 0 else 0.0,
                 "outputs_per_launch": n_out, "tiles": tiles}
     cpu = None

@@ -365,8 +365,10 @@ __device__ __forceinline__ ReinitGeo reinit_geometry(const float* __restrict__ f
     float d_p1 = p2 - 2.0f * p1 + c;
     bool cp = (c * p1) < 0.0f;
     bool cm = (c * m1) < 0.0f;
-    float xp = interface_distance(c, p1, minmod(d_0, d_p1), h);
-    float xm = interface_distance(c, m1, minmod(d_0, d_m1), h);
+    // a distance is read only where its crossing flag is set (most work
+    // nodes have none): the root is solved only there
+    float xp = cp ? interface_distance(c, p1, minmod(d_0, d_p1), h) : h;
+    float xm = cm ? interface_distance(c, m1, minmod(d_0, d_m1), h) : h;
     if (cp) hmin = fminf(hmin, xp);
     if (cm) hmin = fminf(hmin, xm);
     flags |= (cp ? 1u : 0u) << (2 * a);

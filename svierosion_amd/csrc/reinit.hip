@@ -104,13 +104,20 @@ __global__ void k_bits_dilate_xy(const uint32_t* __restrict__ in, uint32_t* __re
   out[w] = acc & valid_bits(wi, nz);
 }
 
-// Set bits per compaction chunk (kCompactChunk words).
+// The bit lists are compacted in chunks of kBitsChunk words: a few hundred
+// words per block, so even a res-400 grid (1.2M words) spreads over thousands
+// of blocks (chunks of kCompactChunk words gave 284 blocks doing 16 serial
+// rounds each).
+constexpr int kBitsRounds = 2;
+constexpr int64_t kBitsChunk = (int64_t)kCompactThreads * kBitsRounds;
+
+// Set bits per compaction chunk (kBitsChunk words).
 __global__ void __launch_bounds__(kCompactThreads) k_bits_count(const uint32_t* __restrict__ m, int64_t nw,
                                                                 int64_t* chunk_counts) {
   __shared__ double red[4];
-  const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
+  const int64_t base = (int64_t)blockIdx.x * kBitsChunk;
   int64_t cnt = 0;
-  for (int r = 0; r < kCompactPerThread; ++r) {
+  for (int r = 0; r < kBitsRounds; ++r) {
     const int64_t w = base + (int64_t)r * kCompactThreads + threadIdx.x;
     if (w < nw) cnt += __popc(m[w]);
   }
@@ -141,26 +148,37 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds_waves, int* 
 }
 
 // Node indices of the set bits, in node order, at the scanned chunk offsets.
+// Per round the block scans its words' bit counts; then each wave expands its
+// non-empty words two at a time, lanes 0-31 one word's bits and lanes 32-63
+// the other's, so the stores of a word's nodes are consecutive (one lane per
+// set bit, its rank by a popcount of the lower bits).
 __global__ void __launch_bounds__(kCompactThreads) k_bits_scatter_list(const uint32_t* __restrict__ m, int64_t nw,
                                                                        int wpl, int nz, const int64_t* offs,
                                                                        int32_t* __restrict__ out, int64_t cap) {
   __shared__ int wave_tot[kCompactThreads / 64];
-  const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
+  const int64_t base = (int64_t)blockIdx.x * kBitsChunk;
+  const int lane = threadIdx.x & 63, wave0 = threadIdx.x & ~63;
+  const int bit = lane & 31;
   int64_t pos = offs[blockIdx.x];
-  for (int r = 0; r < kCompactPerThread; ++r) {
-    const int64_t w = base + (int64_t)r * kCompactThreads + threadIdx.x;
-    uint32_t bits = w < nw ? m[w] : 0u;
+  for (int r = 0; r < kBitsRounds; ++r) {
+    const int64_t w0 = base + (int64_t)r * kCompactThreads + wave0;  // this wave's first word
+    const int64_t w = w0 + lane;
+    const uint32_t bits = w < nw ? m[w] : 0u;
     int total;
     const int excl = block_exclusive_scan(__popc(bits), wave_tot, &total);
-    if (bits) {
-      const int64_t line = w / wpl;
-      const int64_t node0 = line * nz + (w - line * wpl) * 32;
-      int64_t o = pos + excl;
-      while (bits) {
-        const int b = __ffs(bits) - 1;
-        if (o < cap) out[o] = (int32_t)(node0 + b);
-        ++o;
-        bits &= bits - 1u;
+    const unsigned long long nzm = __ballot(bits != 0u);
+    uint32_t pairs = (uint32_t)nzm | (uint32_t)(nzm >> 32);
+    while (pairs) {  // wave-uniform
+      const int j = __ffs(pairs) - 1;
+      pairs &= pairs - 1u;
+      const int src = j + (lane & 32);
+      const uint32_t b = (uint32_t)__shfl((int)bits, src, 64);
+      const int e = __shfl(excl, src, 64);
+      if ((b >> bit) & 1u) {
+        const int64_t ws = w0 + src;
+        const int64_t line = ws / wpl;
+        const int64_t o = pos + e + __popc(b & ((1u << bit) - 1u));
+        if (o < cap) out[o] = (int32_t)(line * nz + (ws - line * wpl) * 32 + bit);
       }
     }
     pos += total;
@@ -170,7 +188,7 @@ __global__ void __launch_bounds__(kCompactThreads) k_bits_scatter_list(const uin
 // Order-preserving list of the set bits of m (synchronises, returns the count).
 static int compact_bits(hipStream_t st, const uint32_t* m, int64_t nw, int wpl, int nz, int32_t* out, int64_t cap,
                         Workspace& ws, int64_t* count) {
-  const int64_t chunks = (nw + kCompactChunk - 1) / kCompactChunk;
+  const int64_t chunks = (nw + kBitsChunk - 1) / kBitsChunk;
   SVIMX_REQUIRE(chunks + 1 <= ws.max_chunks, SVIMX_EBADARG, "workspace too small for compaction");
   if (chunks == 0) {
     *count = 0;
@@ -202,12 +220,6 @@ __global__ void __launch_bounds__(kBandThreads)
     c += phi[list[i]] < 0.0f ? 1.0 : 0.0;
   c = block_reduce<0>(c, red);
   if (threadIdx.x == 0) partials[slot * gridDim.x + blockIdx.x] = c;
-}
-
-__global__ void k_copy_list(const float* __restrict__ src, float* __restrict__ dst, const int32_t* __restrict__ idx,
-                            int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[idx[i]] = src[idx[i]];
 }
 
 // info word: bits 0-5 crossing flags, bits 6-7 sign code.  Nodes with a
@@ -393,8 +405,6 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   if (nw > 0) {
     const int gwb = (int)((blocks_for(nw, 256 * kSweepNodes) + kXcds - 1) / kXcds * kXcds);
     const dim3 gw(blocks_for(nw, 256));
-    hipLaunchKernelGGL(k_copy_list, gw, tb, 0, st, phi, scratch, work_idx, nw);
-    SVIMX_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_reinit_geo, gw, tb, 0, st, phi, work_idx, nw, d, (float)h, (float)cfl,
                        reinterpret_cast<uint8_t*>(work_info), geo);
     SVIMX_CHECK_LAUNCH();

@@ -102,11 +102,13 @@ class ErosionSimulation:
         self.timers = {"stats_sync": 0.0, "volume_sync": 0.0}
         # name -> [(start, end) torch.cuda.Event pairs]; None disables event recording
         self.kernel_events = None
+        # stage names to record (None: every stage); each span costs two event records
+        self.kernel_event_names = None
 
     @contextlib.contextmanager
     def _span(self, name: str):
         """Bracket the launches of one stage with HIP events on the current stream."""
-        if self.kernel_events is None:
+        if self.kernel_events is None or (self.kernel_event_names is not None and name not in self.kernel_event_names):
             yield
             return
         a = torch.cuda.Event(enable_timing=True)

@@ -6,7 +6,10 @@
 
 namespace svimx {
 
-constexpr int kFftTileElems = 2048;  // (L + 1) * lines per tile; lines longer than 2047 are not supported
+constexpr int kFftTileElems = 7168;  // LDS slots per line tile (56 KB): lines * fft_pitch(L)
+constexpr int kFftThreads = 1024;    // threads per line tile
+// butterflies per thread in a radix-R stage of a full tile
+__host__ __device__ constexpr int fft_kb(int R) { return (kFftTileElems / R + kFftThreads - 1) / kFftThreads; }
 
 struct FftPlan {
   Dims d{0, 0, 0};

@@ -12,8 +12,9 @@
 // over one complex buffer (in place per line tile).
 //
 // Line transform: mixed-radix Stockham autosort (radices 4, 2, 3, 5, 7; every
-// grid extent is a product of 2, 3, 5, 7 by construction) on a tile of T lines
-// held in LDS (two ping-pong buffers, row pitch L+1 against bank conflicts).
+// grid extent is a product of 2, 3, 5, 7 by construction) on a tile of T <= 16
+// lines held in one LDS buffer (row pitch L+1 against bank conflicts), each
+// stage transformed in registers between two block barriers.
 // Stage with radix r after m points: butterfly j of L/r (k = j mod m) reads
 // x_q = A[j + q L/r] w^(qk / (m r)), forms the r-point DFT y_p, and writes
 // B[(j / m) m r + k + p m].  Lines along y and x are tiled with consecutive z
@@ -62,13 +63,63 @@ __device__ __forceinline__ int64_t line_offset(const FftPass& P, int64_t l, int 
   return l + (int64_t)t * d.sx();
 }
 
-__global__ void __launch_bounds__(256) k_fft_lines(FftPass P, float2* __restrict__ data,
-                                                   const float* __restrict__ occ, const float* __restrict__ khat,
-                                                   float* __restrict__ out, float* __restrict__ khat_out) {
+// One radix-R Stockham stage over the tile in place: every thread reads and
+// transforms its butterflies into registers (at most fft_kb(R) of them), the
+// block barrier orders all reads before any write, then the outputs go back
+// to the same LDS buffer.  One buffer instead of two ping-pong halves lets a
+// tile hold 16 lines, so the y and x passes move 128-byte row segments.
+__host__ __device__ constexpr int fft_pitch(int L) { return L + 1; }  // row pitch of a line in LDS
+
+template <int R>
+__device__ __forceinline__ void fft_stage(float2* A, const float2* __restrict__ tw, int L, int tl, int m, int pitch) {
+  constexpr int KB = fft_kb(R);
+  const int nb = L / R, mr = m * R, twstep = L / mr, nbt = tl * nb;
+  float2 y[KB][R];
+#pragma unroll
+  for (int i = 0; i < KB; ++i) {
+    const int b = threadIdx.x + i * kFftThreads;
+    if (b < nbt) {
+      const int l = b / nb, j = b - l * nb, k = j % m;
+      float2 x[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const float2 a = A[l * pitch + j + q * nb];
+        x[q] = q ? cmul(a, tw[q * k * twstep]) : a;  // q k twstep < L
+      }
+#pragma unroll
+      for (int p = 0; p < R; ++p) {
+        float2 acc = x[0];
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          const float2 w = c_root[R][(p * q) % R];
+          acc.x += x[q].x * w.x - x[q].y * w.y;
+          acc.y += x[q].x * w.y + x[q].y * w.x;
+        }
+        y[i][p] = acc;
+      }
+    }
+  }
+  __syncthreads();  // every read of A is done before any write
+#pragma unroll
+  for (int i = 0; i < KB; ++i) {
+    const int b = threadIdx.x + i * kFftThreads;
+    if (b < nbt) {
+      const int l = b / nb, j = b - l * nb, k = j % m;
+      const int base = (j / m) * mr + k;
+#pragma unroll
+      for (int p = 0; p < R; ++p) A[l * pitch + base + p * m] = y[i][p];
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_fft_lines(FftPass P, float2* __restrict__ data,
+                                                           const float* __restrict__ occ,
+                                                           const float* __restrict__ khat, float* __restrict__ out,
+                                                           float* __restrict__ khat_out) {
   extern __shared__ float2 lds[];
-  const int L = P.L, T = P.T, pitch = L + 1;
+  const int L = P.L, T = P.T, pitch = fft_pitch(L);
   float2* A = lds;
-  float2* B = lds + T * pitch;
   const int64_t l0 = (int64_t)blockIdx.x * T;
   const int tl = (int)min<int64_t>(T, P.nlines - l0);
   const int n = tl * L;
@@ -90,39 +141,15 @@ __global__ void __launch_bounds__(256) k_fft_lines(FftPass P, float2* __restrict
   __syncthreads();
   int m = 1;
   for (int s = 0; s < P.nst; ++s) {
-    const int r = P.radix[s];
-    const int nb = L / r, mr = m * r, twstep = L / mr;
-    for (int b = threadIdx.x; b < tl * nb; b += blockDim.x) {
-      const int l = b / nb, j = b - l * nb, k = j % m;
-      float2 x[7];
-#pragma unroll
-      for (int q = 0; q < 7; ++q) {
-        if (q < r) {
-          const float2 a = A[l * pitch + j + q * nb];
-          x[q] = q ? cmul(a, P.tw[(q * k * twstep) % L]) : a;
-        }
-      }
-      const int base = l * pitch + (j / m) * mr + k;
-#pragma unroll
-      for (int p = 0; p < 7; ++p) {
-        if (p < r) {
-          float2 y = x[0];
-#pragma unroll
-          for (int q = 1; q < 7; ++q)
-            if (q < r) {
-              const float2 w = c_root[r][(p * q) % r];
-              y.x += x[q].x * w.x - x[q].y * w.y;
-              y.y += x[q].x * w.y + x[q].y * w.x;
-            }
-          B[base + p * m] = y;
-        }
-      }
+    const int r = P.radix[s];  // uniform
+    switch (r) {
+      case 2: fft_stage<2>(A, P.tw, L, tl, m, pitch); break;
+      case 3: fft_stage<3>(A, P.tw, L, tl, m, pitch); break;
+      case 4: fft_stage<4>(A, P.tw, L, tl, m, pitch); break;
+      case 5: fft_stage<5>(A, P.tw, L, tl, m, pitch); break;
+      default: fft_stage<7>(A, P.tw, L, tl, m, pitch); break;
     }
-    __syncthreads();
-    float2* tmp = A;
-    A = B;
-    B = tmp;
-    m = mr;
+    m *= r;
   }
   // store with the pass epilogue
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
@@ -189,7 +216,7 @@ static FftPass make_pass(const FftPlan& F, int axis, int in_mode, int out_mode, 
   P.axis = axis;
   P.L = F.len[axis];
   const int L = P.L;
-  P.T = std::max(1, std::min(16, (kFftTileElems / (L + 1))));
+  P.T = std::max(1, std::min(16, (kFftTileElems / fft_pitch(L))));
   const std::vector<int> r = fft_radices(L);
   P.nst = (int)r.size();
   for (int s = 0; s < P.nst; ++s) P.radix[s] = r[s];
@@ -207,8 +234,8 @@ static FftPass make_pass(const FftPlan& F, int axis, int in_mode, int out_mode, 
 static int run_pass(hipStream_t st, const FftPass& P, float2* data, const float* occ, const float* khat, float* out,
                     float* khat_out) {
   const unsigned grid = (unsigned)((P.nlines + P.T - 1) / P.T);
-  const size_t lds = sizeof(float2) * 2 * (size_t)P.T * (P.L + 1);
-  hipLaunchKernelGGL(k_fft_lines, dim3(grid), dim3(256), lds, st, P, data, occ, khat, out, khat_out);
+  const size_t lds = sizeof(float2) * (size_t)P.T * fft_pitch(P.L);
+  hipLaunchKernelGGL(k_fft_lines, dim3(grid), dim3(kFftThreads), lds, st, P, data, occ, khat, out, khat_out);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
 }
@@ -220,8 +247,9 @@ int fft_plan_create(FftPlan& F, const double* weights, int m, Dims d) {
   F.len[1] = d.ny;
   F.len[2] = d.nz;
   for (int a = 0; a < 3; ++a) {
-    SVIMX_REQUIRE(F.len[a] >= 1 && F.len[a] + 1 <= kFftTileElems && !fft_radices(F.len[a]).empty(), SVIMX_EBADARG,
-                  "FFT needs grid extents that are products of 2, 3, 5, 7 and below %d (got %d)", kFftTileElems,
+    SVIMX_REQUIRE(F.len[a] >= 1 && fft_pitch(F.len[a]) <= kFftTileElems && !fft_radices(F.len[a]).empty(),
+                  SVIMX_EBADARG, "FFT needs grid extents that are products of 2, 3, 5, 7 and below %d (got %d)",
+                  kFftTileElems - 1,
                   F.len[a]);
   }
   int rc = upload_roots();

@@ -202,9 +202,23 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
   float qmin_a = INFINITY, qmax_a = -INFINITY, dvmax_a = -INFINITY, dmax_a = -INFINITY;
   double vsum = 0, vsum_a = 0;
   uint32_t capped = 0, nnear = 0, capped_a = 0, nall = 0;
-  for (int64_t q = lb * kBandThreads + threadIdx.x; q < nb; q += G * kBandThreads) {
-    const int64_t n = band[q];
-    Grad g = central(phi, n, d);
+  // Software pipeline over the grid-stride loop: the band index two nodes
+  // ahead and the gradient stencil of the next node are loaded while this
+  // node's closest point and interpolation run (past the end, the loads
+  // repeat an in-bounds node and are not used).
+  const int64_t stride = G * kBandThreads;
+  int64_t q = lb * kBandThreads + threadIdx.x;
+  int64_t n = 0, n1 = 0;
+  Grad gnext{};
+  if (q < nb) {  // band nodes keep their stencils inside the grid
+    n = band[q];
+    n1 = q + stride < nb ? band[q + stride] : n;
+    gnext = central(phi, n, d);
+  }
+  for (; q < nb; q += stride) {
+    const Grad g = gnext;
+    gnext = central(phi, n1, d);
+    const int64_t n2 = q + 2 * stride < nb ? band[q + 2 * stride] : n1;
     const float x = fabsf(g.c);
     const bool near = x < h;
     float v = 0.0f;
@@ -250,6 +264,8 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
     }
     V[q] = v;
     vmax = fmaxf(vmax, v);
+    n = n1;
+    n1 = n2;
   }
   if (!stats) return;
   // One block reduction of all statistics: max/min in float and counts in

@@ -140,7 +140,9 @@ int svimx_rk_stage(void* stream, int stage, const float* phi_in, const float* u0
  * host in FP64) as a per-column table, the tile marks and the tile list.
  * svimx_nr_exec evaluates N_r = 1 - h^3 (w * f)(x) / (4/3 pi r^3) by direct
  * summation at every node of every marked tile and writes it into nr_out
- * (other nodes untouched).  Replaces BallConvolution (svierosion/svi.py:58-93). */
+ * (other nodes untouched).  The weights must be mirror-symmetric in x, y and z
+ * as FP32 values (a ball's are); others are refused with SVIMX_EBADARG.
+ * Replaces BallConvolution (svierosion/svi.py:58-93). */
 int svimx_nr_plan_create(const double* weights, int m, double h, double radius, int nx, int ny, int nz,
                          void** plan);
 int svimx_nr_plan_destroy(void* plan);

@@ -212,6 +212,19 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   SVIMX_REQUIRE(weights && plan_out && m >= 0 && h > 0 && radius > 0, SVIMX_EBADARG, "bad N_r plan arguments");
   SVIMX_REQUIRE(2 * m + 1 <= nx && 2 * m + 1 <= ny && 2 * m + 1 <= nz, SVIMX_EBADARG,
                 "ball kernel larger than the grid");
+  {
+    // every evaluation path relies on the ball's mirror symmetry in x, y and z
+    // (paired +-dx windows, symmetric z taps, a real kernel spectrum)
+    const int W = 2 * m + 1;
+    auto wf = [&](int a, int b, int c) { return (float)weights[((int64_t)a * W + b) * W + c]; };
+    bool sym = true;
+    for (int a = 0; a < W && sym; ++a)
+      for (int b = 0; b < W && sym; ++b)
+        for (int c = 0; c < W && sym; ++c)
+          sym = wf(a, b, c) == wf(W - 1 - a, b, c) && wf(a, b, c) == wf(a, W - 1 - b, c) &&
+                wf(a, b, c) == wf(a, b, W - 1 - c);
+    SVIMX_REQUIRE(sym, SVIMX_EBADARG, "N_r weights must be mirror-symmetric in x, y and z (as FP32 values)");
+  }
   const bool run_mode = m <= kRunMaxM;
   // balls wider than the LDS plane of the tile kernel (m > 33) go to the FFT
   const bool use_fft = !run_mode && sizeof(float) * 2 * (size_t)(kTY + 2 * m) * nr_pitch(m) > 64 * 1024;

@@ -36,6 +36,28 @@ def test_error_mapping_without_gpu():
     assert "workspace" in _abi.lib.svimx_last_error().decode()
 
 
+def test_nr_plan_refuses_asymmetric_weights_without_gpu():
+    """The N_r paths pair the windows of +dx and -dx (and rely on symmetric z
+    taps and a real spectrum): a weight table that is not mirror-symmetric is
+    refused on the host, before any device call; a ball's table passes that
+    check (the plan then needs the GPU)."""
+    import numpy as np
+    from svierosion_amd import _abi
+    from oracle.flow import ball_cell_weights
+    w = ball_cell_weights(0.3, 0.05)
+    m = (w.shape[0] - 1) // 2
+    assert np.array_equal(w.astype(np.float32), w[::-1].astype(np.float32))
+    assert np.array_equal(w.astype(np.float32), w[:, ::-1].astype(np.float32))
+    assert np.array_equal(w.astype(np.float32), w[:, :, ::-1].astype(np.float32))
+    bad = np.ascontiguousarray(w.copy())
+    bad[m + 1, m, m] += 0.25  # breaks the x mirror only
+    h = ctypes.c_void_p()
+    rc = _abi.lib.svimx_nr_plan_create(bad.ctypes.data_as(ctypes.c_void_p), m, 0.05, 0.3, 40, 40, 40,
+                                       ctypes.byref(h))
+    assert rc == _abi.EBADARG and not h.value
+    assert "mirror-symmetric" in _abi.lib.svimx_last_error().decode()
+
+
 def test_ctypes_structs_match_the_header_layout(tmp_path):
     """ctypes mirrors of the header's structs have the C compiler's offsets and sizes."""
     import shutil

@@ -165,6 +165,48 @@ def served_leg(n_req: int, rate: float, workers: int):
             "identical_histories": bool(same), "peak_device_gb": round(rv.peak_device_bytes / 1e9, 3)}
 
 
+def full_erosion(args) -> int:
+    """The public erode() end to end (SDF, evolution to the target volume with
+    its snapshots, meshing) on one of the single-job workloads: one JSON line
+    with the phase times and ms per accepted step (evolution seconds / steps,
+    reinitialisation and secant retries included, like
+    benchmarks/performance.py:57)."""
+    import torch
+    from svierosion_amd import shapes
+    from svierosion_amd.api import ErosionParams, erode
+    if args.full_erosion == "cube64":
+        mesh = shapes.cube(1.0)
+        params = ErosionParams(law="svi", radius=0.1, k=1, s=1, t=0.93, resolution=64, target_fraction=0.9)
+        name = "cube of side 1 (12 triangles), SVI k=1 s=1 t=0.93, r=0.1, res 64, to 0.9"
+    elif args.full_erosion == "fragment256":
+        mesh = shapes.sh_fragment(level=6, radius=50.0, seed=args.seed)
+        params = ErosionParams(law="svi", radius=2.0, k=1, s=1, t=0.93, resolution=256, target_fraction=0.9)
+        name = "icosphere(6) + 16 SH terms (81,920 triangles), SVI k=1 s=1 t=0.93, r=2.0, res 256, to 0.9"
+    else:
+        mesh = shapes.noisy_ellipsoid(level=args.level, axes=(60.0, 45.0, 30.0),
+                                      amplitudes=(3.0, 1.5, 0.75, 0.375), base_frequency=2.0, seed=args.seed)
+        params = ErosionParams(law="svi", radius=2.0, k=2, s=1, t=0.95, resolution=args.resolution,
+                               target_fraction=0.9, snapshots=(0.97, 0.94))
+        name = (f"{len(mesh.faces)}-triangle noisy ellipsoid scan, SVI k=2 s=1 t=0.95 r=2.0, res "
+                f"{args.resolution}, to 0.9 with snapshots 0.97/0.94")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = erode(mesh, params)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    t = res.timings
+    steps = len(res.history)
+    line = {"full_erosion": args.full_erosion, "workload": name, "triangles": int(len(mesh.faces)),
+            "grid": list(res.grid.shape), "stop_reason": res.stop_reason, "steps": steps,
+            "reinits": sum(1 for r in res.history if r.reinit), "final_fraction": round(res.fraction, 7),
+            "snapshots": [round(f, 4) for f, _, _ in res.snapshots],
+            "sdf_s": round(t.get("sdf", 0.0), 3), "evolve_s": round(t.get("evolve", 0.0), 3),
+            "mesh_s": round(t.get("mesh", 0.0), 3), "total_s": round(total, 3),
+            "ms_per_step": round(1000 * t.get("evolve", 0.0) / max(steps, 1), 3)}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def nr_shape_sweep(sim) -> int:
     """Time each N_r summation shape the plan can take on the current field:
     one JSON line per shape (for per-shape profiler counters)."""
@@ -215,6 +257,8 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=2, help="set-up / meshing threads of the batched forms")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of this many MiB (profiler byte calibration)")
+    ap.add_argument("--full-erosion", choices=["scan400", "fragment256", "cube64"], default=None,
+                    help="diagnostic: run erode() to the workload's target volume end to end and exit")
     ap.add_argument("--nr-shape-sweep", action="store_true",
                     help="diagnostic: time every N_r summation shape on the workload's field "
                          "(5 launches each, in the autotuner's order) and exit")
@@ -225,6 +269,8 @@ def main(argv=None):
         return 2
     import torch
     torch.cuda.set_device(0)
+    if args.full_erosion:
+        return full_erosion(args)
     if args.copy_calibration:
         # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
         # calibrate the HBM byte counts read for the workload's kernels

@@ -148,6 +148,11 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
 int svimx_nr_plan_destroy(void* plan);
 int svimx_nr_reset(void* stream, void* plan);
 int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t n);
+/* Mark the tricubic stencils of n index-space points (n x 3 float32, device):
+ * after select + sum, N_r is valid at every node an interpolation at those
+ * points reads (axis-clamped).  Replaces the stencil-node lists the reference
+ * evaluates at (svierosion/evolve.py:126-127, svierosion/api.py:220-223). */
+int svimx_nr_mark_points(void* stream, void* plan, const float* points, int64_t n);
 /* Evaluation method of a plan: 1 direct summation (at the marked nodes), 2
  * FFT over the whole grid (the reference's circular convolution,
  * svierosion/svi.py:84-93: forward 3-D FFT of f, product with the real
@@ -263,6 +268,18 @@ int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, i
                           double max_shift, float** xyz, int64_t* nv, int32_t** faces, int64_t* nf);
 int svimx_free(void* device_ptr);
 int svimx_copy_to_host(void* dst, const void* device_src, int64_t bytes);
+
+/* ---------------------------------------------------------------- mesh measures (host)
+ * vertices: nv x 3 FP64, faces: nf x 3 int64 vertex indices (row-major).
+ * Host-only: no device call, usable without a GPU. */
+/* *volume = sum over faces of (p0 - c) . ((p1 - c) x (p2 - c)) / 6 with c the
+ * vertex centroid: positive for an outward-oriented closed mesh.  Replaces
+ * TriMesh volume (svierosion/meshio.py:80-85). */
+int svimx_mesh_signed_volume(const double* vertices, int64_t nv, const int64_t* faces, int64_t nf, double* volume);
+/* out[4] = {edges, boundary edges (1 face), non-manifold edges (> 2 faces),
+ * misoriented edges (2 faces, same direction)}.  Replaces the edge counts of
+ * svierosion/meshio.py:88-105. */
+int svimx_mesh_edge_topology(const int64_t* faces, int64_t nf, int64_t nv, int64_t* out);
 
 /* ---------------------------------------------------------------- operators
  * Node-list forms of the reference's @kernel functions (svierosion/kernels.py),
@@ -389,11 +406,11 @@ int svimx_engine_remove(void* engine, int job_id);
 int svimx_engine_use_graphs(void* engine, int on);
 /* Ids (up to cap) of jobs that stopped since the last call, oldest first. */
 int svimx_engine_stopped(void* engine, int* ids, int cap, int* n);
-/* Milliseconds spent in ticks / secant retries / reinitialisations so far. */
 /* Ticks whose graph capture was invalidated (by a device-wide operation in
  * another thread, e.g. a synchronous free) and that ran their launches eagerly
  * instead; results are unaffected. */
 int64_t svimx_engine_capture_fallbacks(void* engine);
+/* Milliseconds spent in ticks / secant retries / reinitialisations so far. */
 int svimx_engine_timers(void* engine, double* out3);
 
 #ifdef __cplusplus

@@ -83,6 +83,9 @@ _SIGNATURES = {
     "svimx_nr_plan_destroy": (_I, [_P]),
     "svimx_nr_reset": (_I, [_P, _P]),
     "svimx_nr_mark_nodes": (_I, [_P, _P, _P, _I64]),
+    "svimx_nr_mark_points": (_I, [_P, _P, _P, _I64]),
+    "svimx_mesh_signed_volume": (_I, [_P, _I64, _P, _I64, C.POINTER(_D)]),
+    "svimx_mesh_edge_topology": (_I, [_P, _I64, _I64, C.POINTER(_I64)]),
     "svimx_nr_exec": (_I, [_P, _P, _P, _P, _P, _I64]),
     "svimx_nr_select": (_I, [_P, _P]),
     "svimx_nr_sum": (_I, [_P, _P, _P, _P, _P, _I64]),

@@ -135,16 +135,6 @@ def point_mesh_distance(points: np.ndarray, mesh: TriMesh, max_edge: float | Non
     return d.double().cpu().numpy() * grid.spacing
 
 
-def _stencil_nodes(p: torch.Tensor, shape) -> torch.Tensor:
-    """Flat indices of the 4^3 tricubic stencils of index points p."""
-    base = torch.floor(p).long() - 1
-    offs = torch.stack(torch.meshgrid(*(torch.arange(4, device=p.device),) * 3, indexing="ij"), -1).reshape(-1, 3)
-    q = (base[:, None, :] + offs[None]).reshape(-1, 3)
-    for a in range(3):
-        q[:, a].clamp_(0, shape[a] - 1)
-    return torch.unique(q[:, 0] * (shape[1] * shape[2]) + q[:, 1] * shape[2] + q[:, 2])
-
-
 def _annotate(sim: ErosionSimulation, mesh: TriMesh, initial: TriMesh, law: SpeedLaw,
               triangles: dict | None = None) -> None:
     """Per-vertex depth, N_r or curvatures, and speed (svierosion/api.py:171-194)."""
@@ -155,7 +145,7 @@ def _annotate(sim: ErosionSimulation, mesh: TriMesh, initial: TriMesh, law: Spee
                                                     triangles=triangles).astype(np.float32)
     if law.requires_svi:
         p = torch.as_tensor(grid.to_index(mesh.vertices), dtype=torch.float32, device=sim.ls.device)
-        N = sim.svi_field(_stencil_nodes(p, grid.shape))
+        N = sim.svi_field_at(p)
         Nv = tricubic(N.reshape(-1), grid.shape, p).double().cpu().numpy()
         mesh.vertex_data["N"] = Nv.astype(np.float32)
         mesh.vertex_data["speed"] = law.speed(torch.as_tensor(Nv)).numpy().astype(np.float32)
@@ -179,7 +169,7 @@ def initial_max_nr(sim: ErosionSimulation) -> float:
     cp = ls.closest_points(ls.interface_nodes())
     if cp.shape[0] == 0:
         return 0.0
-    N = sim.svi_field(_stencil_nodes(cp, ls.dims))
+    N = sim.svi_field_at(cp)
     return float(tricubic(N.reshape(-1), ls.dims, cp).max())
 
 

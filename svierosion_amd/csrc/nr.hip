@@ -57,6 +57,28 @@ __global__ void k_nr_mark_nodes(uint8_t* marks, int run_mode, int nty, int ntz, 
     marks[((int64_t)(i / kTX) * nty + j / kTY) * ntz + k / kTZ] = 1;
 }
 
+// Marks of the tricubic stencils of index-space points: base b = floor(p) - 1
+// clamped to [0, n - 4] per axis, so the stencil [b, b + 3] covers every node
+// an axis-clamped interpolation reads (selection dilates the run marks in x
+// and y by the stencil, as for the closest points of the fractions pass).
+__global__ void k_nr_mark_points(uint8_t* marks, int run_mode, int nty, int ntz, int nzr, Dims d,
+                                 const float* __restrict__ p, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int dims[3] = {d.nx, d.ny, d.nz};
+  int b[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float v = p[3 * q + a];
+    const int f = v == v ? (int)fminf(fmaxf(floorf(v), -2.0f), (float)dims[a]) : 0;  // NaN -> 0
+    b[a] = min(max(f - 1, 0), max(dims[a] - 4, 0));
+  }
+  if (run_mode)
+    mark_stencil_runs(marks, d.ny, nzr, run_mode, b[0], b[1], b[2]);
+  else
+    mark_stencil_tiles(marks, nty, ntz, b[0], b[1], b[2]);
+}
+
 __global__ void k_nr_unit_need(const uint8_t* __restrict__ marks, uint8_t* __restrict__ need, int ny, int nzr,
                                int nyu, int Y, int64_t nunits) {
   unit_need_range(marks, need, ny, nzr, nyu, Y, nunits, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
@@ -404,6 +426,16 @@ int svimx_nr_mark_nodes(void* stream, void* plan, const int32_t* nodes, int64_t 
   if (n <= 0 || p->fft) return SVIMX_OK;
   hipLaunchKernelGGL(k_nr_mark_nodes, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), p->marks,
                      p->run_mode, p->nty, p->ntz, p->nzr, p->d, nodes, n);
+  SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+}
+
+int svimx_nr_mark_points(void* stream, void* plan, const float* points, int64_t n) {
+  NrPlan* p = static_cast<NrPlan*>(plan);
+  SVIMX_REQUIRE(p && (points || n <= 0), SVIMX_EBADARG, "null argument");
+  if (n <= 0 || p->fft) return SVIMX_OK;
+  hipLaunchKernelGGL(k_nr_mark_points, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), p->marks,
+                     p->run_mode, p->nty, p->ntz, p->nzr, p->d, points, n);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
 }

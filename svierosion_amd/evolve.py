@@ -255,6 +255,11 @@ class ErosionSimulation:
         """Dense N_r of the current shape (NaN outside the evaluated tiles when ``nodes`` is given)."""
         return self.conv.exterior_fraction(self.ls.dense_fractions(), nodes)
 
+    def svi_field_at(self, points: torch.Tensor) -> torch.Tensor:
+        """Dense N_r of the current shape, valid wherever a tricubic interpolation
+        at the index-space ``points`` reads (NaN where nothing was evaluated)."""
+        return self.conv.exterior_fraction_at_points(self.ls.dense_fractions(), points)
+
     # ---------------------------------------------------------------- run
     def run(self, target_fraction: float | None = 0.9, max_steps: int = 100000, max_time: float | None = None,
             tol: float = 1e-6, callback=None) -> RunResult:

@@ -46,28 +46,34 @@ class TriMesh:
         return 0.5 * float(np.linalg.norm(self.face_normals(False), axis=1).sum())
 
     def signed_volume(self) -> float:
-        """Enclosed volume, positive for outward orientation (svierosion/meshio.py:80-85)."""
-        t = self.triangles() - self.vertices.mean(0)
-        return float(np.einsum("ij,ij->i", t[:, 0], np.cross(t[:, 1], t[:, 2])).sum() / 6.0)
+        """Enclosed volume, positive for outward orientation (svierosion/meshio.py:80-85):
+        sum of (p0 - c) . ((p1 - c) x (p2 - c)) / 6 about the vertex centroid c,
+        in FP64 by the library's host loop (svimx_mesh_signed_volume)."""
+        import ctypes as C
+        from . import _abi
+        v = np.ascontiguousarray(self.vertices, dtype=np.float64)
+        f = np.ascontiguousarray(self.faces, dtype=np.int64)
+        out = C.c_double()
+        _abi.check(_abi.lib.svimx_mesh_signed_volume(v.ctypes.data_as(C.c_void_p), len(v), f.ctypes.data_as(C.c_void_p),
+                                                     len(f), C.byref(out)), "mesh volume")
+        return float(out.value)
 
     def volume(self) -> float:
         return abs(self.signed_volume())
 
     def edge_topology(self) -> dict:
-        """Edge counts and orientation consistency (svierosion/meshio.py:88-105)."""
-        f = self.faces
-        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
-        lo = np.minimum(e[:, 0], e[:, 1])
-        hi = np.maximum(e[:, 0], e[:, 1])
-        n = max(int(self.vertices.shape[0]), 1)
-        uniq, inv, counts = np.unique(lo * n + hi, return_inverse=True, return_counts=True)
-        inv = inv.reshape(-1)
-        orient = np.where(e[:, 0] < e[:, 1], 1, -1)
-        balance = np.bincount(inv, weights=orient, minlength=len(uniq))
-        two = counts == 2
-        return {"edges": int(len(uniq)), "boundary_edges": int((counts == 1).sum()),
-                "nonmanifold_edges": int((counts > 2).sum()),
-                "misoriented_edges": int((two & (balance != 0)).sum())}
+        """Edge counts and orientation consistency (svierosion/meshio.py:88-105): edges
+        used by one face (boundary), by more than two (non-manifold), and by two
+        faces traversing them in the same direction (misoriented); counted by the
+        library's host loop (svimx_mesh_edge_topology)."""
+        import ctypes as C
+        from . import _abi
+        f = np.ascontiguousarray(self.faces, dtype=np.int64)
+        out = (C.c_int64 * 4)()
+        _abi.check(_abi.lib.svimx_mesh_edge_topology(f.ctypes.data_as(C.c_void_p), len(f), len(self.vertices), out),
+                   "mesh edge topology")
+        return {"edges": int(out[0]), "boundary_edges": int(out[1]), "nonmanifold_edges": int(out[2]),
+                "misoriented_edges": int(out[3])}
 
     def is_watertight(self) -> bool:
         t = self.edge_topology()

@@ -142,6 +142,20 @@ class BallConvolution:
         self.exec_marked(f, out, self._workspace())
         return out
 
+    def exterior_fraction_at_points(self, fractions: torch.Tensor, points: torch.Tensor) -> torch.Tensor:
+        """Dense N_r field, evaluated (at least) at every node that a tricubic
+        interpolation at the index-space ``points`` (n, 3) reads; NaN elsewhere.
+        The stencils are marked on the device, no node list is formed."""
+        f = _abi.occupancy_zeros(self.grid.shape, self.device)
+        _abi.occupancy_view(f, self.grid.shape).copy_(fractions.to(self.device, torch.float32).view(self.grid.shape))
+        out = torch.full(self.grid.shape, float("nan"), dtype=torch.float32, device=self.device)
+        p = points.to(self.device, torch.float32).reshape(-1, 3).contiguous()
+        st = _abi.stream()
+        _abi.check(_abi.lib.svimx_nr_reset(st, self.plan))
+        _abi.check(_abi.lib.svimx_nr_mark_points(st, self.plan, _abi.ptr(p), p.shape[0]))
+        self.exec_marked(f, out, self._workspace())
+        return out
+
     @property
     def run_length(self) -> int:
         """Nodes per z-run of the direct summation (0: tile mode)."""

@@ -148,3 +148,52 @@ def test_result_files_roundtrip():
         assert meta["stop_reason"] == "target_volume" and meta["grid"]["shape"] == [8, 8, 8]
         assert meta["snapshots"] == [{"fraction": 0.97, "time": 0.2}]
         assert abs(meta["final_mesh_volume"] - 1.0) < 1e-9
+
+
+def _np_edge_topology(faces, nv):
+    """Edge counts by sorting (a NumPy restatement of svierosion/meshio.py:88-105)."""
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+    lo, hi = e.min(1), e.max(1)
+    uniq, inv, counts = np.unique(lo * max(nv, 1) + hi, return_inverse=True, return_counts=True)
+    bal = np.bincount(inv.reshape(-1), weights=np.where(e[:, 0] < e[:, 1], 1, -1), minlength=len(uniq))
+    return {"edges": len(uniq), "boundary_edges": int((counts == 1).sum()),
+            "nonmanifold_edges": int((counts > 2).sum()), "misoriented_edges": int(((counts == 2) & (bal != 0)).sum())}
+
+
+def test_mesh_measures_match_numpy():
+    """The library's host loops for the mesh volume and the edge counts agree
+    with NumPy restatements on closed, flipped, holed and non-manifold meshes;
+    the volume also against closed forms (unit cube 1, icosphere -> 4/3 pi)."""
+    from svierosion_amd import shapes
+    from svierosion_amd.meshio import TriMesh
+    cube = shapes.cube(1.0, center=(3.0, -2.0, 5.0))
+    assert abs(cube.signed_volume() - 1.0) < 1e-12
+    assert abs(cube.flipped().signed_volume() + 1.0) < 1e-12
+    ico = shapes.icosphere(5, 1.0)
+    assert abs(ico.volume() - 4.0 / 3.0 * np.pi) < 5e-3  # inscribed polyhedron
+    torus = shapes.torus()
+    rng = np.random.default_rng(3)
+    cases = [cube, ico, torus]
+    f = ico.faces.copy()
+    idx = rng.choice(len(f), 7, replace=False)
+    f[idx] = f[idx][:, ::-1]  # flipped faces
+    cases.append(TriMesh(ico.vertices, f))
+    cases.append(TriMesh(ico.vertices, ico.faces[5:]))  # holes
+    extra = np.array([[ico.faces[0, 0], ico.faces[0, 1], 0]])  # a third face on an edge
+    cases.append(TriMesh(ico.vertices, np.concatenate([ico.faces, extra])))
+    for m in cases:
+        t = m.triangles() - m.vertices.mean(0)
+        want = float(np.einsum("ij,ij->i", t[:, 0], np.cross(t[:, 1], t[:, 2])).sum() / 6.0)
+        assert abs(m.signed_volume() - want) <= 1e-12 * max(1.0, abs(want))
+        assert m.edge_topology() == _np_edge_topology(m.faces, len(m.vertices))
+    assert cases[0].is_watertight() and cases[0].is_consistently_oriented()
+    assert not cases[3].is_consistently_oriented()
+    assert cases[4].edge_topology()["boundary_edges"] > 0
+    assert cases[5].edge_topology()["nonmanifold_edges"] == 1
+    bad = TriMesh(ico.vertices, ico.faces)
+    bad.faces = bad.faces.copy()
+    bad.faces[0, 0] = len(ico.vertices)  # out of range: refused, no crash
+    with pytest.raises(ValueError, match="vertex"):
+        bad.signed_volume()
+    with pytest.raises(ValueError, match="vertex"):
+        bad.edge_topology()

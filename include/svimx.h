@@ -281,6 +281,21 @@ int svimx_mesh_signed_volume(const double* vertices, int64_t nv, const int64_t* 
  * svierosion/meshio.py:88-105. */
 int svimx_mesh_edge_topology(const int64_t* faces, int64_t nf, int64_t nv, int64_t* out);
 
+/* Host arrays of svimx_sdf_build for a mesh whose triangles all have edges
+ * <= max_edge cells (the reference splits longer ones first,
+ * svierosion/sdf.py:54-70): per face t the index coordinates tri_idx[9t..]
+ * ((v - origin) / h, FP64), org = floor of corner 0, the corners relative to
+ * org in FP32 (local), the node box [ceil(min - exact_band), floor(max +
+ * exact_band)] clamped to the grid (box[6t..]: lo xyz, hi xyz), and the
+ * vertex-star CSR (vid: an id per corner shared by bitwise-equal vertices;
+ * star: the triangle of every corner grouped by id in corner order; deg,
+ * first per id; *nvid ids; deg/first need nv entries).  If a longer edge
+ * exists, only *nsplit (the number of such triangles) is written. */
+int svimx_sdf_prepare(const double* vertices, int64_t nv, const int64_t* faces, int64_t nf, const double* origin,
+                      double h, const int32_t* shape, double exact_band, double max_edge, double* tri_idx,
+                      int32_t* org, float* local, int32_t* box, int32_t* vid, int32_t* star, int32_t* deg,
+                      int32_t* first, int64_t* nvid, int64_t* nsplit);
+
 /* ---------------------------------------------------------------- operators
  * Node-list forms of the reference's @kernel functions (svierosion/kernels.py),
  * sharing the device code of the fused stage kernels; used by LevelSet methods

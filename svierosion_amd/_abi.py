@@ -86,6 +86,8 @@ _SIGNATURES = {
     "svimx_nr_mark_points": (_I, [_P, _P, _P, _I64]),
     "svimx_mesh_signed_volume": (_I, [_P, _I64, _P, _I64, C.POINTER(_D)]),
     "svimx_mesh_edge_topology": (_I, [_P, _I64, _I64, C.POINTER(_I64)]),
+    "svimx_sdf_prepare": (_I, [_P, _I64, _P, _I64, _P, _D, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P,
+                               C.POINTER(_I64), C.POINTER(_I64)]),
     "svimx_nr_exec": (_I, [_P, _P, _P, _P, _P, _I64]),
     "svimx_nr_select": (_I, [_P, _P]),
     "svimx_nr_sum": (_I, [_P, _P, _P, _P, _P, _I64]),

@@ -52,7 +52,39 @@ def _vertex_ids(tri: np.ndarray) -> np.ndarray:
     return _point_ids(tri.reshape(-1, 3))[: 3 * len(tri)].reshape(-1, 3)
 
 
+def _prepare_native(mesh: TriMesh, grid: Grid, exact_band: float):
+    """The arrays of _prepare from the library's host loop (svimx_sdf_prepare),
+    or None when some triangle needs splitting."""
+    import ctypes as C
+    v = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+    f = np.ascontiguousarray(mesh.faces, dtype=np.int64)
+    nf, nv = len(f), len(v)
+    tri_idx = np.empty((nf, 3, 3), np.float64)
+    org = np.empty((nf, 3), np.int32)
+    local = np.empty((nf, 3, 3), np.float32)
+    box = np.empty((nf, 6), np.int32)
+    vid = np.empty((nf, 3), np.int32)
+    star = np.empty(3 * nf, np.int32)
+    deg = np.empty(max(nv, 1), np.int32)
+    first = np.empty(max(nv, 1), np.int32)
+    origin = np.ascontiguousarray(grid.origin, dtype=np.float64)
+    shape = np.ascontiguousarray(grid.shape, dtype=np.int32)
+    nvid, nsplit = C.c_int64(), C.c_int64()
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    _abi.check(_abi.lib.svimx_sdf_prepare(ptr(v), nv, ptr(f), nf, ptr(origin), float(grid.spacing), ptr(shape),
+                                          float(exact_band), 8.0, ptr(tri_idx), ptr(org), ptr(local), ptr(box),
+                                          ptr(vid), ptr(star), ptr(deg), ptr(first), C.byref(nvid), C.byref(nsplit)),
+               "SDF preparation")
+    if nsplit.value:
+        return None
+    k = nvid.value
+    return tri_idx, tri_idx, org, local, box, vid, star, deg[:k], first[:k]
+
+
 def _prepare(mesh: TriMesh, grid: Grid, exact_band: float):
+    native = _prepare_native(mesh, grid, exact_band)
+    if native is not None:
+        return native
     h = grid.spacing
     shape = np.asarray(grid.shape)
     tri_idx = (mesh.vertices[mesh.faces] - np.asarray(grid.origin)) / h

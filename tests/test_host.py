@@ -197,3 +197,43 @@ def test_mesh_measures_match_numpy():
         bad.signed_volume()
     with pytest.raises(ValueError, match="vertex"):
         bad.edge_topology()
+
+
+def test_sdf_prepare_native_matches_numpy():
+    """The library's SDF preparation equals the NumPy path for meshes without
+    long edges: index coordinates, frames and boxes exactly; vertex ids as the
+    same grouping (bitwise-equal vertices of a triangle soup share one), and
+    the same star triangles per vertex in corner order.  A mesh with a long
+    edge is left to the NumPy splitting path."""
+    from svierosion_amd import shapes, sdf
+    from svierosion_amd.api import ErosionParams, make_grid
+    from svierosion_amd.meshio import TriMesh
+    ico = shapes.icosphere(4, 1.0)
+    soup = TriMesh(ico.vertices[ico.faces].reshape(-1, 3), np.arange(3 * len(ico.faces)).reshape(-1, 3))
+    for mesh in (ico, soup):
+        g = make_grid(mesh, ErosionParams(law="mean", resolution=48))
+        nat = sdf._prepare_native(mesh, g, 2.0)
+        assert nat is not None
+        # the NumPy path, with the native one bypassed
+        saved = sdf._prepare_native
+        sdf._prepare_native = lambda *a: None
+        try:
+            old = sdf._prepare(mesh, g, 2.0)
+        finally:
+            sdf._prepare_native = saved
+        for a, b in zip(nat[:5], old[:5]):
+            assert np.array_equal(np.asarray(a, np.float64), np.asarray(b, np.float64))
+        vn, vo = nat[5].reshape(-1), old[5].reshape(-1)
+        # same grouping: a bijection between the id sets
+        pairs = set(zip(vn.tolist(), vo.tolist()))
+        assert len(pairs) == len(set(vn.tolist())) == len(set(vo.tolist()))
+        assert len(set(vo.tolist())) == len(ico.vertices)  # the soup's corners regroup into the sphere's vertices
+        m = dict(pairs)
+        for idn in range(len(nat[7])):
+            a = nat[6][nat[8][idn]: nat[8][idn] + nat[7][idn]]
+            ido = m[idn]
+            b = old[6][old[8][ido]: old[8][ido] + old[7][ido]]
+            assert np.array_equal(a, b)
+    tri = shapes.cube(1.0)
+    g = make_grid(tri, ErosionParams(law="mean", resolution=48))
+    assert sdf._prepare_native(tri, g, 2.0) is None  # 48-cell edges: split on the NumPy path

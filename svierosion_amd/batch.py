@@ -146,8 +146,10 @@ class _Worker:
 
 
 def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, graphs: bool = True,
-          keep_state: bool = False, verbose: bool = False, workers: int = 2) -> ServeReport:
-    """Continuous batching of ``requests`` (arrival times in seconds from now)."""
+          keep_state: bool = False, verbose: bool = False, workers: int = 2, solo_tail: bool = True) -> ServeReport:
+    """Continuous batching of ``requests`` (arrival times in seconds from now).  With
+    ``solo_tail`` the last running job finishes on the single-job path once no
+    request is left to admit."""
     import torch
     from .engine import STOP_NAMES, BatchEngine
     _abi.require_device("cuda")
@@ -163,7 +165,7 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
     comps: dict[int, Completion] = {}
     prep: dict = {}      # future -> request index
     fin: dict = {}       # future -> (job id, request index)
-    live: dict = {}      # job id -> (request index, ErosionJob, bytes, evolve start)
+    live: dict = {}      # job id -> (request index, ErosionJob, bytes, evolve start, steps at run start)
     evolve_s: dict = {}  # request index -> seconds in the engine
     used = 0
     peak = 0
@@ -203,16 +205,44 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
                 continue
             jid = eng.add(job.sim)
             eng.run(jid, job.targets[job.next_target], max_steps=job.params.max_steps)
-            live[jid] = (idx, job, comps[idx].device_bytes, clock())
+            live[jid] = (idx, job, comps[idx].device_bytes, clock(), job.sim.steps)
             evolve_s[idx] = 0.0
             n_running += 1
+        # the last running job, with nothing left to admit or mesh, leaves the
+        # engine: a tick of one job costs more than a step of the single-job
+        # path (every job-table kernel loads its tables first), and
+        # ErosionSimulation.run takes the same steps bit for bit
+        if solo_tail and n_running == 1 and not pending and not prep and not fin:
+            jid = next(j for j in live if eng.status(j).running)
+            idx, job, need, t_run, step0 = live.pop(jid)
+            eng.sync(jid)
+            eng.remove(jid)
+            n_running = 0
+            sim, err = job.sim, None
+            try:
+                res = sim.run(target_fraction=job.targets[job.next_target],
+                              max_steps=job.params.max_steps - (sim.steps - step0))
+                evolve_s[idx] += clock() - t_run
+                while finish_target(job, res.stop_reason, res.fraction):
+                    t_run = clock()
+                    res = sim.run(target_fraction=job.targets[job.next_target], max_steps=job.params.max_steps)
+                    evolve_s[idx] += clock() - t_run
+            except Exception as ex:  # as in the meshing worker below
+                err = f"{type(ex).__name__}: {ex}"
+            comps[idx].steps = sim.steps
+            if err is None:
+                complete(idx, result=job_result(job, evolve_s[idx], keep_state))
+            else:
+                complete(idx, error=err)
+            used -= need
+            continue
         # one step of every running job
         if n_running > 0:
             n_running = eng.tick()
             ticks += 1
         # jobs that reached a target: mesh them on the worker
         for jid in eng.stopped():
-            idx, job, need, t_run = live[jid]
+            idx, job, need, t_run, _ = live[jid]
             evolve_s[idx] += clock() - t_run
             st = eng.status(jid)
             if st.stop_reason == ERROR_STOP:
@@ -226,7 +256,7 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
             fin[worker.submit(finish_target, job, STOP_NAMES[st.stop_reason], fraction)] = (jid, idx)
         for fut in [f for f in fin if f.done()]:
             jid, idx = fin.pop(fut)
-            _, job, need, _ = live[jid]
+            _, job, need, _, _ = live[jid]
             try:
                 more = fut.result()
             except Exception as ex:
@@ -235,7 +265,7 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
                 err = None
             if more:
                 eng.run(jid, job.targets[job.next_target], max_steps=job.params.max_steps)
-                live[jid] = (idx, job, need, clock())
+                live[jid] = (idx, job, need, clock(), job.sim.steps)
                 n_running += 1
                 continue
             comps[idx].steps = job.sim.steps

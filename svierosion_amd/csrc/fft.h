@@ -6,8 +6,11 @@
 
 namespace svimx {
 
-constexpr int kFftTileElems = 7168;  // LDS slots per line tile (56 KB): lines * fft_pitch(L)
-constexpr int kFftThreads = 1024;    // threads per line tile
+// LDS slots per line tile (28 KB): lines * fft_pitch(L), 8 lines of 432 points;
+// with 512 threads per tile three tiles share a CU (16 lines of 1024 threads,
+// one tile per CU, were 12% slower)
+constexpr int kFftTileElems = 3584;
+constexpr int kFftThreads = 512;
 // butterflies per thread in a radix-R stage of a full tile
 __host__ __device__ constexpr int fft_kb(int R) { return (kFftTileElems / R + kFftThreads - 1) / kFftThreads; }
 

@@ -13,8 +13,8 @@
 //
 // Line transform: mixed-radix Stockham autosort (radices 4, 2, 3, 5, 7; every
 // grid extent is a product of 2, 3, 5, 7 by construction) on a tile of T <= 16
-// lines held in one LDS buffer (row pitch L+1 against bank conflicts), each
-// stage transformed in registers between two block barriers.
+// lines (8 of 432 points) held in one LDS buffer (row pitch L+1 against bank
+// conflicts), each stage transformed in registers between two block barriers.
 // Stage with radix r after m points: butterfly j of L/r (k = j mod m) reads
 // x_q = A[j + q L/r] w^(qk / (m r)), forms the r-point DFT y_p, and writes
 // B[(j / m) m r + k + p m].  Lines along y and x are tiled with consecutive z
@@ -67,7 +67,8 @@ __device__ __forceinline__ int64_t line_offset(const FftPass& P, int64_t l, int 
 // transforms its butterflies into registers (at most fft_kb(R) of them), the
 // block barrier orders all reads before any write, then the outputs go back
 // to the same LDS buffer.  One buffer instead of two ping-pong halves lets a
-// tile hold 16 lines, so the y and x passes move 128-byte row segments.
+// tile hold twice the lines (the y and x passes move 64-byte row segments at
+// 432 points, 128-byte ones at 224 or less).
 __host__ __device__ constexpr int fft_pitch(int L) { return L + 1; }  // row pitch of a line in LDS
 
 template <int R>

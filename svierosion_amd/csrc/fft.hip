@@ -29,7 +29,6 @@
 
 namespace svimx {
 
-__constant__ float2 c_root[8][8];  // c_root[r][k] = exp(-2 pi i k / r), r in {2, 3, 4, 5, 7}
 
 enum FftIn { kInComplex = 0, kInOccupancy = 1 };
 enum FftOut { kOutComplex = 0, kOutConjMulK = 1, kOutNr = 2, kOutRealK = 3 };
@@ -40,6 +39,8 @@ struct FftPass {
   int T;           // lines per tile
   int nst;         // radix stages
   int radix[16];
+  uint64_t mag_nb[16], mag_m[16];  // floor(2^32 / d) + 1 of L / radix and of the stage's m: exact
+                                   // quotients of numerators below 2^16 by (x * mag) >> 32
   int64_t nlines;  // lines of this axis
   const float2* tw;
   Dims d;
@@ -71,8 +72,57 @@ __device__ __forceinline__ int64_t line_offset(const FftPass& P, int64_t l, int 
 // 432 points, 128-byte ones at 224 or less).
 __host__ __device__ constexpr int fft_pitch(int L) { return L + 1; }  // row pitch of a line in LDS
 
+__device__ __forceinline__ int fft_div(int x, uint64_t mag) { return (int)(((uint64_t)x * mag) >> 32); }
+
+// exp(-2 pi i k / R) as FP32 constants (the FP64 values rounded), so the
+// unrolled butterflies multiply by immediates and drop the exact zero parts
 template <int R>
-__device__ __forceinline__ void fft_stage(float2* A, const float2* __restrict__ tw, int L, int tl, int m, int pitch) {
+__device__ __forceinline__ float2 fft_root(int k) {
+  constexpr double c3 = -0.5, s3 = 0.86602540378443864676;
+  constexpr double c5a = 0.30901699437494742410, s5a = 0.95105651629515357212;
+  constexpr double c5b = -0.80901699437494742410, s5b = 0.58778525229247312917;
+  constexpr double c7a = 0.62348980185873353053, s7a = 0.78183148246802980871;
+  constexpr double c7b = -0.22252093395631440429, s7b = 0.97492791218182360702;
+  constexpr double c7c = -0.90096886790241912624, s7c = 0.43388373911755812048;
+  if constexpr (R == 2) return k ? make_float2(-1.0f, 0.0f) : make_float2(1.0f, 0.0f);
+  if constexpr (R == 4) {
+    switch (k) {
+      case 1: return make_float2(0.0f, -1.0f);
+      case 2: return make_float2(-1.0f, 0.0f);
+      case 3: return make_float2(0.0f, 1.0f);
+      default: return make_float2(1.0f, 0.0f);
+    }
+  }
+  if constexpr (R == 3) {
+    switch (k) {
+      case 1: return make_float2((float)c3, (float)-s3);
+      case 2: return make_float2((float)c3, (float)s3);
+      default: return make_float2(1.0f, 0.0f);
+    }
+  }
+  if constexpr (R == 5) {
+    switch (k) {
+      case 1: return make_float2((float)c5a, (float)-s5a);
+      case 2: return make_float2((float)c5b, (float)-s5b);
+      case 3: return make_float2((float)c5b, (float)s5b);
+      case 4: return make_float2((float)c5a, (float)s5a);
+      default: return make_float2(1.0f, 0.0f);
+    }
+  }
+  switch (k) {  // R == 7
+    case 1: return make_float2((float)c7a, (float)-s7a);
+    case 2: return make_float2((float)c7b, (float)-s7b);
+    case 3: return make_float2((float)c7c, (float)-s7c);
+    case 4: return make_float2((float)c7c, (float)s7c);
+    case 5: return make_float2((float)c7b, (float)s7b);
+    case 6: return make_float2((float)c7a, (float)s7a);
+    default: return make_float2(1.0f, 0.0f);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void fft_stage(float2* A, const float2* tw, int L, int tl, int m, int pitch, uint64_t mnb,
+                                          uint64_t mm) {
   constexpr int KB = fft_kb(R);
   const int nb = L / R, mr = m * R, twstep = L / mr, nbt = tl * nb;
   float2 y[KB][R];
@@ -80,7 +130,7 @@ __device__ __forceinline__ void fft_stage(float2* A, const float2* __restrict__ 
   for (int i = 0; i < KB; ++i) {
     const int b = threadIdx.x + i * kFftThreads;
     if (b < nbt) {
-      const int l = b / nb, j = b - l * nb, k = j % m;
+      const int l = fft_div(b, mnb), j = b - l * nb, k = j - fft_div(j, mm) * m;
       float2 x[R];
 #pragma unroll
       for (int q = 0; q < R; ++q) {
@@ -92,9 +142,11 @@ __device__ __forceinline__ void fft_stage(float2* A, const float2* __restrict__ 
         float2 acc = x[0];
 #pragma unroll
         for (int q = 1; q < R; ++q) {
-          const float2 w = c_root[R][(p * q) % R];
-          acc.x += x[q].x * w.x - x[q].y * w.y;
-          acc.y += x[q].x * w.y + x[q].y * w.x;
+          const float2 w = fft_root<R>((p * q) % R);
+          const float re = (w.x != 0.0f ? x[q].x * w.x : 0.0f) - (w.y != 0.0f ? x[q].y * w.y : 0.0f);
+          const float im = (w.y != 0.0f ? x[q].x * w.y : 0.0f) + (w.x != 0.0f ? x[q].y * w.x : 0.0f);
+          acc.x += re;
+          acc.y += im;
         }
         y[i][p] = acc;
       }
@@ -105,8 +157,8 @@ __device__ __forceinline__ void fft_stage(float2* A, const float2* __restrict__ 
   for (int i = 0; i < KB; ++i) {
     const int b = threadIdx.x + i * kFftThreads;
     if (b < nbt) {
-      const int l = b / nb, j = b - l * nb, k = j % m;
-      const int base = (j / m) * mr + k;
+      const int l = fft_div(b, mnb), j = b - l * nb, jm = fft_div(j, mm), k = j - jm * m;
+      const int base = jm * mr + k;
 #pragma unroll
       for (int p = 0; p < R; ++p) A[l * pitch + base + p * m] = y[i][p];
     }
@@ -121,6 +173,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_lines(FftPass P, float2* __
   extern __shared__ float2 lds[];
   const int L = P.L, T = P.T, pitch = fft_pitch(L);
   float2* A = lds;
+  float2* TW = lds + T * pitch;  // the axis' twiddles, read by every stage
+  for (int t = threadIdx.x; t < L; t += blockDim.x) TW[t] = P.tw[t];
   const int64_t l0 = (int64_t)blockIdx.x * T;
   const int tl = (int)min<int64_t>(T, P.nlines - l0);
   const int n = tl * L;
@@ -143,12 +197,13 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_lines(FftPass P, float2* __
   int m = 1;
   for (int s = 0; s < P.nst; ++s) {
     const int r = P.radix[s];  // uniform
+    const uint64_t mnb = P.mag_nb[s], mm = P.mag_m[s];
     switch (r) {
-      case 2: fft_stage<2>(A, P.tw, L, tl, m, pitch); break;
-      case 3: fft_stage<3>(A, P.tw, L, tl, m, pitch); break;
-      case 4: fft_stage<4>(A, P.tw, L, tl, m, pitch); break;
-      case 5: fft_stage<5>(A, P.tw, L, tl, m, pitch); break;
-      default: fft_stage<7>(A, P.tw, L, tl, m, pitch); break;
+      case 2: fft_stage<2>(A, TW, L, tl, m, pitch, mnb, mm); break;
+      case 3: fft_stage<3>(A, TW, L, tl, m, pitch, mnb, mm); break;
+      case 4: fft_stage<4>(A, TW, L, tl, m, pitch, mnb, mm); break;
+      case 5: fft_stage<5>(A, TW, L, tl, m, pitch, mnb, mm); break;
+      default: fft_stage<7>(A, TW, L, tl, m, pitch, mnb, mm); break;
     }
     m *= r;
   }
@@ -197,21 +252,6 @@ static std::vector<int> fft_radices(int L) {
   return r;
 }
 
-static bool roots_ready = false;
-
-static int upload_roots() {
-  if (roots_ready) return SVIMX_OK;
-  float2 h[8][8] = {};
-  for (int r : {2, 3, 4, 5, 7})
-    for (int k = 0; k < r; ++k) {
-      const double a = -2.0 * M_PI * k / r;
-      h[r][k] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-  SVIMX_CHECK_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_root), h, sizeof(h)));
-  roots_ready = true;
-  return SVIMX_OK;
-}
-
 static FftPass make_pass(const FftPlan& F, int axis, int in_mode, int out_mode, float coef) {
   FftPass P{};
   P.axis = axis;
@@ -220,7 +260,13 @@ static FftPass make_pass(const FftPlan& F, int axis, int in_mode, int out_mode, 
   P.T = std::max(1, std::min(16, (kFftTileElems / fft_pitch(L))));
   const std::vector<int> r = fft_radices(L);
   P.nst = (int)r.size();
-  for (int s = 0; s < P.nst; ++s) P.radix[s] = r[s];
+  int m = 1;
+  for (int s = 0; s < P.nst; ++s) {
+    P.radix[s] = r[s];
+    P.mag_nb[s] = (1ull << 32) / (uint64_t)(L / r[s]) + 1;
+    P.mag_m[s] = (1ull << 32) / (uint64_t)m + 1;
+    m *= r[s];
+  }
   const Dims& d = F.d;
   P.nlines = axis == 2 ? (int64_t)d.nx * d.ny : (axis == 1 ? (int64_t)d.nx * d.nz : (int64_t)d.ny * d.nz);
   P.tw = F.tw[axis];
@@ -235,7 +281,7 @@ static FftPass make_pass(const FftPlan& F, int axis, int in_mode, int out_mode, 
 static int run_pass(hipStream_t st, const FftPass& P, float2* data, const float* occ, const float* khat, float* out,
                     float* khat_out) {
   const unsigned grid = (unsigned)((P.nlines + P.T - 1) / P.T);
-  const size_t lds = sizeof(float2) * (size_t)P.T * fft_pitch(P.L);
+  const size_t lds = sizeof(float2) * ((size_t)P.T * fft_pitch(P.L) + (size_t)P.L);
   hipLaunchKernelGGL(k_fft_lines, dim3(grid), dim3(kFftThreads), lds, st, P, data, occ, khat, out, khat_out);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
@@ -253,8 +299,6 @@ int fft_plan_create(FftPlan& F, const double* weights, int m, Dims d) {
                   kFftTileElems - 1,
                   F.len[a]);
   }
-  int rc = upload_roots();
-  if (rc) return rc;
   const int64_t n = d.size();
   SVIMX_CHECK_HIP(hipMalloc(&F.C, sizeof(float2) * n));
   SVIMX_CHECK_HIP(hipMalloc(&F.khat, sizeof(float) * n));
@@ -282,7 +326,7 @@ int fft_plan_create(FftPlan& F, const double* weights, int m, Dims d) {
   hipStream_t st = nullptr;
   hipLaunchKernelGGL(k_fft_zero, dim3(blocks_for(n, 256)), dim3(256), 0, st, F.C, n);
   hipLaunchKernelGGL(k_fft_place_kernel, dim3(blocks_for((int64_t)W * W * W, 256)), dim3(256), 0, st, F.C, dw, m, d);
-  rc = run_pass(st, make_pass(F, 2, kInComplex, kOutComplex, 0.0f), F.C, nullptr, nullptr, nullptr, nullptr);
+  int rc = run_pass(st, make_pass(F, 2, kInComplex, kOutComplex, 0.0f), F.C, nullptr, nullptr, nullptr, nullptr);
   if (!rc) rc = run_pass(st, make_pass(F, 1, kInComplex, kOutComplex, 0.0f), F.C, nullptr, nullptr, nullptr, nullptr);
   if (!rc) rc = run_pass(st, make_pass(F, 0, kInComplex, kOutRealK, 0.0f), F.C, nullptr, nullptr, nullptr, F.khat);
   e = hipStreamSynchronize(st);

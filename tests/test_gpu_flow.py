@@ -107,6 +107,8 @@ def test_history_matches_oracle(device):
     assert res.stop_reason == reason == "target_volume"
     assert abs(sim.initial_volume - ref.initial_volume) < 1e-6 * ref.initial_volume
     assert len(res.history) == len(hist)
+    print("history: max fraction difference %.2e over %d steps"
+          % (max(abs(a.fraction - b.fraction) for a, b in zip(res.history, hist)), len(hist)), flush=True)
     for a, b in zip(res.history, hist):
         assert abs(a.fraction - b.fraction) < 1e-4
         assert abs(a.dt - b.dt) < 1e-3 * b.dt
@@ -120,4 +122,46 @@ def test_history_matches_oracle(device):
     dist = _sampled_hausdorff(v_gpu, f_gpu, v_ref, f_ref)
     print(f"final surface: sampled Hausdorff distance {dist:.3e} h "
           f"({len(f_gpu)} / {len(f_ref)} triangles)", flush=True)
+    assert dist < 0.05
+
+
+def test_cube_res64_history_matches_oracle(device):
+    """The reference's CPU-runnable case (a cube of side 1 as 12 triangles, resolution
+    64, SVI k=1 s=1 t=0.93 r=0.1) eroded to 90% volume on the GPU and on the oracle
+    from the same signed distance: per-step fraction within 1e-4 and dt within 1e-3
+    relative, the same steps, reinitialisations and stop, and final zero sets within
+    0.05 h Hausdorff."""
+    from oracle.surface import marching_tetrahedra
+    from svierosion_amd.api import ErosionParams, make_grid
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.levelset import LevelSet
+    from svierosion_amd.sdf import mesh_to_sdf
+    from svierosion_amd.shapes import cube
+    from svierosion_amd.surface import device_mesh
+    mesh = cube(1.0)
+    p = ErosionParams(law="svi", radius=0.1, k=1, s=1, t=0.93, resolution=64)
+    g = make_grid(mesh, p)
+    phi = mesh_to_sdf(mesh, g, band=p.band)
+    lo, hi = mesh.bounds
+    ls = LevelSet(phi.clone(), g, width=p.band)
+    sim = ErosionSimulation(ls, p.make_law(length_scale=float((hi - lo).max())), cfl=p.cfl,
+                            feedback_cfl=p.feedback_cfl, reinit_distance=p.reinit_distance,
+                            reinit_iterations=p.reinit_iterations)
+    res = sim.run(target_fraction=0.9)
+    ref_ls = F.LevelSet(phi.cpu().numpy().reshape(g.shape), g.shape, g.spacing, width=p.band)
+    ref = F.Simulation(ref_ls, F.SVILaw(0.1, 1, 1, 0.93), cfl=p.cfl, feedback_cfl=p.feedback_cfl,
+                       reinit_distance=p.reinit_distance, reinit_iterations=p.reinit_iterations)
+    reason, steps, _, frac, hist = ref.run(target_fraction=0.9)
+    assert res.stop_reason == reason == "target_volume"
+    assert res.steps == steps == len(hist) and any(b.reinit for b in hist)
+    err = max(abs(a.fraction - b.fraction) for a, b in zip(res.history, hist))
+    for a, b in zip(res.history, hist):
+        assert abs(a.dt - b.dt) < 1e-3 * b.dt
+        assert a.reinit == b.reinit
+    v_gpu, f_gpu = device_mesh(ls.flat, ls.band, ls.nb, ls.dims, 0.0)
+    v_ref, f_ref = marching_tetrahedra(ref_ls.flat.reshape(g.shape))
+    dist = _sampled_hausdorff(v_gpu, f_gpu, v_ref, f_ref)
+    print(f"cube res 64: {steps} steps, max fraction difference {err:.2e}, final surfaces "
+          f"{dist:.2e} h apart ({len(f_gpu)} / {len(f_ref)} triangles)", flush=True)
+    assert err < 1e-4
     assert dist < 0.05

@@ -168,11 +168,14 @@ def test_nr_direct_matches_fft_and_brute_force(device):
     Nfft = F.exterior_fraction_fft(f, F.ball_kernel_hat(g.shape, w), g.spacing, r)
     m = conv.m
     inner = tuple(slice(m + 1, n - m - 1) for n in g.shape)
-    assert np.abs(N[inner] - Nfft[inner]).max() < 2e-5
+    e_fft = np.abs(N[inner] - Nfft[inner]).max()
     rng = np.random.default_rng(0)
     pts = np.stack([rng.integers(m + 1, n - m - 1, 200) for n in g.shape], 1)
     direct = F.exterior_fraction_direct(f, w, g.spacing, r, pts)
-    assert np.abs(N[pts[:, 0], pts[:, 1], pts[:, 2]] - direct).max() < 2e-6
+    e_fp64 = np.abs(N[pts[:, 0], pts[:, 1], pts[:, 2]] - direct).max()
+    print(f"N_r direct: {e_fft:.2e} from the oracle FFT, {e_fp64:.2e} from the FP64 sum", flush=True)
+    assert e_fft < 2e-5
+    assert e_fp64 < 2e-6
 
 
 @pytest.mark.parametrize("ratio,tol", [(8, 3e-4), (16, 6e-5)])
@@ -317,8 +320,10 @@ def test_nr_fft_matches_direct_and_oracle(device):
     Nref = F.exterior_fraction_fft(f, F.ball_kernel_hat(g.shape, F.ball_cell_weights(r, g.spacing)), g.spacing, r)
     m = fft.m
     inner = tuple(slice(m + 1, n - m - 1) for n in g.shape)
-    assert np.abs(Nf[inner] - Nref[inner]).max() < 2e-5
-    assert np.abs(Nf[inner] - Nd[inner]).max() < 2e-5
+    e_ref, e_dir = np.abs(Nf[inner] - Nref[inner]).max(), np.abs(Nf[inner] - Nd[inner]).max()
+    print(f"N_r FFT: {e_ref:.2e} from the oracle FFT, {e_dir:.2e} from the direct sum", flush=True)
+    assert e_ref < 2e-5
+    assert e_dir < 2e-5
 
 
 def test_nr_fft_wide_ball_sphere_closed_form(device):

@@ -30,8 +30,9 @@ def test_cube_sdf_exact(device, h):
     phi = mesh_to_sdf(m, g, band=6).cpu().numpy()
     ex = np.clip(_box_sdf(_nodes(g)), -6 * h, 6 * h)
     err = np.abs(phi - ex).max() / h
-    assert err < 2e-5  # reference bound; local coordinates give ~1e-7
-    assert err < 2e-6
+    print(f"cube SDF, h={h}: max error {err:.2e} h", flush=True)
+    assert err < 2e-5  # reference bound
+    assert err < 1e-6  # measured 5.3e-7 / 5.9e-7 h: the float32 rounding of phi itself
 
 
 def test_torus_band_vs_brute_force_and_sign(device):
@@ -87,7 +88,9 @@ def test_sdf_matches_oracle_fp64(device):
     d = dist.cpu().numpy()
     od = odist.astype(np.float64)
     within = od < 6.0
-    assert np.abs(d[within] - od[within]).max() < 1e-6 * 8  # cells; FP64 oracle, float32 local frames
+    err = np.abs(d[within] - od[within]).max()
+    print(f"SDF vs FP64 oracle: max distance error {err:.2e} h over {within.sum()} nodes", flush=True)
+    assert err < 1e-6  # cells (= h); FP64 oracle, float32 local frames; measured 4.8e-7
     p = phi.cpu().numpy().reshape(-1)
     assert np.array_equal(p < 0, ophi.reshape(-1) < 0)
     assert np.abs(p - ophi.reshape(-1)).max() < 1e-5 * h

@@ -206,19 +206,27 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
   // ahead and the gradient stencil of the next node are loaded while this
   // node's closest point and interpolation run (past the end, the loads
   // repeat an in-bounds node and are not used).
-  const int64_t stride = G * kBandThreads;
-  int64_t q = lb * kBandThreads + threadIdx.x;
+  // With G a multiple of the XCD count, the blocks of XCD x (lb mod 8 = x)
+  // stride over the x-th contiguous eighth of the band list, so the N_r lines
+  // that neighbouring nodes' tricubic stencils share stay in one XCD's L2.
+  int64_t q = lb * kBandThreads + threadIdx.x, stride = G * kBandThreads, qend = nb;
+  if (G % kXcds == 0) {
+    const int64_t per = (nb + kXcds - 1) / kXcds, x = lb % kXcds;
+    q = x * per + (lb / kXcds) * kBandThreads + threadIdx.x;
+    stride = (G / kXcds) * kBandThreads;
+    qend = min(nb, x * per + per);
+  }
   int64_t n = 0, n1 = 0;
   Grad gnext{};
-  if (q < nb) {  // band nodes keep their stencils inside the grid
+  if (q < qend) {  // band nodes keep their stencils inside the grid
     n = band[q];
-    n1 = q + stride < nb ? band[q + stride] : n;
+    n1 = q + stride < qend ? band[q + stride] : n;
     gnext = central(phi, n, d);
   }
-  for (; q < nb; q += stride) {
+  for (; q < qend; q += stride) {
     const Grad g = gnext;
     gnext = central(phi, n1, d);
-    const int64_t n2 = q + 2 * stride < nb ? band[q + 2 * stride] : n1;
+    const int64_t n2 = q + 2 * stride < qend ? band[q + 2 * stride] : n1;
     const float x = fabsf(g.c);
     const bool near = x < h;
     float v = 0.0f;

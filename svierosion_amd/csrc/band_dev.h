@@ -5,8 +5,10 @@
 // G * kBandThreads, block partials at partials[slot * G + lb].  The
 // single-job kernels call it with (blockIdx.x, gridDim.x); the batch engine
 // with the job's own (lb, G), so a job's arithmetic and reduction order are
-// the same alone and in a batch.  (Contiguous XCD-local chunks of the list
-// were measured slower in a batch of curvature jobs.)
+// the same alone and in a batch.  Only velocity_body gives each XCD a
+// contiguous eighth of the list (its tricubic/trilinear gathers share lines
+// between neighbouring nodes); for the other bodies that split measured
+// slower or within noise.
 #pragma once
 
 #include "common.h"

@@ -8,7 +8,7 @@ import torch
 from scipy.integrate import solve_ivp
 
 from oracle import flow as F
-from tests.conftest import CTR, sphere_phi
+from tests.conftest import CTR, sampled_hausdorff, sphere_phi
 
 pytestmark = pytest.mark.gpu
 
@@ -66,31 +66,6 @@ def test_curvature_sphere_radius(device):
         assert abs(Rn.mean() - Rex) < 5e-4, law.describe()
 
 
-def _sampled_hausdorff(va, fa, vb, fb, k=128):
-    """Symmetric Hausdorff distance (index units = cells) between two triangle meshes,
-    sampled at every vertex and triangle centroid of each mesh, each sample's distance
-    to the other mesh exact (point-triangle distance) over the k triangles with the
-    nearest centroids.  A marching-tetrahedra triangle lies in one unit cube (diameter
-    <= sqrt(3)) and no point of a triangle is farther than 2/3 of its diameter from its
-    centroid, so a triangle whose centroid is c away from a sample is at least
-    c - 2/sqrt(3) away: the check below proves that no other triangle is closer."""
-    from scipy.spatial import cKDTree
-    from oracle.stencils import point_triangle_distance2
-
-    def one_sided(v, f, w, g):
-        p = np.concatenate([v, v[f].mean(axis=1)])
-        tri = w[g]
-        c, cols = cKDTree(tri.mean(axis=1)).query(p, k=min(k, len(tri)))
-        t = tri[cols]
-        d2 = point_triangle_distance2(p[:, None, :], t[..., 0, :], t[..., 1, :], t[..., 2, :])
-        best = np.sqrt(d2.min(axis=1))
-        assert (c[:, -1] - 2.0 / math.sqrt(3.0) >= best).all() or len(tri) <= k
-        return float(best.max())
-
-    va, vb = va.astype(np.float64), vb.astype(np.float64)
-    return max(one_sided(va, fa, vb, fb), one_sided(vb, fb, va, fa))
-
-
 def test_history_matches_oracle(device):
     """Same seeded sphere, SVI k=1 s=1 t=0.93 run to a target fraction on GPU and oracle:
     per-step volume fraction within 1e-4, step count and stop reason equal, and the
@@ -119,7 +94,7 @@ def test_history_matches_oracle(device):
     v_gpu, f_gpu = device_mesh(ls.flat, ls.band, ls.nb, ls.dims, 0.0)
     v_ref, f_ref = marching_tetrahedra(ref_ls.flat.reshape(g.shape))
     assert len(f_gpu) > 1000 and len(f_ref) > 1000
-    dist = _sampled_hausdorff(v_gpu, f_gpu, v_ref, f_ref)
+    dist = sampled_hausdorff(v_gpu, f_gpu, v_ref, f_ref)
     print(f"final surface: sampled Hausdorff distance {dist:.3e} h "
           f"({len(f_gpu)} / {len(f_ref)} triangles)", flush=True)
     assert dist < 0.05
@@ -160,7 +135,7 @@ def test_cube_res64_history_matches_oracle(device):
         assert a.reinit == b.reinit
     v_gpu, f_gpu = device_mesh(ls.flat, ls.band, ls.nb, ls.dims, 0.0)
     v_ref, f_ref = marching_tetrahedra(ref_ls.flat.reshape(g.shape))
-    dist = _sampled_hausdorff(v_gpu, f_gpu, v_ref, f_ref)
+    dist = sampled_hausdorff(v_gpu, f_gpu, v_ref, f_ref)
     print(f"cube res 64: {steps} steps, max fraction difference {err:.2e}, final surfaces "
           f"{dist:.2e} h apart ({len(f_gpu)} / {len(f_ref)} triangles)", flush=True)
     assert err < 1e-4

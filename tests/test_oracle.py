@@ -284,3 +284,26 @@ def test_inside_mask_sphere_mesh_parity_and_winding():
         ins = O.inside_mask(m.vertices, m.faces, origin, h, shape, signed=signed)
         clear = np.abs(r - 0.79) > 0.02
         assert np.array_equal(ins[clear], (r < 0.79)[clear])
+
+
+def test_sampled_hausdorff_measures_a_known_offset():
+    """The surface-distance checker of the evolution parity tests: marching-tetrahedra
+    spheres whose level differs by delta are delta apart (up to the linear
+    interpolation along tetrahedron edges), a mesh is 0 from itself, and a mesh
+    with one vertex displaced reads at least that displacement's normal part."""
+    from oracle.surface import marching_tetrahedra
+    from tests.conftest import sampled_hausdorff
+    n = 30
+    ax = np.arange(n) - 14.3
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    R = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    va, fa = marching_tetrahedra((R - 9.0).astype(np.float32))
+    assert sampled_hausdorff(va, fa, va, fa) < 1e-5
+    for delta in (0.02, 0.3):
+        vb, fb = marching_tetrahedra((R - 9.0 - delta).astype(np.float32))
+        d = sampled_hausdorff(va, fa, vb, fb)
+        assert abs(d - delta) < 0.1 * delta, (delta, d)
+    vc = va.copy()
+    i = int(np.argmax(vc[:, 0]))                     # a vertex at the +x pole: normal ~ x
+    vc[i, 0] += 0.5
+    assert sampled_hausdorff(va, fa, vc, fa) > 0.45

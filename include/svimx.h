@@ -269,6 +269,36 @@ int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, i
 int svimx_free(void* device_ptr);
 int svimx_copy_to_host(void* dst, const void* device_src, int64_t bytes);
 
+/* Per-point output fields of an eroded surface: p holds n index-coordinate
+ * points (clamped to [1, dim - 3.001]).  SVI laws: q1 = N_r by tricubic
+ * interpolation of `field`, speed = V(N).  Curvature laws: (k1, k2) by tricubic
+ * interpolation of the dense pairs `k12` (2 floats per node, zero off the band),
+ * q1 = H = (k1 + k2)/2, q2 = K = k1 k2 (may be NULL), speed = V(k1, k2).
+ * Replaces the N / H / K / speed annotation of svierosion/api.py:171-194. */
+int svimx_annotate_points(void* stream, const svimx_law_t* law, const float* field, const float* k12, int nx,
+                          int ny, int nz, const float* p, int64_t n, float* q1, float* q2, float* speed);
+
+/* ---------------------------------------------------------------- parameters (host)
+ * Host-only: no device call, usable without a GPU. */
+/* Smallest 2^a 3^b 5^c 7^d >= n.  Replaces next_fast_size (svierosion/grid.py:18-28). */
+int svimx_smooth_size(int64_t n, int64_t* out);
+/* Grid holding the box [lo, hi] centred with >= padding on every side:
+ * shape = ceil((hi - lo + 2 padding) / spacing) + 1 per axis (rounded up to a
+ * 7-smooth size when fast), origin = centre - (shape - 1) spacing / 2, in the
+ * reference's FP64 operation order.  Replaces Grid.around (svierosion/grid.py:45-56). */
+int svimx_grid_around(const double* lo, const double* hi, double spacing, double padding, int fast,
+                      int64_t* shape, double* origin);
+/* Validated speed-law record (kind 0 SVI, 1 mean curvature, 2 Gauss curvature);
+ * for SVI laws with s = 1, n_cap is the largest double with
+ * V(n_cap) < max_speed_ratio * V(1/2).  SVIMX_EBADARG with a message for invalid
+ * parameters.  Replaces the law constructors of svierosion/flows.py:37-80,
+ * :127-137. */
+int svimx_law_init(int kind, double radius, double k, double s, double t, double eps, double max_speed_ratio,
+                   double alpha, double delta, svimx_law_t* out);
+/* *out = n^k / (t - n)^s + eps in FP64 (SVI laws).  Replaces SVIPowerLaw._raw
+ * (svierosion/flows.py:82-87). */
+int svimx_law_raw(const svimx_law_t* law, double n, double* out);
+
 /* ---------------------------------------------------------------- mesh measures (host)
  * vertices: nv x 3 FP64, faces: nf x 3 int64 vertex indices (row-major).
  * Host-only: no device call, usable without a GPU. */
@@ -321,6 +351,13 @@ int svimx_op_reinit_geometry(void* stream, const float* phi, const int32_t* idx,
                              int ny, int nz, double h, double cfl, float* geo); /* :335-359 */
 int svimx_op_point_triangle_distance(void* stream, const float* p, const float* tri, int64_t n,
                                      float* out);                             /* :382-427 */
+/* Speed law at n values, with the velocity pass's device functions: SVI laws
+ * read N from q1 and write V and dV/dN; curvature laws read k1, k2 from q1, q2
+ * and write V and D = dV/dk1 + dV/dk2.  D may be NULL.  Replaces
+ * SVIPowerLaw.speed / dspeed and speed_and_diffusivity (svierosion/flows.py:89-108,
+ * :165-205). */
+int svimx_op_law(void* stream, const svimx_law_t* law, const float* q1, const float* q2, int64_t n, float* V,
+                 float* D);
 
 /* ---------------------------------------------------------------- batch engine
  * Many erosion jobs of different grids and laws stepped together: every

@@ -132,6 +132,12 @@ _SIGNATURES = {
     "svimx_op_interface_mask": (_I, [_P, _P, _P, _I64, _I, _I, _I, _P]),
     "svimx_op_reinit_geometry": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _D, _P]),
     "svimx_op_point_triangle_distance": (_I, [_P, _P, _P, _I64, _P]),
+    "svimx_op_law": (_I, [_P, C.POINTER(Law), _P, _P, _I64, _P, _P]),
+    "svimx_annotate_points": (_I, [_P, C.POINTER(Law), _P, _P, _I, _I, _I, _P, _I64, _P, _P, _P]),
+    "svimx_smooth_size": (_I, [_I64, C.POINTER(_I64)]),
+    "svimx_grid_around": (_I, [C.POINTER(_D), C.POINTER(_D), _D, _D, _I, C.POINTER(_I64), C.POINTER(_D)]),
+    "svimx_law_init": (_I, [_I, _D, _D, _D, _D, _D, _D, _D, _D, C.POINTER(Law)]),
+    "svimx_law_raw": (_I, [C.POINTER(Law), _D, C.POINTER(_D)]),
 }
 
 

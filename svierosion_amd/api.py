@@ -10,6 +10,7 @@ libsvimx.
 
 from __future__ import annotations
 
+import ctypes as C
 import time
 import warnings
 from dataclasses import asdict, dataclass, field
@@ -24,7 +25,7 @@ from .grid import Grid, get_device
 from .levelset import MARGIN, LevelSet, tricubic
 from .meshio import TriMesh
 from .sdf import mesh_to_sdf, triangle_distances, triangle_frames
-from .surface import extract_surface, sample_field
+from .surface import extract_surface
 
 
 @dataclass
@@ -87,11 +88,11 @@ class ErosionResult:
     simulation: ErosionSimulation | None = None
 
     def summary(self) -> str:
+        """One line: outcome, grid and phase times."""
         t = self.timings
-        return (f"stop: {self.stop_reason}, steps {len(self.history)}, time {self.time:.6g}, "
-                f"volume fraction {self.fraction:.5f} | {self.grid.describe()} | "
-                f"sdf {t.get('sdf', 0):.1f}s, evolution {t.get('evolve', 0):.1f}s, "
-                f"meshing {t.get('mesh', 0):.1f}s")
+        phases = ", ".join(f"{k} {t.get(k, 0.0):.2f} s" for k in ("sdf", "evolve", "mesh"))
+        return (f"{self.stop_reason} after {len(self.history)} steps (t = {self.time:.6g}), "
+                f"{100.0 * self.fraction:.3f}% of the volume left; {self.grid.describe()}; {phases}")
 
 
 def make_grid(mesh: TriMesh, params: ErosionParams) -> Grid:
@@ -137,30 +138,41 @@ def point_mesh_distance(points: np.ndarray, mesh: TriMesh, max_edge: float | Non
 
 def _annotate(sim: ErosionSimulation, mesh: TriMesh, initial: TriMesh, law: SpeedLaw,
               triangles: dict | None = None) -> None:
-    """Per-vertex depth, N_r or curvatures, and speed (svierosion/api.py:171-194)."""
+    """Per-vertex fields of an output surface: ``depth`` below the input mesh, then
+    ``N`` (SVI laws) or ``H`` and ``K`` (curvature laws) and the law's ``speed``
+    there (the field set of svierosion/api.py:171-194).
+
+    One device pass (``svimx_annotate_points``) interpolates N_r -- evaluated at
+    the stencils of the vertices only -- or the dense (k1, k2) pairs of the final
+    field tricubically and evaluates the law with the velocity pass's functions."""
     if len(mesh.vertices) == 0:
         return
-    grid = sim.ls.grid
-    mesh.vertex_data["depth"] = point_mesh_distance(mesh.vertices, initial, device=sim.ls.device, grid=grid,
+    ls = sim.ls
+    grid = ls.grid
+    mesh.vertex_data["depth"] = point_mesh_distance(mesh.vertices, initial, device=ls.device, grid=grid,
                                                     triangles=triangles).astype(np.float32)
+    p = torch.as_tensor(grid.to_index(mesh.vertices), dtype=torch.float32, device=ls.device).contiguous()
+    n = int(p.shape[0])
+    q1, q2, speed = (torch.empty(n, dtype=torch.float32, device=ls.device) for _ in range(3))
+    st = _abi.stream()
+    field = k12 = None
     if law.requires_svi:
-        p = torch.as_tensor(grid.to_index(mesh.vertices), dtype=torch.float32, device=sim.ls.device)
-        N = sim.svi_field_at(p)
-        Nv = tricubic(N.reshape(-1), grid.shape, p).double().cpu().numpy()
-        mesh.vertex_data["N"] = Nv.astype(np.float32)
-        mesh.vertex_data["speed"] = law.speed(torch.as_tensor(Nv)).numpy().astype(np.float32)
+        field = sim.svi_field_at(p)
     else:
-        ls = sim.ls
-        k1, k2 = ls.principal_curvatures(ls.band)
-        dense = torch.zeros(grid.size, dtype=torch.float32, device=ls.device)
-        dense[ls.band.long()] = k1
-        K1 = sample_field(dense, grid, mesh.vertices)
-        dense[ls.band.long()] = k2
-        K2 = sample_field(dense, grid, mesh.vertices)
-        mesh.vertex_data["H"] = (0.5 * (K1 + K2)).astype(np.float32)
-        mesh.vertex_data["K"] = (K1 * K2).astype(np.float32)
-        V, _ = law.speed_and_diffusivity(torch.as_tensor(K1), torch.as_tensor(K2))
-        mesh.vertex_data["speed"] = V.numpy().astype(np.float32)
+        # the pairs array is zero off the band; refresh it on the final field
+        k12 = sim.k12
+        _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(ls.flat), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
+                                             _abi.ptr(k12)), "curvatures")
+    rec = law.to_abi()
+    _abi.check(_abi.lib.svimx_annotate_points(st, C.byref(rec), _abi.ptr(field), _abi.ptr(k12), *ls.dims,
+                                              _abi.ptr(p), n, _abi.ptr(q1), _abi.ptr(q2), _abi.ptr(speed)),
+               "vertex fields")
+    host = [t.cpu().numpy() for t in (q1, q2, speed)]
+    if law.requires_svi:
+        mesh.vertex_data["N"] = host[0]
+    else:
+        mesh.vertex_data["H"], mesh.vertex_data["K"] = host[0], host[1]
+    mesh.vertex_data["speed"] = host[2]
 
 
 def initial_max_nr(sim: ErosionSimulation) -> float:
@@ -199,9 +211,9 @@ def prepare_job(mesh: TriMesh, params: ErosionParams, verbose: bool = False) -> 
     dev = _abi.require_device(get_device(params.device))
     topo = mesh.edge_topology()
     if topo["boundary_edges"] or topo["nonmanifold_edges"]:
-        warnings.warn(f"input mesh is not closed ({topo['boundary_edges']} boundary and "
-                      f"{topo['nonmanifold_edges']} non-manifold edges); inside/outside is decided by a "
-                      "majority vote of ray casts along the three axes, check the result", RuntimeWarning)
+        warnings.warn(f"the mesh has {topo['boundary_edges']} boundary and {topo['nonmanifold_edges']} "
+                      "non-manifold edges, so it does not bound a solid; each grid node is classified by "
+                      "the majority of three axis-parallel ray casts", RuntimeWarning)
     lo, hi = mesh.bounds
     law = params.make_law(length_scale=float((hi - lo).max()))
     timings = {}
@@ -213,17 +225,17 @@ def prepare_job(mesh: TriMesh, params: ErosionParams, verbose: bool = False) -> 
     torch.cuda.current_stream(dev).synchronize()
     timings["sdf"] = time.time() - t0
     if verbose:
-        print(f"[erode] {grid.describe()}, device {dev}; law: {law.describe()}")
+        print(f"[erode] {law.describe()} on a {grid.describe()}, {dev}")
     sim = ErosionSimulation(ls, law, cfl=params.cfl, feedback_cfl=params.feedback_cfl,
                             parabolic_cfl=params.parabolic_cfl, reinit_distance=params.reinit_distance,
                             reinit_iterations=params.reinit_iterations)
     if law.requires_svi and law.s == 1:
         nmax = initial_max_nr(sim)
         if nmax >= law.t:
-            raise ValueError(f"max N_r on the initial surface is {nmax:.4f} >= t = {law.t}; "
-                             "choose t closer to 1 (the flow needs t > N_r everywhere)")
+            raise ValueError(f"t = {law.t} does not exceed N_r on the input surface (largest value "
+                             f"{nmax:.4f}); the speed N^k/(t - N) needs t > N everywhere, so raise t")
         if verbose:
-            print(f"[erode] max N_r on initial surface = {nmax:.4f} (t = {law.t})")
+            print(f"[erode] initial surface: N_r <= {nmax:.4f} below t = {law.t}")
     targets = sorted({float(f) for f in params.snapshots if f > params.target_fraction},
                      reverse=True) + [params.target_fraction]
     return ErosionJob(mesh, params, law, grid, ls, sim, targets, timings, triangles=tris)
@@ -270,10 +282,11 @@ def erode(mesh: TriMesh, params: ErosionParams, callback=None, verbose: bool = F
 
 
 def volume_fraction_report(res: ErosionResult) -> str:
-    v_in = res.initial_volume_mesh
-    v_out = res.mesh.volume() if len(res.mesh.faces) else 0.0
-    return (f"mesh volume {v_in:.6g} -> {v_out:.6g} (ratio {v_out / v_in:.5f}); "
-            f"level-set volume ratio {res.fraction:.5f}")
+    """The remaining volume fraction as measured on the output mesh and on the level set."""
+    before = res.initial_volume_mesh
+    after = res.mesh.volume() if len(res.mesh.faces) else 0.0
+    return (f"volume left: {after / before:.5f} by the output mesh ({after:.6g} of {before:.6g}), "
+            f"{res.fraction:.5f} by the level set")
 
 
 __all__ = ["ErosionParams", "ErosionResult", "ErosionJob", "erode", "prepare_job", "finish_target", "job_result", "make_grid", "point_mesh_distance", "volume_fraction_report",

@@ -212,6 +212,34 @@ __device__ __forceinline__ float tricubic(const float* __restrict__ f, const Dim
   return out;
 }
 
+// tricubic() of two interleaved fields at once: per component the same
+// operations in the same order, so each equals tricubic() of that field.
+__device__ __forceinline__ float2 tricubic2(const float2* __restrict__ f, const Dims& d, const float p[3]) {
+  float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
+  float wx[4], wy[4], wz[4];
+  cubic_w(p[0] - fx, wx);
+  cubic_w(p[1] - fy, wy);
+  cubic_w(p[2] - fz, wz);
+  int64_t b0 = ((int64_t)fx - 1) * d.sx() + ((int64_t)fy - 1) * d.sy() + ((int64_t)fz - 1);
+  float2 out = make_float2(0.0f, 0.0f);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      float wab = wx[a] * wy[b];
+      const float2* row = f + b0 + a * d.sx() + b * d.sy();
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float2 v = row[c];
+        const float w = wab * wz[c];
+        out.x = out.x + w * v.x;
+        out.y = out.y + w * v.y;
+      }
+    }
+  }
+  return out;
+}
+
 // Derivatives of the Lagrange cubic weights (svierosion/kernels.py:139-141).
 __device__ __forceinline__ void cubic_dw(float t, float w[4]) {
   w[0] = -(3.0f * t * t - 6.0f * t + 2.0f) * kSixth;

@@ -122,6 +122,51 @@ __global__ void k_op_ptd(const float* p, const float* t, int64_t n, float* out) 
   out[q] = point_triangle_d2(pp, tt, tt + 3, tt + 6);
 }
 
+// Speed law at given values: SVI laws read N from q1 (V, dV/dN); curvature
+// laws read (k1, k2) from q1, q2 (V, dV/dk1 + dV/dk2).  The same device
+// functions as the velocity pass.
+__global__ void k_op_law(LawF L, const float* q1, const float* q2, int64_t n, float* V, float* D) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  float v, dv;
+  if (L.kind == 0) {
+    v = svi_speed(L, q1[q]);
+    dv = svi_dspeed(L, q1[q]);
+  } else {
+    curv_speed(L, q1[q], q2[q], v, dv);
+  }
+  V[q] = v;
+  if (D) D[q] = dv;
+}
+
+// Per-point output fields of a surface (svierosion/api.py:171-194): SVI laws
+// interpolate N_r tricubically (q1 = N); curvature laws interpolate the dense
+// (k1, k2) pairs tricubically (q1 = H = (k1 + k2)/2, q2 = K = k1 k2); both
+// then evaluate the law's speed there.  Points are clamped to [1, n - 3.001]
+// per axis so every stencil stays on the grid.
+__global__ void k_annotate_points(LawF L, const float* __restrict__ field, const float2* __restrict__ k12, Dims d,
+                                  Hi3 hi, const float* __restrict__ p, int64_t n, float* q1, float* q2,
+                                  float* speed) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  float pp[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) pp[a] = fminf(fmaxf(p[3 * q + a], 1.0f), hi.v[a]);
+  if (L.kind == 0) {
+    const float N = tricubic(field, d, pp);
+    q1[q] = N;
+    if (q2) q2[q] = 0.0f;
+    speed[q] = svi_speed(L, N);
+  } else {
+    const float2 k = tricubic2(k12, d, pp);
+    float v, dv;
+    curv_speed(L, k.x, k.y, v, dv);
+    q1[q] = 0.5f * (k.x + k.y);
+    if (q2) q2[q] = k.x * k.y;
+    speed[q] = v;
+  }
+}
+
 }  // namespace svimx
 
 using namespace svimx;
@@ -294,6 +339,30 @@ int svimx_op_reinit_geometry(void* stream, const float* phi, const int32_t* idx,
 int svimx_op_point_triangle_distance(void* stream, const float* p, const float* tri, int64_t n, float* out) {
   if (n <= 0) return SVIMX_OK;
   hipLaunchKernelGGL(k_op_ptd, GRID1(n), p, tri, n, out);
+  SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+}
+int svimx_op_law(void* stream, const svimx_law_t* law, const float* q1, const float* q2, int64_t n, float* V,
+                 float* D) {
+  SVIMX_REQUIRE(law && law->kind >= 0 && law->kind <= 2, SVIMX_EBADARG, "svimx_op_law: bad law");
+  SVIMX_REQUIRE(n <= 0 || (q1 && V && (law->kind == 0 || q2)), SVIMX_EBADARG,
+                "svimx_op_law: missing input or output (curvature laws need q1 = k1 and q2 = k2)");
+  if (n <= 0) return SVIMX_OK;
+  hipLaunchKernelGGL(k_op_law, GRID1(n), law_to_f(*law), q1, q2, n, V, D);
+  SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+}
+int svimx_annotate_points(void* stream, const svimx_law_t* law, const float* field, const float* k12, int nx,
+                          int ny, int nz, const float* p, int64_t n, float* q1, float* q2, float* speed) {
+  SVIMX_REQUIRE(law && law->kind >= 0 && law->kind <= 2, SVIMX_EBADARG, "svimx_annotate_points: bad law");
+  SVIMX_REQUIRE(law->kind != 0 || field, SVIMX_EBADARG, "svimx_annotate_points: SVI laws need the N_r field");
+  SVIMX_REQUIRE(law->kind == 0 || k12, SVIMX_EBADARG, "svimx_annotate_points: curvature laws need (k1, k2)");
+  SVIMX_REQUIRE(nx >= 5 && ny >= 5 && nz >= 5, SVIMX_EBADARG, "svimx_annotate_points: grid below 5 nodes");
+  SVIMX_REQUIRE(n <= 0 || (p && q1 && speed), SVIMX_EBADARG, "svimx_annotate_points: NULL buffer");
+  if (n <= 0) return SVIMX_OK;
+  Dims d{nx, ny, nz};
+  hipLaunchKernelGGL(k_annotate_points, GRID1(n), law_to_f(*law), field, reinterpret_cast<const float2*>(k12), d,
+                     hi_of(d), p, n, q1, q2, speed);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
 }

@@ -95,10 +95,15 @@ def _read_ascii(fh, elements):
     return data
 
 
-def read_ply(path, vertex_properties=None) -> TriMesh:
-    """Triangle mesh from a PLY file; extra scalar vertex properties go to ``vertex_data``."""
+def read_ply(path, vertex_properties=True) -> TriMesh:
+    """Triangle mesh from a PLY file (polygons fan-triangulated).
+
+    ``vertex_properties``: True keeps every extra scalar vertex property in
+    ``vertex_data`` with its on-disk type, False none; a collection of names
+    keeps only those."""
     with open(path, "rb") as fh:
         fmt, elements = _header(fh)
+        types = {el["name"]: {name: t for name, t, _ in el["props"]} for el in elements}
         if fmt == "ascii":
             data = _read_ascii(fh, elements)
         else:
@@ -111,26 +116,40 @@ def read_ply(path, vertex_properties=None) -> TriMesh:
     f = data.get("face")
     if f is not None:
         key = next((k for k in ("vertex_indices", "vertex_index") if k in f), None)
-        if key is not None:
-            faces = _fan([np.asarray(p, dtype=np.int64) for p in f[key]])
-    extra = {k: np.asarray(val, dtype=np.float64) for k, val in v.items() if k not in ("x", "y", "z")}
-    if vertex_properties is not None:
-        extra = {k: val for k, val in extra.items() if k in vertex_properties}
+        if key is None:
+            raise ValueError("PLY face element lists no vertex_indices / vertex_index property")
+        faces = _fan([np.asarray(p, dtype=np.int64) for p in f[key]])
+    if vertex_properties is True:
+        keep = [k for k in v if k not in ("x", "y", "z")]
+    elif not vertex_properties:
+        keep = []
+    else:
+        keep = [k for k in v if k in set(vertex_properties) and k not in ("x", "y", "z")]
+    # on-disk scalar type in native byte order
+    extra = {k: np.asarray(v[k]).astype(np.dtype(types["vertex"][k]).newbyteorder("="), copy=True) for k in keep}
     return TriMesh(verts, faces, extra)
 
 
-def write_ply(path, mesh: TriMesh, binary: bool = True, precision: str = "float", vertex_data=None) -> None:
-    """Write ``mesh`` (plus ``mesh.vertex_data`` and ``vertex_data``) as PLY."""
+def write_ply(path, mesh: TriMesh, binary: bool = True, precision: str = "float", vertex_data=None,
+              comments=None) -> None:
+    """Write ``mesh`` (plus ``mesh.vertex_data`` and ``vertex_data`` as float32
+    vertex scalars) as PLY, binary little-endian or ascii, creating missing parent
+    directories; each entry of ``comments`` becomes a ``comment`` header line."""
     extra = dict(mesh.vertex_data)
     if vertex_data:
         extra.update(vertex_data)
+    n, m = len(mesh.vertices), len(mesh.faces)
+    for k, val in extra.items():
+        if np.shape(val) != (n,):
+            raise ValueError(f"vertex property {k!r} has shape {np.shape(val)}, expected ({n},)")
     ctype = "float" if precision == "float" else "double"
     vt = "<f4" if precision == "float" else "<f8"
-    n, m = len(mesh.vertices), len(mesh.faces)
-    head = ["ply", f"format {'binary_little_endian' if binary else 'ascii'} 1.0", f"element vertex {n}",
-            f"property {ctype} x", f"property {ctype} y", f"property {ctype} z"]
+    head = ["ply", f"format {'binary_little_endian' if binary else 'ascii'} 1.0"]
+    head += ["comment " + str(c).replace("\r", " ").replace("\n", " ") for c in (comments or [])]
+    head += [f"element vertex {n}", f"property {ctype} x", f"property {ctype} y", f"property {ctype} z"]
     head += [f"property float {k}" for k in extra]
     head += [f"element face {m}", "property list uchar int vertex_indices", "end_header"]
+    Path(path).parent.mkdir(parents=True, exist_ok=True)
     with open(Path(path), "wb") as fh:
         fh.write(("\n".join(head) + "\n").encode("ascii"))
         if binary:

@@ -75,3 +75,43 @@ def test_stop_reasons_max_steps_and_max_time(device):
     r = sim.run(target_fraction=None, max_time=sim.time + 1e-3)
     assert r.stop_reason == "max_time"
     assert abs(sim.time - sim.history[-1].time) < 1e-15
+
+
+@pytest.mark.parametrize("law_kw", [dict(law="svi", radius=0.2, k=2, s=1, t=0.95),
+                                    dict(law="mean", alpha=1.0), dict(law="gauss", alpha=0.25)])
+def test_vertex_fields_follow_the_reference_definition(device, law_kw):
+    """N / H, K and speed at the output vertices (one fused pass, svimx_annotate_points)
+    against the definition of svierosion/api.py:171-194 rebuilt from separate pieces:
+    tricubic interpolation of the full dense N_r (SVI) or of dense k1, k2 fields that
+    are zero off the band (curvature), and the oracle's law in FP64."""
+    import torch
+    from oracle import flow as F
+    from svierosion_amd import ErosionParams, erode
+    from svierosion_amd.levelset import tricubic
+    from svierosion_amd.shapes import icosphere
+    res = erode(icosphere(4, 0.8), ErosionParams(resolution=40, target_fraction=0.96, **law_kw), keep_state=True)
+    sim, ls, mesh = res.simulation, res.level_set, res.mesh
+    p = torch.as_tensor(ls.grid.to_index(mesh.vertices), dtype=torch.float32, device=ls.device)
+    law = sim.law
+    vd = mesh.vertex_data
+    if law.requires_svi:
+        N = tricubic(sim.svi_field().reshape(-1), ls.dims, p).double().cpu().numpy()
+        ref = F.SVILaw(law.radius, law.k, law.s, law.t).speed(N)
+        err_q = np.abs(vd["N"] - N).max()
+        assert err_q < 2e-6
+    else:
+        k1, k2 = ls.principal_curvatures(ls.band)
+        dense = torch.zeros(ls.grid.size, dtype=torch.float32, device=ls.device)
+        dense[ls.band.long()] = k1
+        K1 = tricubic(dense, ls.dims, p).double().cpu().numpy()
+        dense.zero_()
+        dense[ls.band.long()] = k2
+        K2 = tricubic(dense, ls.dims, p).double().cpu().numpy()
+        kind = "mean" if law.name == "mean" else "gauss"
+        ref, _ = F.CurvatureLaw(kind, law.alpha, law.eps, law.delta).speed_and_diffusivity(K1, K2)
+        scale = max(np.abs(K1).max(), np.abs(K2).max())
+        err_q = max(np.abs(vd["H"] - 0.5 * (K1 + K2)).max() / scale, np.abs(vd["K"] - K1 * K2).max() / scale ** 2)
+        assert err_q < 1e-5
+    err_v = np.abs(vd["speed"] - ref).max() / max(np.abs(ref).max(), 1e-30)
+    print(f"{law_kw['law']}: {len(mesh.vertices)} vertices, field error {err_q:.2e}, speed error {err_v:.2e}")
+    assert err_v < 1e-5

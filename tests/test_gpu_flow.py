@@ -42,7 +42,10 @@ def test_svi_sphere_radius_ode(device, k, s, t, tol):
     from svierosion_amd.flows import SVIPowerLaw
     r, T = 0.4, 0.15
     law = SVIPowerLaw(r, k, s, t)
-    Fn = lambda R: float(law.speed(torch.tensor(0.5 + 3.0 * r / (16.0 * R), dtype=torch.float64)))
+
+    def Fn(R):                      # the law's closed form in FP64, N from the sphere formula
+        n = 0.5 + 3.0 * r / (16.0 * R)
+        return n ** k / (t - n) if s else n ** k
     Rex = solve_ivp(lambda _t, y: [-Fn(y[0])], (0, T), [1.0], rtol=1e-12, atol=1e-14).y[0, -1]
     ls, g, _ = _gpu_sphere(1.0, r / 8, r + 10 * r / 8)
     ErosionSimulation(ls, law).run(target_fraction=None, max_time=T)
@@ -140,3 +143,38 @@ def test_cube_res64_history_matches_oracle(device):
           f"{dist:.2e} h apart ({len(f_gpu)} / {len(f_ref)} triangles)", flush=True)
     assert err < 1e-4
     assert dist < 0.05
+
+
+@pytest.mark.parametrize("kind,alpha,delta", [("mean", 1.0, 0.0), ("mean", 0.5, 0.05), ("gauss", 0.25, 1e-3),
+                                              ("gauss", 1.0, 0.0)])
+def test_curvature_history_matches_oracle(device, kind, alpha, delta):
+    """A curvature law (svierosion/flows.py:124-213; velocity svierosion/evolve.py:136-145)
+    run to 60% volume on the GPU and on the oracle from the same sphere: per-step
+    fraction within 1e-4, dt within 1e-3 relative, the same reinitialisations and
+    stop, and final zero sets within 0.05 h.  Covers the regularised alpha < 1 power
+    (hardware exp2/log2 on the device) and the parabolic time-step limit."""
+    from oracle.surface import marching_tetrahedra
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.flows import GaussCurvatureLaw, MeanCurvatureLaw
+    from svierosion_amd.surface import device_mesh
+    h = 0.1
+    ls, g, phi = _gpu_sphere(1.0, h, 12 * h)
+    cls = MeanCurvatureLaw if kind == "mean" else GaussCurvatureLaw
+    sim = ErosionSimulation(ls, cls(alpha, 0.0, delta))
+    res = sim.run(target_fraction=0.6, max_steps=2000)
+    ref_ls = F.LevelSet(phi.astype(np.float32), g.shape, h)
+    ref = F.Simulation(ref_ls, F.CurvatureLaw(kind, alpha, delta=delta))
+    reason, steps, _, frac, hist = ref.run(target_fraction=0.6, max_steps=2000)
+    assert res.stop_reason == reason == "target_volume"
+    assert len(res.history) == len(hist) and any(b.reinit for b in hist)
+    err = max(abs(a.fraction - b.fraction) for a, b in zip(res.history, hist))
+    derr = max(abs(a.dt - b.dt) / b.dt for a, b in zip(res.history, hist))
+    for a, b in zip(res.history, hist):
+        assert a.reinit == b.reinit
+    v_gpu, f_gpu = device_mesh(ls.flat, ls.band, ls.nb, ls.dims, 0.0)
+    v_ref, f_ref = marching_tetrahedra(ref_ls.flat.reshape(g.shape))
+    dist = sampled_hausdorff(v_gpu, f_gpu, v_ref, f_ref)
+    print(f"{kind} alpha={alpha}: {steps} steps, max fraction difference {err:.2e}, max dt difference "
+          f"{derr:.2e} (relative), final surfaces {dist:.2e} h apart", flush=True)
+    assert err < 1e-4 and derr < 1e-3 and dist < 0.05
+    assert abs(res.fraction - 0.6) < 1e-5

@@ -344,3 +344,79 @@ def test_nr_fft_wide_ball_sphere_closed_form(device):
     vals = tricubic(N.view(-1), shape, pts)
     err = (vals.double() - F.sphere_exterior_fraction(r, R)).abs().max().item()
     assert err < 6e-5
+
+
+@pytest.mark.parametrize("kw", [dict(k=0, s=0), dict(k=1, s=0), dict(k=2, s=1, t=0.95), dict(k=1, s=1, t=0.93),
+                                dict(k=3, s=1, t=0.8, eps=0.01)])
+def test_svi_law_on_the_device_matches_the_oracle(device, kw):
+    """svimx_op_law (the velocity pass's device functions) against the oracle's law over
+    N in [-0.1, 1.1]: in FP32 (the reference's precision, t rounded to float32 as
+    there) speed and dV/dN within 2e-6 relative to their maxima; in FP64 within 1e-5
+    (near n_cap, t - N ~ 1e-2, so one float32 ulp of t is ~2e-6 of 1/(t - N))."""
+    from oracle import flow as F
+    from svierosion_amd.flows import SVIPowerLaw
+    N = torch.linspace(-0.1, 1.1, 20001, dtype=torch.float32)
+    law = SVIPowerLaw(0.3, **kw)
+    ref = F.SVILaw(0.3, **kw)
+    V, D = law.speed(N), law.dspeed(N)
+    assert V.device == N.device and V.dtype == N.dtype
+    errs = []
+    for dt in (np.float32, np.float64):
+        n = N.numpy().astype(dt)
+        rv, rd = ref.speed(n).astype(np.float64), ref.dspeed(n).astype(np.float64)
+        errs.append((np.abs(V.double().numpy() - rv).max() / np.abs(rv).max(),
+                     np.abs(D.double().numpy() - rd).max() / max(np.abs(rd).max(), 1e-30)))
+    print(f"SVI {kw}: vs FP32 oracle speed {errs[0][0]:.2e}, dspeed {errs[0][1]:.2e}; "
+          f"vs FP64 speed {errs[1][0]:.2e}, dspeed {errs[1][1]:.2e}")
+    assert max(errs[0]) < 2e-6 and max(errs[1]) < 1e-5
+
+
+@pytest.mark.parametrize("kind,alpha,delta,eps", [("mean", 1.0, 0.0, 0.0), ("mean", 0.5, 0.05, 0.0),
+                                                  ("mean", 2.0, 0.0, 0.1), ("gauss", 1.0, 0.0, 0.0),
+                                                  ("gauss", 0.25, 1e-3, 0.0)])
+def test_curvature_law_on_the_device_matches_the_oracle(device, kind, alpha, delta, eps):
+    """svimx_op_law for the curvature laws (incl. the alpha < 1 regularised power, which
+    runs on the hardware exp2/log2) against the oracle in FP64: V and D within 1e-5
+    relative to their maxima."""
+    from oracle import flow as F
+    from svierosion_amd.flows import GaussCurvatureLaw, MeanCurvatureLaw
+    cls = MeanCurvatureLaw if kind == "mean" else GaussCurvatureLaw
+    law = cls(alpha, eps, delta)
+    g = torch.Generator().manual_seed(3)
+    k1 = (torch.rand(50000, generator=g) * 8 - 3).float()
+    k2 = (torch.rand(50000, generator=g) * 8 - 3).float()
+    V, D = law.speed_and_diffusivity(k1, k2)
+    rv, rd = F.CurvatureLaw(kind, alpha, eps, delta).speed_and_diffusivity(k1.double().numpy(), k2.double().numpy())
+    ev = np.abs(V.double().numpy() - rv).max() / np.abs(rv).max()
+    ed = np.abs(D.double().numpy() - rd).max() / np.abs(rd).max()
+    print(f"{kind} alpha={alpha}: V {ev:.2e}, D {ed:.2e}")
+    assert ev < 1e-5 and ed < 1e-5
+
+
+def test_cube_res64_nr_closed_forms_and_oracle(device):
+    """The cube case's own oracle: N_r from the GPU path (device SDF, fractions, direct
+    N_r at the stencils of the points, tricubic) at the cube's face centres, edge
+    midpoints and corners against 1/2, 3/4, 7/8 (discretisation error), and against
+    the oracle's FFT N_r of the same field (arithmetic error, bound 2e-5)."""
+    from svierosion_amd.api import ErosionParams, make_grid
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.flows import SVIPowerLaw
+    from svierosion_amd.levelset import LevelSet, tricubic
+    from svierosion_amd.sdf import mesh_to_sdf
+    from svierosion_amd.shapes import cube
+    from tests.test_oracle_flow import CUBE_NR_BOUNDS, cube_surface_points
+    mesh = cube(1.0)
+    g = make_grid(mesh, ErosionParams(law="svi", radius=0.1, resolution=64))
+    phi = mesh_to_sdf(mesh, g)
+    sim = ErosionSimulation(LevelSet(phi.clone(), g), SVIPowerLaw(0.1, 1, 1, 0.93))
+    ref_ls = F.LevelSet(phi.cpu().numpy().reshape(g.shape), g.shape, g.spacing)
+    w = F.ball_cell_weights(0.1, g.spacing)
+    Nref = F.exterior_fraction_fft(ref_ls.dense_fractions(), F.ball_kernel_hat(g.shape, w), g.spacing, 0.1)
+    for name, (P, exact) in cube_surface_points().items():
+        p = torch.as_tensor(g.to_index(P), dtype=torch.float32, device=device)
+        val = tricubic(sim.svi_field_at(p).reshape(-1), g.shape, p).double().cpu().numpy()
+        ref = S.tricubic(Nref.reshape(-1).astype(np.float64), g.shape, p.double().cpu().numpy())
+        disc, arith = np.abs(val - exact).max(), np.abs(val - ref).max()
+        print(f"cube N_r at {len(P)} {name} points: discretisation |N - {exact}| <= {disc:.2e} "
+              f"(bound {CUBE_NR_BOUNDS[name]:.0e}), GPU vs oracle {arith:.2e}")
+        assert disc < CUBE_NR_BOUNDS[name] and arith < 2e-5

@@ -49,6 +49,44 @@ def test_ply_quads_are_triangulated():
         assert read_ply(p).faces.shape == (2, 3)
 
 
+def test_ply_vertex_properties_flag_types_and_writer_options():
+    """read_ply(vertex_properties=True/False) as in svierosion/meshio.py:264 (a list of
+    names is an extension), extra properties keep their on-disk types (binary and
+    ascii), write_ply creates missing directories and writes comment lines
+    (svierosion/meshio.py:313-333)."""
+    from svierosion_amd import shapes
+    from svierosion_amd.plyio import read_ply, write_ply
+    m = shapes.icosphere(1)
+    q = np.linspace(0, 1, len(m.vertices))
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "new", "deeper", "s.ply")
+        write_ply(p, m, vertex_data={"quality": q, "depth": 2 * q}, comments=["made by a test", "two\nlines"])
+        head = open(p, "rb").read(400).split(b"end_header")[0].decode()
+        assert "comment made by a test\n" in head and "comment two lines\n" in head
+        a = read_ply(p, vertex_properties=True)
+        assert set(a.vertex_data) == {"quality", "depth"} and a.vertex_data["quality"].dtype == np.float32
+        assert read_ply(p, vertex_properties=False).vertex_data == {}
+        assert set(read_ply(p, vertex_properties=["depth"]).vertex_data) == {"depth"}
+        for fmt, rows in (("binary_big_endian", None), ("ascii", None)):
+            txt = (f"ply\nformat {fmt} 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+                   "property float z\nproperty uchar label\nproperty double w\nelement face 1\n"
+                   "property list uchar int vertex_indices\nend_header\n").encode()
+            if fmt == "ascii":
+                body = b"0 0 0 7 0.25\n1 0 0 8 0.5\n0 1 0 9 0.75\n3 0 1 2\n"
+            else:
+                vdt = np.dtype([("x", ">f4"), ("y", ">f4"), ("z", ">f4"), ("label", "u1"), ("w", ">f8")])
+                va = np.array([(0, 0, 0, 7, .25), (1, 0, 0, 8, .5), (0, 1, 0, 9, .75)], dtype=vdt)
+                body = va.tobytes() + np.array([3], "u1").tobytes() + np.array([0, 1, 2], ">i4").tobytes()
+            f = os.path.join(d, f"t_{fmt}.ply")
+            open(f, "wb").write(txt + body)
+            r = read_ply(f)
+            assert r.vertex_data["label"].dtype == np.uint8 and r.vertex_data["label"].tolist() == [7, 8, 9]
+            assert r.vertex_data["w"].dtype == np.float64 and r.vertex_data["w"].tolist() == [.25, .5, .75]
+            assert r.faces.tolist() == [[0, 1, 2]]
+        with pytest.raises(ValueError):
+            write_ply(os.path.join(d, "bad.ply"), m, vertex_data={"x2": q[:-1]})
+
+
 @pytest.mark.parametrize("name", ["cube", "icosphere", "torus", "sh_fragment", "noisy_ellipsoid"])
 def test_generators_closed_and_outward(name):
     from svierosion_amd import shapes
@@ -81,6 +119,46 @@ def test_law_parameters_and_errors():
         ErosionParams(law="nope").make_law()
     p = ErosionParams(law="gauss", alpha=0.25)
     assert abs(p.make_law(2.0).delta - 0.25) < 1e-15
+
+
+def test_n_cap_equals_the_oracle_bisection():
+    """svimx_law_init closes a Newton bracket to adjacent doubles; the reference defines
+    n_cap as the limit of a 200-step bisection (svierosion/flows.py:71-80), which the
+    oracle restates: the two agree to the last bit, and V(n_cap) = 100 V(1/2)."""
+    from oracle import flow as F
+    from svierosion_amd.flows import SVIPowerLaw, flat_speed
+    for k in range(4):
+        for t in (0.51, 0.8, 0.93, 0.95, 0.999):
+            for ratio in (100.0, 7.0, 1.0):
+                a = SVIPowerLaw(2.0, k, 1, t, max_speed_ratio=ratio)
+                b = F.SVILaw(2.0, k, 1, t, max_speed_ratio=ratio)
+                assert a.n_cap == b.n_cap, (k, t, ratio, a.n_cap, b.n_cap)
+                if ratio > 1:
+                    assert abs(a._raw(a.n_cap) / flat_speed(a) / ratio - 1) < 1e-11
+    assert SVIPowerLaw(2.0, 1, 0).n_cap == 1.0
+
+
+def test_grid_sizing_equals_the_oracle():
+    """svimx_grid_around / svimx_smooth_size (C++ host) against the oracle's restatement
+    of svierosion/grid.py:18-56: identical shapes and bit-identical origins."""
+    from oracle import stencils as S
+    from svierosion_amd.grid import Grid, next_fast_size
+    assert [next_fast_size(n) for n in range(1, 2000)] == [S.next_fast_size(n) for n in range(1, 2000)]
+    rng = np.random.default_rng(0)
+    for _ in range(500):
+        lo = rng.normal(size=3) * 50
+        hi = lo + rng.random(3) * 100
+        h, pad = rng.uniform(0.01, 2), rng.uniform(0, 10)
+        g = Grid.around(lo, hi, h, pad)
+        o, _, shp = S.grid_around(lo, hi, h, pad)
+        assert tuple(shp) == g.shape and np.array_equal(np.asarray(o), np.asarray(g.origin))
+    g = Grid.around([0, 0, 0], [1, 1, 1], 0.1, 0.0, fast=False)
+    assert g.shape == (11, 11, 11)
+    with pytest.raises(ValueError):
+        Grid.around([0, 0, 0], [1, 1, 1], -0.1, 0.0)
+    import torch
+    idx = torch.tensor([0, 5, 11 * 11 * 3 + 11 * 4 + 7])
+    assert g.node_coords(idx).tolist()[2] == pytest.approx([0.3, 0.4, 0.7])
 
 
 def test_marching_tetrahedra_closed_and_accurate():

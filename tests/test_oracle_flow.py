@@ -250,3 +250,43 @@ def test_marching_tetrahedra_closed_oriented_ellipsoid():
     exact = 4.0 / 3.0 * math.pi * a * b * c
     print(f"oracle marching tetrahedra: {len(f)} triangles, volume ratio {vol / exact:.5f}")
     assert abs(vol / exact - 1) < 5e-3
+
+
+# ------------------------------------------------------------------ N_r on the cube (closed forms)
+def cube_surface_points():
+    """Face centres, edge midpoints and corners of the unit cube centred at 0, with the
+    exterior ball fraction the closed form gives there for r below half the side:
+    1/2 (half space), 3/4 (right dihedral), 7/8 (octant) (report/report.tex:124-127)."""
+    face = [np.eye(3)[a] * s for a in range(3) for s in (-0.5, 0.5)]
+    edge = []
+    for a in range(3):
+        b, c = [x for x in range(3) if x != a]
+        for sb in (-0.5, 0.5):
+            for sc in (-0.5, 0.5):
+                p = np.zeros(3)
+                p[b], p[c] = sb, sc
+                edge.append(p)
+    corner = [np.array([x, y, z]) for x in (-.5, .5) for y in (-.5, .5) for z in (-.5, .5)]
+    return {"face": (np.array(face), 0.5), "edge": (np.array(edge), 0.75), "corner": (np.array(corner), 0.875)}
+
+
+# discretisation bounds of N_r at res 64, r = 0.1 (R = 6.4 cells): measured 5.8e-6
+# (faces), 3.0e-4 (edges), 4.1e-4 (corners); the cells cut by the edges and corners
+# are not planar cuts, so their fractions carry the tangent-plane error
+CUBE_NR_BOUNDS = {"face": 2e-5, "edge": 1e-3, "corner": 1e-3}
+
+
+def test_cube_res64_nr_closed_forms():
+    """The cube case (side 1, 12 triangles, resolution 64, r = 0.1): the oracle's FFT
+    N_r, interpolated tricubically at the face centres, edge midpoints and corners,
+    equals 1/2, 3/4 and 7/8 within the discretisation bounds above."""
+    v, f = _cube_mesh()
+    origin, h, shape = S.make_grid(v.min(0), v.max(0), resolution=64, radius=0.1)
+    phi, _, _ = O.mesh_to_sdf(v, f, origin, h, shape)
+    ls = F.LevelSet(phi.astype(np.float32), shape, h)
+    N = F.exterior_fraction_fft(ls.dense_fractions(), F.ball_kernel_hat(shape, F.ball_cell_weights(0.1, h)), h, 0.1)
+    for name, (P, exact) in cube_surface_points().items():
+        val = S.tricubic(N.reshape(-1).astype(np.float64), shape, (P - origin) / h)
+        err = np.abs(val - exact).max()
+        print(f"oracle cube N_r at {len(P)} {name} points: |N - {exact}| <= {err:.2e}")
+        assert err < CUBE_NR_BOUNDS[name]

@@ -67,5 +67,5 @@ def test_erode_batch_reports_parameter_errors(device):
     out = erode_batch([cube(1.0), cube(1.0)],
                       [ErosionParams(law="svi", radius=0.2, k=1, s=1, t=0.8, resolution=40),
                        ErosionParams(law="svi", radius=0.2, k=1, s=0, resolution=40, target_fraction=0.95)])
-    assert isinstance(out[0], Exception) and "max N_r" in str(out[0])
+    assert isinstance(out[0], Exception) and "does not exceed N_r" in str(out[0])
     assert out[1].stop_reason == "target_volume"

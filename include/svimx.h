@@ -123,6 +123,19 @@ int svimx_curvatures(void* stream, const float* phi, const int32_t* band, int64_
 int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
                    int nz, double h, const svimx_law_t* law, const float* field, const float* k12,
                    double cut_lo, double cut_hi, int stats, float* V, void* ws, int64_t ws_bytes);
+/* svimx_velocity without a workspace: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The block partials of
+ * the statistics live in a fixed 256 KiB device array per GPU (held until the
+ * process ends; it covers every band size up to 2^31 - 1 nodes, larger nb is
+ * refused); the statistics go to `results` (a caller device array of
+ * SVIMX_R_COUNT doubles; slots SVIMX_R_VMAX .. SVIMX_R_DMAX_ALL are written,
+ * the same values svimx_velocity writes into its workspace).  The stream must
+ * belong to the current device.  Calls that overlap on one GPU share that array
+ * and race: serialise them.  Allocates nothing. */
+int svimx_velocity_static(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny, int nz,
+                          double h, const svimx_law_t* law, const float* field, const float* k12, double cut_lo,
+                          double cut_hi, int stats, float* V, double* results);
 
 /* One TVD-RK3 stage over the band (svierosion/evolve.py:157-160, :172-179):
  *   L = V * |grad phi_in|_Godunov-WENO5

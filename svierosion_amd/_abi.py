@@ -78,7 +78,9 @@ _SIGNATURES = {
     "svimx_curvatures": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
     "svimx_velocity": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _I, _P,
                             _P, _I64]),
-    "svimx_rk_stage": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P]),
+    "svimx_velocity_static": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _I, _P,
+                                   _P]),
+    "svimx_rk_stage":(_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P]),
     "svimx_nr_plan_create": (_I, [_P, _I, _D, _D, _I, _I, _I, C.POINTER(C.c_void_p)]),
     "svimx_nr_plan_destroy": (_I, [_P]),
     "svimx_nr_reset": (_I, [_P, _P]),

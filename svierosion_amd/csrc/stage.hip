@@ -1,6 +1,8 @@
 // Per-stage band kernels: cell fractions (+ volume, interface flags, N_r tile
 // marks), curvature scatter, velocity (+ dt statistics), TVD-RK3 stage update,
 // and the node-list operator entry points that share their device code.
+#include <mutex>
+
 #include "band_dev.h"
 
 namespace svimx {
@@ -167,6 +169,72 @@ __global__ void k_annotate_points(LawF L, const float* __restrict__ field, const
   }
 }
 
+// Velocity pass + (with stats) the finalisation of its 15 statistic slots into
+// results[SVIMX_R_VMAX ..]: one kernel over the band list, then one fold of the
+// block partials.  With statistics the kernel runs even for an empty band (one
+// block of identities), so the fold never reads partials of an earlier call.
+int velocity_launch(hipStream_t st, const float* phi, const int32_t* band, int64_t nb, Dims d, double h,
+                    const svimx_law_t* law, const float* field, const float* k12, double cut_lo, double cut_hi,
+                    int stats, float* V, double* partials, double* results) {
+  SVIMX_REQUIRE(law && law->kind >= 0 && law->kind <= 2, SVIMX_EBADARG, "bad law");
+  SVIMX_REQUIRE(law->kind != 0 || field, SVIMX_EBADARG, "SVI law needs the N_r field");
+  SVIMX_REQUIRE(law->kind == 0 || k12, SVIMX_EBADARG, "curvature law needs the (k1, k2) field");
+  SVIMX_REQUIRE(nb <= 0 || (phi && band && V), SVIMX_EBADARG, "velocity: NULL field, band or output");
+  const int nblk = reduce_grid(nb);
+  if (nb > 0 || stats) {
+    if (law->kind == 0)
+      hipLaunchKernelGGL(k_velocity<true>, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
+                         law_to_f(*law), field, k12, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
+                         V, partials);
+    else
+      hipLaunchKernelGGL(k_velocity<false>, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
+                         law_to_f(*law), field, k12, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
+                         V, partials);
+    SVIMX_CHECK_LAUNCH();
+  }
+  if (!stats) return SVIMX_OK;
+  ReduceSpec spec;
+  const int8_t ops[15] = {1, 2, 1, 1, 0, 0, 0, 1, 2, 1, 1, 0, 0, 0, 1};
+  for (int s = 0; s < kPartialSlots; ++s) spec.op[s] = -1, spec.dst[s] = 0;
+  for (int s = 0; s < 15; ++s) spec.op[s] = ops[s], spec.dst[s] = (int8_t)(SVIMX_R_VMAX + s);
+  return launch_finalize(st, partials, nblk, results, spec);
+}
+
+// Block partials of svimx_velocity_static: a fixed device array, so that entry
+// point allocates nothing.  Its size covers the largest reducing grid
+// (kMaxReduceBlocks blocks x kPartialSlots doubles = 256 KiB per GPU, held
+// until the process ends).  Every call on one GPU shares it: calls that
+// overlap on one GPU (different streams or threads) race.  Each call writes
+// every partial it folds, so nothing needs clearing between calls.
+constexpr int64_t kVelocityScratchDoubles = (int64_t)kMaxReduceBlocks * kPartialSlots;
+static __device__ __align__(256) double scratch[kVelocityScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the GPU of `stream` (the current device),
+// resolved once per device with that device current; host code uses only this
+// address, never the array's name.
+int velocity_scratch(void* stream, double** out) {
+  int dev = -1;
+  SVIMX_CHECK_HIP(hipGetDevice(&dev));
+  hipDevice_t sdev = dev;
+  if (stream) SVIMX_CHECK_HIP(hipStreamGetDevice(as_stream(stream), &sdev));
+  SVIMX_REQUIRE(sdev == dev, SVIMX_EBADARG,
+                "svimx_velocity_static: the stream belongs to device %d but device %d is current", (int)sdev, dev);
+  SVIMX_REQUIRE(dev >= 0 && dev < kMaxDevices, SVIMX_EBADARG, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SVIMX_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SVIMX_REQUIRE(p, SVIMX_EDEVICE, "hipGetSymbolAddress returned NULL for the velocity scratch");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SVIMX_OK;
+}
+
 }  // namespace svimx
 
 using namespace svimx;
@@ -229,31 +297,28 @@ int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t 
                    double h, const svimx_law_t* law, const float* field, const float* k12, double cut_lo,
                    double cut_hi, int stats, float* V, void* ws, int64_t ws_bytes) {
   SVIMX_TRY_BEGIN
-  Dims d{nx, ny, nz};
   Workspace w;
-  if (!carve_workspace(ws, ws_bytes, d.size(), w)) return SVIMX_EBADARG;
-  SVIMX_REQUIRE(law && law->kind >= 0 && law->kind <= 2, SVIMX_EBADARG, "bad law");
-  SVIMX_REQUIRE(law->kind != 0 || field, SVIMX_EBADARG, "SVI law needs the N_r field");
-  SVIMX_REQUIRE(law->kind == 0 || k12, SVIMX_EBADARG, "curvature law needs the (k1, k2) field");
-  hipStream_t st = as_stream(stream);
-  int nblk = reduce_grid(nb);
-  if (nb > 0) {
-    if (law->kind == 0)
-      hipLaunchKernelGGL(k_velocity<true>, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
-                         law_to_f(*law), field, k12, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
-                         V, w.partials);
-    else
-      hipLaunchKernelGGL(k_velocity<false>, dim3(nblk), dim3(kBandThreads), 0, st, phi, band, nb, d, (float)h,
-                         law_to_f(*law), field, k12, (float)(cut_lo * h), (float)(cut_hi * h), hi_of(d), stats,
-                         V, w.partials);
-    SVIMX_CHECK_LAUNCH();
+  if (!carve_workspace(ws, ws_bytes, Dims{nx, ny, nz}.size(), w)) return SVIMX_EBADARG;
+  return velocity_launch(as_stream(stream), phi, band, nb, Dims{nx, ny, nz}, h, law, field, k12, cut_lo, cut_hi,
+                         stats, V, w.partials, w.results);
+  SVIMX_TRY_END
+}
+
+int svimx_velocity_static(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny, int nz,
+                          double h, const svimx_law_t* law, const float* field, const float* k12, double cut_lo,
+                          double cut_hi, int stats, float* V, double* results) {
+  SVIMX_TRY_BEGIN
+  SVIMX_REQUIRE(!stats || results, SVIMX_EBADARG, "svimx_velocity_static: statistics need the results array");
+  SVIMX_REQUIRE(nb >= 0 && nb <= INT32_MAX, SVIMX_EBADARG,
+                "svimx_velocity_static: %lld band nodes; the band list holds at most 2^31 - 1", (long long)nb);
+  // the partials of reduce_grid(nb) <= kMaxReduceBlocks blocks always fit the static array
+  double* partials = nullptr;
+  if (stats) {
+    const int rc = velocity_scratch(stream, &partials);
+    if (rc != SVIMX_OK) return rc;
   }
-  if (!stats) return SVIMX_OK;
-  ReduceSpec spec;
-  const int8_t ops[15] = {1, 2, 1, 1, 0, 0, 0, 1, 2, 1, 1, 0, 0, 0, 1};
-  for (int s = 0; s < kPartialSlots; ++s) spec.op[s] = -1, spec.dst[s] = 0;
-  for (int s = 0; s < 15; ++s) spec.op[s] = ops[s], spec.dst[s] = (int8_t)(SVIMX_R_VMAX + s);
-  return launch_finalize(st, w.partials, nblk, w.results, spec);
+  return velocity_launch(as_stream(stream), phi, band, nb, Dims{nx, ny, nz}, h, law, field, k12, cut_lo, cut_hi,
+                         stats, V, partials, results);
   SVIMX_TRY_END
 }
 

@@ -123,6 +123,17 @@ int svimx_curvatures(void* stream, const float* phi, const int32_t* band, int64_
 int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny,
                    int nz, double h, const svimx_law_t* law, const float* field, const float* k12,
                    double cut_lo, double cut_hi, int stats, float* V, void* ws, int64_t ws_bytes);
+/* Velocity fused with the TVD-RK3 stage that consumes it (svierosion/evolve.py:
+ * 108-160, :172-179): stage 0 computes the velocity with the dt statistics (as
+ * svimx_velocity with stats) and stores L0 = V |grad phi_in| (band-indexed);
+ * the update u0 + dt L0 is then svimx_rk_stage stage 3.  Stages 1 and 2 write
+ * phi_out = 3/4 u0 + 1/4 (phi_in + dt L) and 1/3 u0 + 2/3 (phi_in + dt L) at
+ * the band nodes, V never stored.  The same floats as svimx_velocity followed
+ * by svimx_rk_stage (bit-identical fields and statistics). */
+int svimx_velocity_rk(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                      const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const svimx_law_t* law,
+                      const float* field, const float* k12, double cut_lo, double cut_hi, double dt, float* L0,
+                      void* ws, int64_t ws_bytes);
 /* svimx_velocity without a workspace: no counterpart in the reference.  For
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  The block partials of
@@ -222,10 +233,12 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
  * nx*ny*nz; on entry it holds the nb nodes of the current band).
  * `inside_outside` is the count of inside nodes outside the current band.
  * work_idx/work_info hold work_cap nodes, geo 8*work_cap floats; mask_a/mask_b
- * are dense byte fields.  Synchronises.  info (host, 10 int64): new band
- * count, inside nodes outside the new band, margin ok, band min i/j/k, max
- * i/j/k, work-set size.  Replaces LevelSet.reinitialize
- * (svierosion/levelset.py:154-183). */
+ * are dense byte fields.
+ * Synchronises once, at the end.  A work set above work_cap returns
+ * SVIMX_ECAPACITY with info[9] = its size and leaves phi and the band as they
+ * were.  info (host, 10 int64): new band count, inside nodes outside the new
+ * band, margin ok, band min i/j/k, max i/j/k, work-set size.  Replaces
+ * LevelSet.reinitialize (svierosion/levelset.py:154-183). */
 int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, float* occupancy, int32_t* band,
                  int64_t nb, int64_t inside_outside, int nx, int ny, int nz, double h, double width,
                  int iterations, double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b,
@@ -279,8 +292,10 @@ int svimx_point_mesh_distance(void* stream, const float* tri_local, const int32_
  * extract_surface (svierosion/surface.py:16-56). */
 int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny, int nz,
                           double max_shift, float** xyz, int64_t* nv, int32_t** faces, int64_t* nf);
-int svimx_free(void* device_ptr);
-int svimx_copy_to_host(void* dst, const void* device_src, int64_t bytes);
+/* Release an array of svimx_extract_surface, ordered on `stream`. */
+int svimx_free(void* stream, void* device_ptr);
+/* Copy `bytes` of device memory to host memory, ordered on `stream` (synchronises it). */
+int svimx_copy_to_host(void* stream, void* dst, const void* device_src, int64_t bytes);
 
 /* Per-point output fields of an eroded surface: p holds n index-coordinate
  * points (clamped to [1, dim - 3.001]).  SVI laws: q1 = N_r by tricubic

@@ -78,9 +78,11 @@ _SIGNATURES = {
     "svimx_curvatures": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
     "svimx_velocity": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _I, _P,
                             _P, _I64]),
+    "svimx_velocity_rk": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _D,
+                               _P, _P, _I64]),
     "svimx_velocity_static": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _I, _P,
                                    _P]),
-    "svimx_rk_stage":(_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P]),
+    "svimx_rk_stage": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P]),
     "svimx_nr_plan_create": (_I, [_P, _I, _D, _D, _I, _I, _I, C.POINTER(C.c_void_p)]),
     "svimx_nr_plan_destroy": (_I, [_P]),
     "svimx_nr_reset": (_I, [_P, _P]),
@@ -111,8 +113,8 @@ _SIGNATURES = {
     "svimx_extract_surface": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "svimx_point_mesh_distance": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _I64, _P]),
-    "svimx_free": (_I, [_P]),
-    "svimx_copy_to_host": (_I, [_P, _P, _I64]),
+    "svimx_free": (_I, [_P, _P]),
+    "svimx_copy_to_host": (_I, [_P, _P, _P, _I64]),
     "svimx_engine_create": (_I, [_P, _I, C.POINTER(_P)]),
     "svimx_engine_destroy": (_I, [_P]),
     "svimx_engine_add": (_I, [_P, C.POINTER(JobDesc), C.POINTER(C.c_int)]),

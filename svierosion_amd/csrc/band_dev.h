@@ -192,13 +192,29 @@ __device__ __forceinline__ void curvatures_body(const float* __restrict__ phi, c
 // SVI selects the law family at compile time (true: SVI power law, false:
 // curvature laws), so each variant gets the registers it needs: the SVI
 // tricubic holds 64 values in flight, the curvature trilinear 16.
-template <bool SVI>
+//
+// MODE fuses the TVD-RK3 stage that consumes the velocity (svierosion/evolve.py:
+// 157-160, :172-179), so V never goes through memory and the gathers of the
+// velocity overlap the WENO5 arithmetic of other waves: 0 stores V[q] (the
+// plain pass); 1 stores L0[q] = V G (stage 0 before dt is known; the update
+// u0 + dt L0 is the stage-3 replay); 2 and 3 write stages 1 and 2 of the
+// update into rk.out.  V G, u0 and phi are the same floats as in the separate
+// passes (rk_node), combined by the same operations: the fields are
+// bit-identical to the unfused kernels.
+struct RkFuse {
+  const float* u0 = nullptr;
+  float* out = nullptr;
+  float* L0 = nullptr;
+  float inv_h = 0.0f, dt = 0.0f;
+};
+
+template <bool SVI, int MODE = 0>
 __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, const int32_t* __restrict__ band,
                                               int64_t nb, Dims d, float h, const LawF& L,
                                               const float* __restrict__ field, const float* __restrict__ k12,
                                               float cut_lo, float cut_hi, Hi3 hi,
                                               int stats, float* __restrict__ V, double* __restrict__ partials,
-                                              int64_t lb, int64_t G) {
+                                              int64_t lb, int64_t G, const RkFuse& rk = RkFuse{}) {
   float vmax = 0.0f;
   float qmin = INFINITY, qmax = -INFINITY, dvmax = -INFINITY, dmax = -INFINITY;
   float qmin_a = INFINITY, qmax_a = -INFINITY, dvmax_a = -INFINITY, dmax_a = -INFINITY;
@@ -272,7 +288,18 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
         }
       }
     }
-    V[q] = v;
+    if constexpr (MODE == 0) {
+      V[q] = v;
+    } else {
+      const float Lq = v * godunov_inward(phi, n, d, rk.inv_h);
+      if constexpr (MODE == 1) {
+        rk.L0[q] = Lq;
+      } else if constexpr (MODE == 2) {
+        rk.out[n] = 0.75f * rk.u0[n] + 0.25f * (phi[n] + rk.dt * Lq);
+      } else {
+        rk.out[n] = rk.u0[n] * kThird + (2.0f / 3.0f) * (phi[n] + rk.dt * Lq);
+      }
+    }
     vmax = fmaxf(vmax, v);
     n = n1;
     n1 = n2;

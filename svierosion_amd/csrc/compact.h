@@ -143,4 +143,49 @@ int compact(hipStream_t st, Pred pred, int64_t n, int32_t* out, int64_t cap, Wor
   return SVIMX_OK;
 }
 
+// ---------------------------------------------------------------- device-count form
+// The same three passes over a list whose length lives on the device (n_dev,
+// clamped to nmax; the launches are sized for nmax): no host synchronisation.
+// When *skip_dev exceeds skip_cap the passes do nothing (an earlier stage of
+// the caller overflowed its buffers).  The count is left at
+// ws.chunk_counts[chunks(nmax)], returned in *count_dev.
+template <class Pred>
+__global__ void __launch_bounds__(kCompactThreads)
+    k_compact_count_dev(Pred pred, const int64_t* n_dev, int64_t nmax, const int64_t* skip_dev, int64_t skip_cap,
+                        int64_t* chunk_counts) {
+  const int64_t n = *skip_dev > skip_cap ? 0 : min(*n_dev, nmax);
+  compact_count_body(pred, n, chunk_counts, blockIdx.x);
+}
+template <class Pred, class Map>
+__global__ void __launch_bounds__(kCompactThreads)
+    k_compact_scatter_dev(Pred pred, const int64_t* n_dev, int64_t nmax, const int64_t* skip_dev, int64_t skip_cap,
+                          const int64_t* offs, int32_t* out, int64_t cap, Map map) {
+  const int64_t n = *skip_dev > skip_cap ? 0 : min(*n_dev, nmax);
+  if ((int64_t)blockIdx.x * kCompactChunk >= n) return;  // block-uniform
+  compact_scatter_body(pred, n, offs, out, cap, map, blockIdx.x);
+}
+
+template <class Pred, class Map = IdentityMap>
+int compact_async(hipStream_t st, Pred pred, const int64_t* n_dev, int64_t nmax, const int64_t* skip_dev,
+                  int64_t skip_cap, int32_t* out, int64_t cap, Workspace& ws, const int64_t** count_dev,
+                  Map map = Map()) {
+  const int64_t chunks = (nmax + kCompactChunk - 1) / kCompactChunk;
+  SVIMX_REQUIRE(chunks + 1 <= ws.max_chunks, SVIMX_EBADARG, "workspace too small for compaction");
+  *count_dev = ws.chunk_counts + chunks;
+  if (chunks == 0) {
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, ws.chunk_counts, (int64_t)0);
+    SVIMX_CHECK_LAUNCH();
+    return SVIMX_OK;
+  }
+  hipLaunchKernelGGL(k_compact_count_dev<Pred>, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, pred, n_dev,
+                     nmax, skip_dev, skip_cap, ws.chunk_counts);
+  SVIMX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, ws.chunk_counts, chunks);
+  SVIMX_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_compact_scatter_dev<Pred, Map>), dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, pred,
+                     n_dev, nmax, skip_dev, skip_cap, ws.chunk_counts, out, cap, map);
+  SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+}
+
 }  // namespace svimx

@@ -660,6 +660,15 @@ static int launch_main(Engine& E) {
       (void)hipGetLastError();
       hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
       SVIMX_CHECK_HIP(hipStreamIsCapturing(E.st, &cs));
+      if (cs != hipStreamCaptureStatusNone) {
+        // an invalidated capture whose end was refused: end it again and drop
+        // what it recorded before running the tick eagerly
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(E.st, &g2);
+        if (g2) hipGraphDestroy(g2);
+        (void)hipGetLastError();
+        SVIMX_CHECK_HIP(hipStreamIsCapturing(E.st, &cs));
+      }
       SVIMX_REQUIRE(cs == hipStreamCaptureStatusNone, SVIMX_EDEVICE, "engine stream left in capture state");
       ++E.capture_fallbacks;
       return main_sequence(E);

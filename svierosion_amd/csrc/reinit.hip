@@ -12,6 +12,15 @@
 // crossing; they keep dtau and the six subcell distances in their slot of
 // `geo` (slot = work index), every other node only its 1-byte info word (sign,
 // no crossing, dtau = cfl*h), so a sweep streams ~9 bytes for most nodes.
+// Measured and dropped (res-400 scan, per sweep): splitting the work set into
+// a plain and a crossing list so plain waves skip the subcell arithmetic
+// (25 -> 38 us: consecutive crossing nodes sit on different cache lines, so
+// their waves' loads stop coalescing); 32-bit load offsets with the six
+// reciprocals precomputed in 16-float slots (25 -> 30 us: twice the slot
+// bytes and 62 instead of 29 VGPRs for 2% fewer VALU instructions).
+// Every count stays on the device (kernels read it, launches are sized for the
+// caller's capacity): one host synchronisation per reinitialisation, at the
+// end, for the new band size and extent.
 //
 // Each TVD-RK2 iteration is two Jacobi sweeps ping-ponging between phi and a
 // scratch field:  scratch = phi + dtau R(phi);  phi = (phi + scratch +
@@ -185,36 +194,51 @@ __global__ void __launch_bounds__(kCompactThreads) k_bits_scatter_list(const uin
   }
 }
 
-// Order-preserving list of the set bits of m (synchronises, returns the count).
+// Order-preserving list of the set bits of m; the count stays on the device
+// at ws.chunk_counts[chunks] (*count_dev), no host synchronisation.
 static int compact_bits(hipStream_t st, const uint32_t* m, int64_t nw, int wpl, int nz, int32_t* out, int64_t cap,
-                        Workspace& ws, int64_t* count) {
+                        Workspace& ws, const int64_t** count_dev) {
   const int64_t chunks = (nw + kBitsChunk - 1) / kBitsChunk;
   SVIMX_REQUIRE(chunks + 1 <= ws.max_chunks, SVIMX_EBADARG, "workspace too small for compaction");
-  if (chunks == 0) {
-    *count = 0;
-    return SVIMX_OK;
+  *count_dev = ws.chunk_counts + chunks;
+  if (chunks > 0) {
+    hipLaunchKernelGGL(k_bits_count, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, m, nw, ws.chunk_counts);
+    SVIMX_CHECK_LAUNCH();
   }
-  hipLaunchKernelGGL(k_bits_count, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, m, nw, ws.chunk_counts);
-  SVIMX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, ws.chunk_counts, chunks);
   SVIMX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_bits_scatter_list, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, m, nw, wpl, nz,
-                     ws.chunk_counts, out, cap);
-  SVIMX_CHECK_LAUNCH();
-  int64_t total = 0;
-  SVIMX_CHECK_HIP(hipMemcpyAsync(&total, ws.chunk_counts + chunks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  SVIMX_CHECK_HIP(hipStreamSynchronize(st));
-  *count = total;
-  SVIMX_REQUIRE(total <= cap, SVIMX_ECAPACITY, "compaction output needs %lld entries, capacity %lld",
-                (long long)total, (long long)cap);
+  if (chunks > 0) {
+    hipLaunchKernelGGL(k_bits_scatter_list, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, m, nw, wpl, nz,
+                       ws.chunk_counts, out, cap);
+    SVIMX_CHECK_LAUNCH();
+  }
   return SVIMX_OK;
 }
 
-// partials[slot * G + b] = #{i in block b's share : phi[list[i]] < 0}
+// Device counters of one reinitialisation (integer slots of the workspace
+// header): the work-set size.  A work set larger than the caller's capacity leaves every kernel
+// after the compaction idle (the field is not touched) and is reported at
+// the end, so the caller can retry with the exact size.
+enum { kCtrWork = 0 };
+
+__global__ void k_reinit_counters(long long* ctr, const int64_t* work_total) {
+  if (threadIdx.x == 0) {
+    ctr[kCtrWork] = (long long)*work_total;
+  }
+}
+
+__device__ __forceinline__ int64_t work_count(const long long* ctr, int64_t cap) {
+  const int64_t n = (int64_t)ctr[kCtrWork];
+  return n > cap ? 0 : n;  // overflow: nothing runs
+}
+
+// partials[slot * G + b] = #{i in block b's share : phi[list[i]] < 0}; the list
+// length is n, or the device work count when ctr is given.
 __global__ void __launch_bounds__(kBandThreads)
-    k_count_neg(const float* __restrict__ phi, const int32_t* __restrict__ list, int64_t n,
-                double* __restrict__ partials, int slot) {
+    k_count_neg(const float* __restrict__ phi, const int32_t* __restrict__ list, int64_t n, const long long* ctr,
+                int64_t cap, double* __restrict__ partials, int slot) {
   __shared__ double red[4];
+  if (ctr) n = work_count(ctr, cap);
   double c = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     c += phi[list[i]] < 0.0f ? 1.0 : 0.0;
@@ -224,8 +248,10 @@ __global__ void __launch_bounds__(kBandThreads)
 
 // info word: bits 0-5 crossing flags, bits 6-7 sign code.  Nodes with a
 // crossing store dtau and the six distances at geo[8 i].
-__global__ void k_reinit_geo(const float* __restrict__ phi, const int32_t* __restrict__ idx, int64_t n, Dims d,
-                             float h, float cfl, uint8_t* __restrict__ info, float* __restrict__ geo) {
+__global__ void k_reinit_geo(const float* __restrict__ phi, const int32_t* __restrict__ idx, const long long* ctr,
+                             int64_t cap, Dims d, float h, float cfl, uint8_t* __restrict__ info,
+                             float* __restrict__ geo) {
+  const int64_t n = work_count(ctr, cap);
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   ReinitGeo g = reinit_geometry(phi, idx[i], d, h, cfl);
@@ -240,15 +266,23 @@ __global__ void k_reinit_geo(const float* __restrict__ phi, const int32_t* __res
 // mode 0: out = in + dtau R(in);  mode 1: out = (out + in + dtau R(in)) / 2.
 // Each thread takes kSweepNodes work nodes kBandThreads apart (coalesced) and
 // issues the stencil loads of all of them before computing any, so a wave
-// keeps kSweepNodes x 13 gathers in flight.
+// keeps kSweepNodes x 13 gathers in flight.  The launch is sized for the
+// capacity; blocks past the device work count exit.
 constexpr int kSweepNodes = 2;
 __global__ void __launch_bounds__(256) k_reinit_sweep(int mode, const float* __restrict__ in, float* out,
-                                                      const int32_t* __restrict__ idx, int64_t n, Dims d, float h,
-                                                      float dtau0, const uint8_t* __restrict__ info,
+                                                      const int32_t* __restrict__ idx, const long long* ctr,
+                                                      int64_t cap, Dims d, float h, float dtau0,
+                                                      const uint8_t* __restrict__ info,
                                                       const float* __restrict__ geo) {
-  // XCD-contiguous slices of the (sorted) work list keep stencil neighbours in one L2
-  const int lb = (blockIdx.x % kXcds) * (gridDim.x / kXcds) + blockIdx.x / kXcds;
-  const int64_t base = (int64_t)lb * blockDim.x * kSweepNodes + threadIdx.x;
+  const int64_t n = work_count(ctr, cap);
+  // XCD-contiguous slices of the (sorted) work list keep stencil neighbours in
+  // one L2: the blocks needed for n, taken in XCD order
+  const int64_t need = (n + blockDim.x * kSweepNodes - 1) / (blockDim.x * kSweepNodes);
+  const int64_t g8 = (need + kXcds - 1) / kXcds;  // blocks per XCD
+  const int64_t xcd = blockIdx.x % kXcds, lbx = blockIdx.x / kXcds;
+  if (lbx >= g8) return;  // block-uniform
+  const int64_t lb = xcd * g8 + lbx;
+  const int64_t base = lb * blockDim.x * kSweepNodes + threadIdx.x;
   int64_t node[kSweepNodes];
   uint32_t word[kSweepNodes];
 #pragma unroll
@@ -291,15 +325,16 @@ __global__ void __launch_bounds__(256) k_reinit_sweep(int mode, const float* __r
 // 3..5 / 6..8 min / max i, j, k of the new band.
 __global__ void __launch_bounds__(kBandThreads)
     k_rebuild_work(float* __restrict__ phi, float* __restrict__ c1, float* __restrict__ c2, float* __restrict__ occ,
-                   const int32_t* __restrict__ idx, int64_t n, Dims d, float cap, float thr,
+                   const int32_t* __restrict__ idx, const long long* ctr, int64_t cap, Dims d, float capv, float thr,
                    double* __restrict__ partials) {
   __shared__ double red[4];
+  const int64_t n = work_count(ctr, cap);
   const OccLayout OL = OccLayout::of(d.nx, d.ny, d.nz);
   double outneg = 0.0;
   double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t node = idx[i];
-    const float v = fminf(fmaxf(phi[node], -cap), cap);
+    const float v = fminf(fmaxf(phi[node], -capv), capv);
     phi[node] = v;
     c1[node] = v;
     c2[node] = v;
@@ -340,6 +375,12 @@ using namespace svimx;
 
 extern "C" {
 
+// Sequence of one reinitialisation, all on `stream` with a single host
+// synchronisation at the end: band bits -> dilation -> work list (count on the
+// device) -> negatives of the old band and of the work set -> geometry and the
+// plain / crossing split -> 2 x iterations sweeps -> rebuild over the work set
+// -> new band (order-preserving compaction of the work list) -> one readback of
+// the counts and extents.
 int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, float* occupancy, int32_t* band,
                  int64_t nb, int64_t inside_outside, int nx, int ny, int nz, double h, double width, int iterations,
                  double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b, int32_t* work_idx,
@@ -351,7 +392,8 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
                     info,
                 SVIMX_EBADARG, "null reinit buffer");
   SVIMX_REQUIRE(phi != scratch && phi != phi_copy && scratch != phi_copy, SVIMX_EBADARG, "aliased reinit buffers");
-  SVIMX_REQUIRE(iterations >= 0 && dilation >= 0 && margin >= 0, SVIMX_EBADARG, "bad reinit parameters");
+  SVIMX_REQUIRE(iterations >= 0 && dilation >= 0 && margin >= 0 && work_cap >= 0, SVIMX_EBADARG,
+                "bad reinit parameters");
   Workspace w;
   if (!carve_workspace(ws, ws_bytes, n, w)) return SVIMX_EBADARG;
   hipStream_t st = as_stream(stream);
@@ -374,7 +416,7 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   hipLaunchKernelGGL(k_bits_fill, gwd, tb, 0, st, ba, nwords);
   SVIMX_CHECK_LAUNCH();
   int nblk = reduce_grid(nb);
-  hipLaunchKernelGGL(k_count_neg, dim3(nblk), tb, 0, st, phi, band, nb, w.partials, 0);
+  hipLaunchKernelGGL(k_count_neg, dim3(nblk), tb, 0, st, phi, band, nb, nullptr, (int64_t)0, w.partials, 0);
   SVIMX_CHECK_LAUNCH();
   one_slot(0, 0, kRsOldNeg);
   int rc = launch_finalize(st, w.partials, nblk, w.results, spec);
@@ -390,36 +432,40 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   SVIMX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_bits_dilate_xy, gwd, tb, 0, st, ba, bb, nx, ny, wpl, nz, 1, dilation, margin);
   SVIMX_CHECK_LAUNCH();
-  int64_t nw = 0;
-  rc = compact_bits(st, bb, nwords, wpl, nz, work_idx, work_cap, w, &nw);
-  info[9] = nw;
+  const int64_t* work_total = nullptr;
+  rc = compact_bits(st, bb, nwords, wpl, nz, work_idx, work_cap, w, &work_total);
   if (rc) return rc;
+  long long* ctr = w.counters;
+  hipLaunchKernelGGL(k_reinit_counters, dim3(1), dim3(64), 0, st, ctr, work_total);
+  SVIMX_CHECK_LAUNCH();
   const float capf = (float)(width * h);
   const float thr = (float)(width * h * (1.0 - 1e-5));
-  nblk = reduce_grid(nw);
-  hipLaunchKernelGGL(k_count_neg, dim3(nblk), tb, 0, st, phi, work_idx, nw, w.partials, 1);
+  nblk = reduce_grid(work_cap);
+  hipLaunchKernelGGL(k_count_neg, dim3(nblk), tb, 0, st, phi, work_idx, (int64_t)0, ctr, work_cap, w.partials, 1);
   SVIMX_CHECK_LAUNCH();
   one_slot(1, 0, kRsWorkNeg);
   rc = launch_finalize(st, w.partials, nblk, w.results, spec);
   if (rc) return rc;
-  if (nw > 0) {
-    const int gwb = (int)((blocks_for(nw, 256 * kSweepNodes) + kXcds - 1) / kXcds * kXcds);
-    const dim3 gw(blocks_for(nw, 256));
-    hipLaunchKernelGGL(k_reinit_geo, gw, tb, 0, st, phi, work_idx, nw, d, (float)h, (float)cfl,
-                       reinterpret_cast<uint8_t*>(work_info), geo);
+  if (work_cap > 0) {
+    uint8_t* info_b = reinterpret_cast<uint8_t*>(work_info);
+    hipLaunchKernelGGL(k_reinit_geo, dim3(blocks_for(work_cap, 256)), tb, 0, st, phi, work_idx, ctr, work_cap, d,
+                       (float)h, (float)cfl, info_b, geo);
     SVIMX_CHECK_LAUNCH();
     const float dtau0 = (float)cfl * fmaxf((float)h, 1e-3f * (float)h);
+    // blocks for the largest work set, kXcds times the per-XCD share (idle ones exit)
+    const int64_t need = (work_cap + 256 * kSweepNodes - 1) / (256 * kSweepNodes);
+    const int gs = (int)(((need + kXcds - 1) / kXcds) * kXcds);
     for (int it = 0; it < iterations; ++it) {
-      hipLaunchKernelGGL(k_reinit_sweep, dim3(gwb), tb, 0, st, 0, phi, scratch, work_idx, nw, d, (float)h, dtau0,
-                         reinterpret_cast<const uint8_t*>(work_info), geo);
+      hipLaunchKernelGGL(k_reinit_sweep, dim3(gs), tb, 0, st, 0, phi, scratch, work_idx, ctr, work_cap, d, (float)h,
+                         dtau0, info_b, geo);
       SVIMX_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_reinit_sweep, dim3(gwb), tb, 0, st, 1, scratch, phi, work_idx, nw, d, (float)h, dtau0,
-                         reinterpret_cast<const uint8_t*>(work_info), geo);
+      hipLaunchKernelGGL(k_reinit_sweep, dim3(gs), tb, 0, st, 1, scratch, phi, work_idx, ctr, work_cap, d, (float)h,
+                         dtau0, info_b, geo);
       SVIMX_CHECK_LAUNCH();
     }
   }
-  hipLaunchKernelGGL(k_rebuild_work, dim3(nblk), tb, 0, st, phi, scratch, phi_copy, occupancy, work_idx, nw, d, capf,
-                     thr, w.partials);
+  hipLaunchKernelGGL(k_rebuild_work, dim3(nblk), tb, 0, st, phi, scratch, phi_copy, occupancy, work_idx, ctr,
+                     work_cap, d, capf, thr, w.partials);
   SVIMX_CHECK_LAUNCH();
   for (int s = 0; s < kPartialSlots; ++s) spec.op[s] = -1, spec.dst[s] = 0;
   spec.op[2] = 0; spec.dst[2] = kRsOutNeg;
@@ -429,13 +475,22 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   }
   rc = launch_finalize(st, w.partials, nblk, w.results, spec);
   if (rc) return rc;
-  int64_t nbn = 0;
-  rc = compact(st, WorkBandPred{phi, work_idx, thr}, nw, band, n, w, &nbn, ListMap{work_idx});
-  info[0] = nbn;
+  const int64_t* band_count = nullptr;
+  const int64_t* work_ctr = reinterpret_cast<const int64_t*>(ctr + kCtrWork);
+  rc = compact_async(st, WorkBandPred{phi, work_idx, thr}, work_ctr, work_cap, work_ctr, work_cap, band, n, w,
+                     &band_count, ListMap{work_idx});
   if (rc) return rc;
   double r[9];
+  int64_t counts[2];
   SVIMX_CHECK_HIP(hipMemcpyAsync(r, w.results + kRsOldNeg, sizeof(r), hipMemcpyDeviceToHost, st));
+  SVIMX_CHECK_HIP(hipMemcpyAsync(&counts[0], work_ctr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SVIMX_CHECK_HIP(hipMemcpyAsync(&counts[1], band_count, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SVIMX_CHECK_HIP(hipStreamSynchronize(st));
+  const int64_t nw = counts[0], nbn = counts[1];
+  info[9] = nw;
+  SVIMX_REQUIRE(nw <= work_cap, SVIMX_ECAPACITY, "reinitialisation work set needs %lld entries, capacity %lld",
+                (long long)nw, (long long)work_cap);
+  info[0] = nbn;
   // inside nodes outside the band: (all negatives before) - (work negatives before) + (new outside-band negatives)
   info[1] = inside_outside + (int64_t)r[0] - (int64_t)r[1] + (int64_t)r[2];
   const int dims[3] = {nx, ny, nz};

@@ -42,6 +42,15 @@ __global__ void __launch_bounds__(kBandThreads)
                      gridDim.x);
 }
 
+template <bool SVI, int MODE>
+__global__ void __launch_bounds__(kBandThreads)
+    k_velocity_rk(const float* __restrict__ phi, const int32_t* __restrict__ band, int64_t nb, Dims d, float h,
+                  LawF L, const float* __restrict__ field, const float* __restrict__ k12, float cut_lo, float cut_hi,
+                  Hi3 hi, int stats, double* __restrict__ partials, RkFuse rk) {
+  velocity_body<SVI, MODE>(phi, band, nb, d, h, L, field, k12, cut_lo, cut_hi, hi, stats, nullptr, partials,
+                           blockIdx.x, gridDim.x, rk);
+}
+
 __global__ void __launch_bounds__(kBandThreads)
     k_rk_stage(int stage, const float* __restrict__ in, const float* __restrict__ u0, float* __restrict__ out,
                const int32_t* __restrict__ band, int64_t nb, Dims d, float inv_h, float dt,
@@ -169,6 +178,17 @@ __global__ void k_annotate_points(LawF L, const float* __restrict__ field, const
   }
 }
 
+// The folds of the velocity statistics (partial slot s -> result
+// SVIMX_R_VMAX + s): max, min, max, max, sum, sum, sum, max, then the same over
+// every evaluated node.
+ReduceSpec velocity_stats_spec() {
+  ReduceSpec spec;
+  const int8_t ops[15] = {1, 2, 1, 1, 0, 0, 0, 1, 2, 1, 1, 0, 0, 0, 1};
+  for (int s = 0; s < kPartialSlots; ++s) spec.op[s] = -1, spec.dst[s] = 0;
+  for (int s = 0; s < 15; ++s) spec.op[s] = ops[s], spec.dst[s] = (int8_t)(SVIMX_R_VMAX + s);
+  return spec;
+}
+
 // Velocity pass + (with stats) the finalisation of its 15 statistic slots into
 // results[SVIMX_R_VMAX ..]: one kernel over the band list, then one fold of the
 // block partials.  With statistics the kernel runs even for an empty band (one
@@ -193,10 +213,7 @@ int velocity_launch(hipStream_t st, const float* phi, const int32_t* band, int64
     SVIMX_CHECK_LAUNCH();
   }
   if (!stats) return SVIMX_OK;
-  ReduceSpec spec;
-  const int8_t ops[15] = {1, 2, 1, 1, 0, 0, 0, 1, 2, 1, 1, 0, 0, 0, 1};
-  for (int s = 0; s < kPartialSlots; ++s) spec.op[s] = -1, spec.dst[s] = 0;
-  for (int s = 0; s < 15; ++s) spec.op[s] = ops[s], spec.dst[s] = (int8_t)(SVIMX_R_VMAX + s);
+  const ReduceSpec spec = velocity_stats_spec();
   return launch_finalize(st, partials, nblk, results, spec);
 }
 
@@ -301,6 +318,50 @@ int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t 
   if (!carve_workspace(ws, ws_bytes, Dims{nx, ny, nz}.size(), w)) return SVIMX_EBADARG;
   return velocity_launch(as_stream(stream), phi, band, nb, Dims{nx, ny, nz}, h, law, field, k12, cut_lo, cut_hi,
                          stats, V, w.partials, w.results);
+  SVIMX_TRY_END
+}
+
+int svimx_velocity_rk(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                      const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const svimx_law_t* law,
+                      const float* field, const float* k12, double cut_lo, double cut_hi, double dt, float* L0,
+                      void* ws, int64_t ws_bytes) {
+  SVIMX_TRY_BEGIN
+  Dims d{nx, ny, nz};
+  Workspace w;
+  if (!carve_workspace(ws, ws_bytes, d.size(), w)) return SVIMX_EBADARG;
+  SVIMX_REQUIRE(stage >= 0 && stage <= 2, SVIMX_EBADARG, "svimx_velocity_rk: stage must be 0, 1 or 2");
+  SVIMX_REQUIRE(law && law->kind >= 0 && law->kind <= 2, SVIMX_EBADARG, "bad law");
+  SVIMX_REQUIRE(law->kind != 0 || field, SVIMX_EBADARG, "SVI law needs the N_r field");
+  SVIMX_REQUIRE(law->kind == 0 || k12, SVIMX_EBADARG, "curvature law needs the (k1, k2) field");
+  SVIMX_REQUIRE(stage != 0 || L0, SVIMX_EBADARG, "stage 0 stores L0");
+  SVIMX_REQUIRE(stage == 0 || (u0 && phi_out && phi_out != phi_in && phi_out != u0), SVIMX_EBADARG,
+                "stages 1 and 2 need u0 and an output buffer distinct from the inputs");
+  hipStream_t st = as_stream(stream);
+  const int stats = stage == 0 ? 1 : 0;
+  const int nblk = reduce_grid(nb);
+  if (nb > 0 || stats) {
+    RkFuse rk;
+    rk.u0 = u0;
+    rk.out = phi_out;
+    rk.L0 = L0;
+    rk.inv_h = (float)(1.0 / h);
+    rk.dt = (float)dt;
+    const LawF lf = law_to_f(*law);
+    const float clo = (float)(cut_lo * h), chi = (float)(cut_hi * h);
+#define SVIMX_VRK(SVI_, MODE_)                                                                                   \
+  hipLaunchKernelGGL((k_velocity_rk<SVI_, MODE_>), dim3(nblk), dim3(kBandThreads), 0, st, phi_in, band, nb, d,   \
+                     (float)h, lf, field, k12, clo, chi, hi_of(d), stats, w.partials, rk)
+    if (law->kind == 0) {
+      if (stage == 0) SVIMX_VRK(true, 1); else if (stage == 1) SVIMX_VRK(true, 2); else SVIMX_VRK(true, 3);
+    } else {
+      if (stage == 0) SVIMX_VRK(false, 1); else if (stage == 1) SVIMX_VRK(false, 2); else SVIMX_VRK(false, 3);
+    }
+#undef SVIMX_VRK
+    SVIMX_CHECK_LAUNCH();
+  }
+  if (!stats) return SVIMX_OK;
+  const ReduceSpec spec = velocity_stats_spec();
+  return launch_finalize(st, w.partials, nblk, w.results, spec);
   SVIMX_TRY_END
 }
 

@@ -334,17 +334,20 @@ int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, i
   SVIMX_TRY_END
 }
 
-int svimx_free(void* p) {
-  // the arrays come from the stream-ordered allocator; the caller has copied
-  // them out (synchronously), so a free ordered on the null stream is safe
-  if (p) SVIMX_CHECK_HIP(hipFreeAsync(p, nullptr));
+// Both are ordered on the caller's stream, never on the legacy null stream:
+// another thread may be capturing a HIP graph on its own stream meanwhile (the
+// batch engine), and null-stream work is the kind of cross-stream dependency
+// that can invalidate a capture.
+int svimx_free(void* stream, void* p) {
+  if (p) SVIMX_CHECK_HIP(hipFreeAsync(p, as_stream(stream)));
   return SVIMX_OK;
 }
 
-int svimx_copy_to_host(void* dst, const void* src, int64_t bytes) {
+int svimx_copy_to_host(void* stream, void* dst, const void* src, int64_t bytes) {
   if (bytes <= 0) return SVIMX_OK;
   SVIMX_REQUIRE(dst && src, SVIMX_EBADARG, "null argument");
-  SVIMX_CHECK_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  SVIMX_CHECK_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+  SVIMX_CHECK_HIP(hipStreamSynchronize(as_stream(stream)));
   return SVIMX_OK;
 }
 

@@ -1,12 +1,16 @@
 """Time integration of the erosion flows on the device (API of svierosion/evolve.py).
 
-Each RHS evaluation (one TVD-RK3 stage) is four libsvimx launches over the
+Each RHS evaluation (one TVD-RK3 stage) is three libsvimx launches over the
 band plus the N_r summation over the marked tiles:
 
     fractions (+ N_r tile marks)   (SVI laws)
     N_r direct summation           (SVI laws)   | curvature scatter (curvature laws)
-    velocity (+ dt statistics on stage 0)
-    RK stage (Godunov/WENO5, L = V |grad phi|, stage combination)
+    velocity fused with the RK stage (Godunov/WENO5, L = V |grad phi|, stage
+    combination); on stage 0 it stores L0 with the dt statistics, and once the
+    host has dt the update u0 + dt L0 is a short streaming pass
+
+(``fused = False`` runs the velocity and the RK stage as two passes: the same
+floats, bit-identical fields.)
 
 Buffers: the step-start field u0 stays in one dense copy (A); stage outputs go
 to B, C, B, so a secant retry restarts from A without a copy and replays
@@ -90,6 +94,10 @@ class ErosionSimulation:
             self.k12 = torch.zeros(2 * n, dtype=torch.float32, device=dev)
         self._alloc_band_arrays()
         self._frac_valid = None     # buffer index whose fractions/marks are current
+        # run(): the velocity pass fused with the RK stage that consumes it
+        # (svimx_velocity_rk); False runs svimx_velocity + svimx_rk_stage, the
+        # same floats in two passes
+        self.fused = True
         self.tuning = {}
         if self.conv is not None and autotune and levelset.nb > 0:
             # pick the N_r z-run length on this job's field (results do not depend on it)
@@ -146,8 +154,9 @@ class ErosionSimulation:
     def _plan(self):
         return self.conv.plan if self.conv is not None else None
 
-    def _prepare(self, buf: int, stats: bool) -> None:
-        """Fractions, N_r / curvatures and velocity for the field in bufs[buf]."""
+    def _inputs(self, buf: int) -> None:
+        """What the velocity of the field in bufs[buf] reads: fractions and N_r
+        (SVI laws) or the dense curvature pairs (curvature laws)."""
         ls = self.ls
         st = _abi.stream()
         saved = ls.cur
@@ -159,7 +168,6 @@ class ErosionSimulation:
                 with self._span("fractions"):
                     ls.fractions(self._plan(), self.cutoff[1])
                 self._frac_valid = buf
-            phi = ls.flat
             if self.conv is not None:
                 with self._span("nr_select"):
                     _abi.check(_abi.lib.svimx_nr_select(st, self.conv.plan), "N_r select")
@@ -168,12 +176,41 @@ class ErosionSimulation:
                                                      _abi.ptr(ls.ws), ls.ws.numel()), "N_r")
             else:
                 with self._span("curvatures"):
-                    _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(phi), _abi.ptr(ls._band), ls.nb, *ls.dims,
+                    _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(ls.flat), _abi.ptr(ls._band), ls.nb, *ls.dims,
                                                          ls.h, _abi.ptr(self.k12)))
-            with self._span("velocity"):
-                self._velocity_call(st, phi, stats)
         finally:
             ls.cur = saved
+
+    def _prepare(self, buf: int, stats: bool) -> None:
+        """Fractions, N_r / curvatures and velocity (into self.V) for the field in bufs[buf]."""
+        self._inputs(buf)
+        with self._span("velocity"):
+            self._velocity_call(_abi.stream(), self.ls.bufs[buf], stats)
+
+    def _velocity_rk(self, stage: int, src: int, u0: int, dst: int, dt: float) -> None:
+        """Fused velocity + RK stage on bufs[src] (inputs already prepared): stage 0
+        stores L0 and the statistics, stages 1 and 2 write bufs[dst]."""
+        ls = self.ls
+        with self._span("velocity_rk"):
+            _abi.check(_abi.lib.svimx_velocity_rk(_abi.stream(), stage, _abi.ptr(ls.bufs[src]), _abi.ptr(ls.bufs[u0]),
+                                                  _abi.ptr(ls.bufs[dst]), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
+                                                  self._law_abi, _abi.ptr(self.field), _abi.ptr(self.k12),
+                                                  self.cutoff[0], self.cutoff[1], float(dt), _abi.ptr(self.L0),
+                                                  _abi.ptr(ls.ws), ls.ws.numel()), "velocity + rk stage")
+        if stage > 0 and self._frac_valid == dst:
+            self._frac_valid = None
+
+    def _stage0(self, a: int) -> None:
+        """Velocity with the dt statistics on bufs[a] (and L0 when fused)."""
+        if self.fused:
+            self._inputs(a)
+            self._velocity_rk(0, a, a, a, 0.0)
+        else:
+            self._prepare(a, True)
+
+    def _stage0_update(self, a: int, b: int, dt: float) -> None:
+        """bufs[b] = u0 + dt L0 (the fused stage 0 left L0; the replay is the same update)."""
+        self._stage(3 if self.fused else 0, a, a, b, dt)
 
     def _velocity_call(self, st, phi, stats: bool) -> None:
         ls = self.ls
@@ -197,6 +234,12 @@ class ErosionSimulation:
 
     def _rk3_tail(self, a: int, b: int, c: int, dt: float) -> None:
         """Stages 1 and 2 after stage 0 wrote bufs[b]; the result lands in bufs[b]."""
+        if self.fused:
+            self._inputs(b)
+            self._velocity_rk(1, b, a, c, dt)
+            self._inputs(c)
+            self._velocity_rk(2, c, a, b, dt)
+            return
         self._prepare(b, False)
         self._stage(1, b, a, c, dt)
         self._prepare(c, False)
@@ -284,7 +327,7 @@ class ErosionSimulation:
             t_wall = _time.time()
             a = ls.cur
             b, c = (a + 1) % 3, (a + 2) % 3
-            self._prepare(a, True)
+            self._stage0(a)
             vmax, info = self._stats()
             dt = self.stable_dt(vmax, info)
             if max_time is not None:
@@ -292,7 +335,7 @@ class ErosionSimulation:
             if not math.isfinite(dt) or dt <= 0:
                 reason = "stalled"
                 break
-            self._stage(0, a, a, b, dt)
+            self._stage0_update(a, b, dt)
             self._rk3_tail(a, b, c, dt)
             vnew, res = self._volume_of(b)
             if target is not None and vnew < target:

@@ -57,17 +57,18 @@ def device_mesh(phi_flat: torch.Tensor, band: torch.Tensor, nb: int, dims, max_s
     """(vertices in index coordinates float32 (V, 3), faces int32 (F, 3)) of the zero set over ``band``."""
     xyz, faces = C.c_void_p(), C.c_void_p()
     nv, nf = C.c_int64(), C.c_int64()
+    st = _abi.stream()
     _abi.check(_abi.lib.svimx_extract_surface(
-        _abi.stream(), _abi.ptr(phi_flat), _abi.ptr(band), int(nb), *map(int, dims), float(max_shift),
+        st, _abi.ptr(phi_flat), _abi.ptr(band), int(nb), *map(int, dims), float(max_shift),
         C.byref(xyz), C.byref(nv), C.byref(faces), C.byref(nf)), "svimx_extract_surface")
     try:
         v = np.empty((nv.value, 3), dtype=np.float32)
         f = np.empty((nf.value, 3), dtype=np.int32)
-        _abi.check(_abi.lib.svimx_copy_to_host(v.ctypes.data_as(C.c_void_p), xyz, v.nbytes))
-        _abi.check(_abi.lib.svimx_copy_to_host(f.ctypes.data_as(C.c_void_p), faces, f.nbytes))
+        _abi.check(_abi.lib.svimx_copy_to_host(st, v.ctypes.data_as(C.c_void_p), xyz, v.nbytes))
+        _abi.check(_abi.lib.svimx_copy_to_host(st, f.ctypes.data_as(C.c_void_p), faces, f.nbytes))
     finally:
-        _abi.lib.svimx_free(xyz)
-        _abi.lib.svimx_free(faces)
+        _abi.lib.svimx_free(st, xyz)
+        _abi.lib.svimx_free(st, faces)
     return v, f
 
 

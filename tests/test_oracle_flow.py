@@ -290,3 +290,33 @@ def test_cube_res64_nr_closed_forms():
         err = np.abs(val - exact).max()
         print(f"oracle cube N_r at {len(P)} {name} points: |N - {exact}| <= {err:.2e}")
         assert err < CUBE_NR_BOUNDS[name]
+
+
+# ------------------------------------------------------------------ the C++ CPU restatement
+def test_cpu_port_history_matches_the_numpy_oracle():
+    """oracle/cpu/svicpu.cpp (the timed CPU baseline) against the NumPy oracle on the
+    sphere, SVI k=1 s=1 t=0.93, 8 steps with two reinitialisations: the same
+    reinitialisation steps, fractions within 1e-4 and dt within 1e-3 relative (the
+    GPU history bounds); reinitialisation alone within 1e-5 h."""
+    import ctypes as C
+    from oracle import cpu_port
+    r, h = 0.3, 0.05
+    phi, origin, h, shape = sphere_phi(1.0, h, r + 10 * h)
+    law = F.SVILaw(r, 1, 1, 0.93)
+    _, rows, threads = cpu_port.run_steps(phi.astype(np.float32), h, law, 8)
+    ref = F.Simulation(F.LevelSet(phi.astype(np.float32), shape, h), law)
+    _, _, _, _, hist = ref.run(target_fraction=None, max_steps=8)
+    ef = max(abs(vol / ref.initial_volume - b.fraction) for (_, vol, _), b in zip(rows, hist))
+    ed = max(abs(dt - b.dt) / b.dt for (dt, _, _), b in zip(rows, hist))
+    print(f"C++ port ({threads} threads) vs NumPy oracle over 8 steps: fraction {ef:.2e}, dt {ed:.2e} relative")
+    assert [re for _, _, re in rows] == [b.reinit for b in hist] and sum(b.reinit for b in hist) == 2
+    assert ef < 1e-4 and ed < 1e-3
+    lib = cpu_port._load()
+    lib.svicpu_reinit.restype = C.c_int64
+    lib.svicpu_reinit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
+    d, _, h2, shp = sphere_phi(1.0, 0.08, 12 * 0.08, distort=True)
+    a = F.LevelSet(d.astype(np.float32), shp, h2)
+    a.reinitialize(20)
+    b = np.ascontiguousarray(d.astype(np.float32))
+    assert lib.svicpu_reinit(b.ctypes.data, *shp, h2, 6.0, 20) == a.band.size
+    assert np.abs(b.reshape(-1) - a.flat).max() < 1e-5 * h2

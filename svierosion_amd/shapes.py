@@ -1,8 +1,11 @@
 """Seeded test solids for the erosion workloads.
 
 * ``box`` / ``cube``       -- axis-aligned box as 12 outward triangles;
+* ``triangular_prism``, ``l_block``, ``cylinder``, ``tube_fragment``
+                           -- straight prisms over a planar outline whose cap
+                              triangulation is given with it;
 * ``icosphere``            -- subdivided icosahedron projected to a sphere
-                              (20 * 4**s triangles);
+                              (20 * 4**s triangles); ``ellipsoid`` scales it;
 * ``torus``                -- quad-strip torus split into triangles;
 * ``sh_fragment``          -- icosphere with a radial displacement by a sum of
                               real spherical harmonics (the 81,920-triangle
@@ -12,6 +15,8 @@
                               1.3M-triangle scan at level 8).
 
 All generators are deterministic given their seed (numpy default_rng).
+``SHAPES`` names the toy solids of the ``shapes`` command; ``sphere_sdf`` and
+``box_sdf`` are exact signed distances for tests.
 """
 
 from __future__ import annotations
@@ -43,6 +48,86 @@ def box(size=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0)) -> TriMesh:
 
 def cube(edge: float = 1.0, center=(0.0, 0.0, 0.0)) -> TriMesh:
     return box((edge, edge, edge), center)
+
+
+def _prism(outline, cap, height: float) -> TriMesh:
+    """Closed prism over a planar outline, |z| <= height/2.
+
+    ``outline`` (n, 2) is the boundary loop and ``cap`` (m, 3) a triangulation
+    of it by outline indices; both may come in either orientation, the walls
+    and caps are oriented outward here (svierosion/shapes.py:58-78 builds the
+    same solids, triangulating the cap by ear clipping)."""
+    p = np.asarray(outline, dtype=np.float64)
+    cap = np.asarray(cap, dtype=np.int64)
+    n = len(p)
+    x, y = p[:, 0], p[:, 1]
+    if np.dot(x, np.roll(y, -1)) - np.dot(np.roll(x, -1), y) < 0:      # clockwise: reverse the loop
+        p = p[::-1]
+        cap = n - 1 - cap
+    a, b, c = p[cap[:, 0]], p[cap[:, 1]], p[cap[:, 2]]
+    ccw = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]) > 0
+    cap = np.where(ccw[:, None], cap, cap[:, ::-1])
+    z = 0.5 * height
+    verts = np.vstack([np.column_stack([p, np.full(n, -z)]), np.column_stack([p, np.full(n, z)])])
+    i = np.arange(n)
+    j = (i + 1) % n
+    faces = np.vstack([cap[:, ::-1], cap + n, np.column_stack([i, j, j + n]), np.column_stack([i, j + n, i + n])])
+    return _outward(TriMesh(verts, faces))
+
+
+def _fan(n: int) -> np.ndarray:
+    """Triangulation of a convex n-gon from its first corner."""
+    k = np.arange(1, n - 1)
+    return np.column_stack([np.zeros(n - 2, dtype=np.int64), k, k + 1])
+
+
+def triangular_prism(side: float = 1.0, length: float = 1.0) -> TriMesh:
+    """Prism along z over an equilateral triangle of edge ``side`` centred on the axis,
+    one corner on +y (svierosion/shapes.py:101-105)."""
+    ang = np.pi / 2 + np.array([0.0, 2.0, 4.0]) * np.pi / 3
+    r = side / np.sqrt(3.0)
+    return _prism(np.column_stack([r * np.cos(ang), r * np.sin(ang)]), _fan(3), length)
+
+
+def l_block(a: float = 1.0, b: float = 0.5, thickness: float = 0.5) -> TriMesh:
+    """An a x a square with its (a-b) x (a-b) corner at +x+y removed, extruded to
+    ``thickness`` (svierosion/shapes.py:108-116): one re-entrant edge."""
+    h = a / 2
+    outline = [(-h, -h), (h, -h), (h, b - h), (b - h, b - h), (b - h, h), (-h, h)]
+    # two rectangles sharing the corner square at (-h, -h): [0 1 2 3] and [0 3 4 5]
+    cap = [(0, 1, 2), (0, 2, 3), (0, 3, 4), (0, 4, 5)]
+    return _prism(outline, cap, thickness)
+
+
+def cylinder(radius: float = 0.5, height: float = 1.0, segments: int = 256) -> TriMesh:
+    """Regular ``segments``-gon prism inscribed in the cylinder (svierosion/shapes.py:119-121)."""
+    t = 2 * np.pi * np.arange(segments) / segments
+    return _prism(np.column_stack([radius * np.cos(t), radius * np.sin(t)]), _fan(segments), height)
+
+
+def tube_fragment(r_in: float = 0.6, r_out: float = 1.0, angle: float = 2.2,
+                  length: float = 1.6, segments: int = 160) -> TriMesh:
+    """Thick-walled tube sector r_in <= rho <= r_out, |theta| <= angle/2, |z| <= length/2,
+    shifted by -(r_in + r_out)/2 in x (svierosion/shapes.py:124-138): convex outer wall,
+    concave inner wall and four break edges.  The cap is the strip of quads between
+    the two arcs."""
+    t = np.linspace(-angle / 2, angle / 2, segments + 1)
+    outer = np.column_stack([r_out * np.cos(t), r_out * np.sin(t)])
+    inner = np.column_stack([r_in * np.cos(t), r_in * np.sin(t)])[::-1]
+    m = segments + 1
+    k = np.arange(segments)
+    o0, o1 = k, k + 1                                  # outer arc, increasing angle
+    i0, i1 = 2 * m - 1 - k, 2 * m - 2 - k              # inner arc at the same angles
+    cap = np.vstack([np.column_stack([o0, o1, i1]), np.column_stack([o0, i1, i0])])
+    mesh = _prism(np.vstack([outer, inner]), cap, length)
+    mesh.vertices[:, 0] -= 0.5 * (r_in + r_out)
+    return mesh
+
+
+def ellipsoid(radii=(1.0, 0.7, 0.5), subdivisions: int = 5) -> TriMesh:
+    """Icosphere scaled by the semi-axes (svierosion/shapes.py:167-170)."""
+    m = icosphere(subdivisions, 1.0)
+    return _outward(TriMesh(m.vertices * np.asarray(radii, dtype=np.float64), m.faces))
 
 
 def icosphere(subdivisions: int = 5, radius: float = 1.0) -> TriMesh:
@@ -157,3 +242,35 @@ def noisy_ellipsoid(level: int = 8, axes=(60.0, 45.0, 30.0), amplitudes=(3.0, 1.
         disp += amp * value_noise(dirs, rng, base_frequency * 2 ** o)
     r = np.linalg.norm(e, axis=1)
     return _outward(TriMesh(e * ((r + disp) / r)[:, None], base.faces))
+
+
+# The toy solids written by ``python -m svierosion_amd shapes`` (names and sizes of
+# svierosion/shapes.py:241-252).
+SHAPES = {
+    "cube": lambda: cube(1.0),
+    "box": lambda: box((1.6, 1.0, 0.6)),
+    "prism": lambda: triangular_prism(1.2, 1.2),
+    "ellipsoid": lambda: ellipsoid((0.8, 0.55, 0.4), 6),
+    "sphere": lambda: icosphere(6, 0.6),
+    "lblock": lambda: l_block(1.2, 0.6, 0.6),
+    "cylinder": lambda: cylinder(0.45, 1.2),
+    "torus": lambda: torus(0.6, 0.25, 192, 96),
+    "tube_fragment": lambda: tube_fragment(),
+}
+
+
+def sphere_sdf(points, center=(0.0, 0.0, 0.0), radius: float = 1.0) -> np.ndarray:
+    """Exact signed distance to a sphere, negative inside (svierosion/shapes.py:255-257)."""
+    p = np.asarray(points, dtype=np.float64) - np.asarray(center, dtype=np.float64)
+    return np.linalg.norm(p, axis=-1) - radius
+
+
+def box_sdf(points, size=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Exact signed distance to an axis-aligned box, negative inside (svierosion/shapes.py:260-265):
+    outside, the length of the excess over the half sizes; inside, minus the distance
+    to the nearest face."""
+    q = np.abs(np.asarray(points, dtype=np.float64) - np.asarray(center, dtype=np.float64)) \
+        - 0.5 * np.asarray(size, dtype=np.float64)
+    outside = np.linalg.norm(np.maximum(q, 0.0), axis=-1)
+    inside = np.minimum(q.max(axis=-1), 0.0)
+    return outside + inside

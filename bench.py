@@ -123,7 +123,7 @@ def _same_history(a, b) -> bool:
                                     for x, y in zip(a, b))
 
 
-def batched_leg(n_jobs: int, workers: int):
+def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, slack: float):
     """The fragment mix (first n_jobs of the 64-fragment set) eroded batched and one at a time."""
     from svierosion_amd.batch import Request, serve, serve_sequential
     from svierosion_amd.workloads import fragment_mix
@@ -131,7 +131,7 @@ def batched_leg(n_jobs: int, workers: int):
     jobs = fragment_mix(n_jobs)
     reqs = [Request(m, p) for m, p in jobs]
     gen_s = time.time() - t0
-    rb = serve(reqs, workers=workers)
+    rb = serve(reqs, workers=workers, schedule=schedule, budget_nodes=budget_nodes, slack=slack)
     print(f"[bench] batched {n_jobs} jobs: {rb.summary()}", file=sys.stderr, flush=True)
     rs = serve_sequential(reqs)
     print(f"[bench] sequential {n_jobs} jobs: {rs.summary()}", file=sys.stderr, flush=True)
@@ -144,6 +144,7 @@ def batched_leg(n_jobs: int, workers: int):
     return {"workload": (f"first {n_jobs} of the 64-fragment mix (icosphere level 4-7 + SH displacement, "
                          "res log-uniform 96-320, 50% SVI / 25% mean / 25% Gauss curvature, target 0.9)"),
             "jobs": n_jobs, "laws": laws, "steps_min_max": [min(steps), max(steps)],
+            "schedule": {"policy": schedule, "budget_nodes": budget_nodes, "slack": slack},
             "erosions_per_hour": round(rb.erosions_per_hour(), 1),
             "sequential_erosions_per_hour": round(rs.erosions_per_hour(), 1),
             "speedup": round(rb.erosions_per_hour() / max(rs.erosions_per_hour(), 1e-9), 3),
@@ -152,12 +153,12 @@ def batched_leg(n_jobs: int, workers: int):
             "engine_ticks": rb.ticks, "mesh_generation_s": round(gen_s, 2)}, rs
 
 
-def served_leg(n_req: int, rate: float, workers: int):
+def served_leg(n_req: int, rate: float, workers: int, schedule: str, budget_nodes: int, slack: float):
     """Poisson arrivals of the served workload (first n_req requests), continuous batching vs FIFO."""
     from svierosion_amd.batch import serve, serve_sequential
     from svierosion_amd.workloads import poisson_requests
     reqs = poisson_requests(n_req, rate=rate)
-    rv = serve(reqs, workers=workers)
+    rv = serve(reqs, workers=workers, schedule=schedule, budget_nodes=budget_nodes, slack=slack)
     print(f"[bench] served {n_req} requests at {rate:.4f}/s: {rv.summary()}", file=sys.stderr, flush=True)
     rq = serve_sequential(reqs)
     print(f"[bench] sequential FIFO: {rq.summary()}", file=sys.stderr, flush=True)
@@ -166,6 +167,7 @@ def served_leg(n_req: int, rate: float, workers: int):
     return {"workload": (f"first {n_req} of 512 requests (fragments as in the mix, res log-uniform 64-400, "
                          f"target U[0.85, 0.97]), seeded Poisson arrivals at {rate:.4f}/s"),
             "requests": n_req, "rate_per_s": round(rate, 5),
+            "schedule": {"policy": schedule, "budget_nodes": budget_nodes, "slack": slack},
             "erosions_per_hour": round(rv.erosions_per_hour(), 1),
             "latency_p50_s": round(rv.latency(0.5), 3), "latency_p99_s": round(rv.latency(0.99), 3),
             "sequential_erosions_per_hour": round(rq.erosions_per_hour(), 1),
@@ -264,7 +266,13 @@ def main(argv=None):
     ap.add_argument("--serve-requests", type=int, default=12, help="requests of the served leg (0: skip)")
     ap.add_argument("--serve-load", type=float, default=2.0,
                     help="offered load of the served leg as a multiple of the measured sequential throughput")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="arrivals per second of the served leg (default: --serve-load x the sequential rate)")
     ap.add_argument("--workers", type=int, default=2, help="set-up / meshing threads of the batched forms")
+    ap.add_argument("--batch-schedule", choices=["lockstep", "longest", "fifo"], default="lockstep")
+    ap.add_argument("--serve-schedule", choices=["fifo", "longest", "lockstep"], default="fifo")
+    ap.add_argument("--budget-nodes", type=int, default=None, help="band nodes a tick may hold (schedule budget)")
+    ap.add_argument("--slack", type=float, default=None, help="tick budget over the first job's band, as a fraction")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of this many MiB (profiler byte calibration)")
     ap.add_argument("--full-erosion", choices=["scan400", "fragment256", "cube64"], default=None,
@@ -372,12 +380,15 @@ def main(argv=None):
     torch.cuda.empty_cache()
     batched = served = None
     seq_rate = None
+    from svierosion_amd.batch import DEFAULT_BUDGET_NODES, DEFAULT_SLACK
+    budget = DEFAULT_BUDGET_NODES if args.budget_nodes is None else args.budget_nodes
+    slack = DEFAULT_SLACK if args.slack is None else args.slack
     if args.batch_jobs > 0:
-        batched, rs = batched_leg(args.batch_jobs, args.workers)
+        batched, rs = batched_leg(args.batch_jobs, args.workers, args.batch_schedule, budget, slack)
         seq_rate = len(rs.done) / rs.makespan if rs.makespan > 0 else None
     if args.serve_requests > 0:
-        rate = args.serve_load * (seq_rate if seq_rate else 0.5)
-        served = served_leg(args.serve_requests, rate, args.workers)
+        rate = args.serve_rate or args.serve_load * (seq_rate if seq_rate else 0.5)
+        served = served_leg(args.serve_requests, rate, args.workers, args.serve_schedule, budget, slack)
     line = {"metric": "RK3 steps/s + HBM GB/s @res400 1.3M-tri; batched erosions/hour, p99 latency",
             "value": round(value, 3), "unit": "RK3 steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
             "ms_per_step": round(1000 * elapsed / max(steps, 1), 3), "higher_is_better": True, "scaling": "weak",

@@ -475,14 +475,30 @@ int svimx_engine_add(void* engine, const svimx_job_desc_t* desc, int* job_id);
  * max_steps more steps were taken.  Computes the current volume; synchronises. */
 int svimx_engine_run(void* engine, int job_id, double target_fraction, double max_time, int64_t max_steps,
                      double tol);
-/* One step of every running job; *n_running = jobs still running after it. */
+/* One step of each running job that the schedule selects for this tick
+ * (every running job under SVIMX_SCHED_LOCKSTEP); *n_running = jobs still
+ * running after it, selected or not. */
 int svimx_engine_tick(void* engine, int* n_running);
+/* Which running jobs a tick steps.  The reference runs one erosion at a time
+ * (svierosion/compare.py:36-85); this has no counterpart there.
+ *   SVIMX_SCHED_LOCKSTEP: all of them (every job advances one step per tick).
+ *   SVIMX_SCHED_FIFO:     jobs in the order they were added ...
+ *   SVIMX_SCHED_LONGEST:  ... or by most estimated steps left first (from each
+ *                         job's volume loss per step since its run started;
+ *                         jobs with fewer than 4 steps come first),
+ * taken while the selected band nodes stay within max(budget_nodes,
+ * (1 + slack) * band nodes of the first job): the first job is always
+ * stepped, a job that does not fit is passed over for this tick.  A job's
+ * results do not depend on the schedule. */
+enum { SVIMX_SCHED_LOCKSTEP = 0, SVIMX_SCHED_FIFO = 1, SVIMX_SCHED_LONGEST = 2 };
+int svimx_engine_schedule(void* engine, int policy, int64_t budget_nodes, double slack);
 int svimx_engine_status(void* engine, int job_id, svimx_job_status_t* out);
 /* Copy step records [first, first + n) of job_id (records accumulate over runs). */
 int svimx_engine_records(void* engine, int job_id, int64_t first, int64_t n, svimx_step_record_t* out);
 /* Forget job_id (its buffers are the caller's again); frees the engine-owned arrays. */
 int svimx_engine_remove(void* engine, int job_id);
-/* Capture the main tick as a HIP graph and replay it (1) or launch eagerly (0). */
+/* Capture the main tick as a HIP graph and replay it (1) or launch eagerly (0).
+ * Graphs are kept for the last few job sets a tick stepped. */
 int svimx_engine_use_graphs(void* engine, int on);
 /* Ids (up to cap) of jobs that stopped since the last call, oldest first. */
 int svimx_engine_stopped(void* engine, int* ids, int cap, int* n);

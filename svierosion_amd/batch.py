@@ -11,8 +11,19 @@ the batch engine (svierosion_amd.engine):
   each admitted job (mesh SDF, level set, N_r plan: ``prepare_job``) and
   mesh and annotate each job that reached a target (``finish_target``),
   while the main thread keeps ticking the engine;
-* stepping -- every tick advances all running jobs by one RK3 step; jobs join
-  and leave between ticks, so new requests start while others are running.
+* stepping -- a tick advances the running jobs its schedule selects by one
+  RK3 step each; jobs join and leave between ticks, so new requests start
+  while others are running.  ``schedule="fifo"`` (the default of
+  :func:`serve`) takes jobs in arrival order and fills each tick with later
+  ones while the selected band nodes stay within ``budget_nodes`` (or
+  ``1 + slack`` times the first job's band): the oldest request runs at close
+  to its own speed and small requests run beside it instead of waiting.
+  ``"longest"`` puts the jobs with the most estimated steps left first;
+  ``"lockstep"`` (the default of :func:`erode_batch`) steps every job in
+  every tick, which finishes a batch soonest when one job needs far more
+  steps than the rest: the others run beside it at full concurrency and the
+  long job ends alone on the single-job path (12 jobs of the fragment mix:
+  7.1 s lockstep, 7.8 s longest-first or FIFO).
 
 Each job computes exactly what ``erode`` computes for it alone: the engine
 reproduces the single-job path bit for bit, and set-up and meshing are the
@@ -145,11 +156,20 @@ class _Worker:
         self.pool.shutdown(wait=True)
 
 
+# Tick budget of the served schedule: measured on 12 Poisson requests of the
+# served workload (budgets 1M-4M band nodes, slack 0.25-1.0 all gave p50 and
+# p99 below one-at-a-time FIFO; 2M / 1.0 the lowest p99).
+DEFAULT_BUDGET_NODES = 2_000_000
+DEFAULT_SLACK = 1.0
+
+
 def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, graphs: bool = True,
-          keep_state: bool = False, verbose: bool = False, workers: int = 2, solo_tail: bool = True) -> ServeReport:
+          keep_state: bool = False, verbose: bool = False, workers: int = 2, solo_tail: bool = True,
+          schedule: str = "fifo", budget_nodes: int = DEFAULT_BUDGET_NODES, slack: float = DEFAULT_SLACK) -> ServeReport:
     """Continuous batching of ``requests`` (arrival times in seconds from now).  With
     ``solo_tail`` the last running job finishes on the single-job path once no
-    request is left to admit."""
+    request is left to admit.  ``schedule``, ``budget_nodes`` and ``slack`` select
+    the jobs of each tick (:meth:`BatchEngine.schedule`)."""
     import torch
     from .engine import STOP_NAMES, BatchEngine
     _abi.require_device("cuda")
@@ -159,6 +179,7 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
     order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
     pending = deque(order)
     eng = BatchEngine(max_jobs=max_jobs, graphs=graphs)
+    eng.schedule(schedule, budget_nodes, slack)
     worker = _Worker(workers)
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0  # noqa: E731
@@ -317,11 +338,13 @@ def serve_sequential(requests: list, keep_state: bool = False, verbose: bool = F
 
 
 def erode_batch(meshes: list, params, max_jobs: int = 64, graphs: bool = True, keep_state: bool = False,
-                hbm_budget: float | None = None, workers: int = 2) -> list:
+                hbm_budget: float | None = None, workers: int = 2, schedule: str = "lockstep",
+                budget_nodes: int = DEFAULT_BUDGET_NODES, slack: float = DEFAULT_SLACK) -> list:
     """``erode`` of every mesh (one ErosionParams or one per mesh), batched on one GPU.
 
     Returns ErosionResult per mesh, or the exception a job raised."""
     plist = params if isinstance(params, (list, tuple)) else [params] * len(meshes)
     rep = serve([Request(m, p) for m, p in zip(meshes, plist)], max_jobs=max_jobs, hbm_budget=hbm_budget,
-                graphs=graphs, keep_state=keep_state, workers=workers)
+                graphs=graphs, keep_state=keep_state, workers=workers, schedule=schedule,
+                budget_nodes=budget_nodes, slack=slack)
     return [c.result if c.error is None else RuntimeError(c.error) for c in rep.completions]

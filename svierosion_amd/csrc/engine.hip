@@ -1,5 +1,5 @@
 // Batch engine: many erosion jobs of different grids and laws stepped in
-// lockstep ticks (one accepted TVD-RK3 step per job per tick).
+// ticks (one accepted TVD-RK3 step for each job a tick selects).
 //
 // Device side.  Each job's pointers and scalars live in a JobDev record of a
 // device table.  A launch covers a set of jobs through a segment table Seg:
@@ -14,8 +14,17 @@
 // on the device (svierosion/evolve.py:162-170), so a tick needs one host
 // synchronisation: the gathered result slots of all jobs.
 //
+// Schedule.  Which running jobs a tick steps: all of them (lockstep), or a
+// priority order (the order jobs were added, or most estimated steps left
+// first) filled up to a band-node budget.  Lockstep holds every job to the
+// tick rate of the whole set: a 4,000-step job sharing ticks with large grids
+// advanced at ~0.7 ms per step instead of the ~0.16 ms of its own; the
+// budget keeps ticks near the size that fills the GPU and lets the first
+// job's steps pass at close to their own rate.  Graphs are captured per job
+// set and the last few sets' graphs are kept, as the selection alternates.
+//
 // Host side.  The tick mirrors one iteration of ErosionSimulation.run
-// (svierosion/evolve.py:182-250) for every running job: stop tests, stage 0
+// (svierosion/evolve.py:182-250) for every selected job: stop tests, stage 0
 // with statistics and device dt, stages 1-2, volume; then the regula falsi
 // retries for the jobs that passed their target (in lockstep over that set),
 // then reinitialisation per job where the displacement reached
@@ -348,6 +357,8 @@ struct EngJob {
   int running = 0, stop = SVIMX_STOP_NONE, error = 0;
   std::string message;
   // run goal
+  int64_t seq = 0;      // order of svimx_engine_add (the FIFO schedule's key)
+  double vol_run0 = 0;  // volume when the current run started
   bool has_target = false, has_max_time = false;
   double target = 0, max_time = 0, tol = 1e-6;
   int64_t max_steps = 0, n_start = 0;
@@ -377,8 +388,17 @@ struct Engine {
   double* h_dts = nullptr;
   double* h_res = nullptr;
   bool graphs = false;
-  hipGraphExec_t graph = nullptr;
-  std::vector<int> graph_set;  // job slots the captured graph was built for
+  struct Captured {
+    std::vector<int> set;  // job slots the graph was captured for
+    hipGraphExec_t exec;
+    uint64_t last_use;
+  };
+  std::vector<Captured> captured;  // the graphs of the last few job sets
+  uint64_t graph_clock = 0;
+  int policy = SVIMX_SCHED_LOCKSTEP;
+  int64_t budget_nodes = 0;
+  double slack = 0.0;
+  int64_t next_seq = 0;
   int64_t capture_fallbacks = 0;  // ticks run eagerly because their capture was invalidated
   bool graph_has_tile = false;
   double ms_tick = 0, ms_secant = 0, ms_reinit = 0;
@@ -632,18 +652,34 @@ static bool tile_jobs_in(const Engine& E) {
   return false;
 }
 
-static void drop_graph(Engine& E) {
-  if (E.graph) hipGraphExecDestroy(E.graph);
-  E.graph = nullptr;
-  E.graph_set.clear();
+constexpr int kGraphCache = 6;
+
+// Drop the captured graphs (of every set, or of the sets holding slot).
+static void drop_graph(Engine& E, int slot = -1) {
+  auto holds = [&](const Engine::Captured& c) {
+    return slot < 0 || std::find(c.set.begin(), c.set.end(), slot) != c.set.end();
+  };
+  for (auto& c : E.captured)
+    if (holds(c)) hipGraphExecDestroy(c.exec);
+  E.captured.erase(std::remove_if(E.captured.begin(), E.captured.end(), holds), E.captured.end());
 }
 
 // The main tick's launches for the current set (captured once per set when
 // graphs are on: the tables it reads are refreshed before every replay).
 static int launch_main(Engine& E) {
   if (!E.graphs || tile_jobs_in(E)) return main_sequence(E);
-  if (!E.graph || E.graph_set != E.set) {
-    drop_graph(E);
+  Engine::Captured* hit = nullptr;
+  for (auto& c : E.captured)
+    if (c.set == E.set) hit = &c;
+  if (!hit) {
+    if ((int)E.captured.size() >= kGraphCache) {  // evict the least recently used
+      auto lru = std::min_element(E.captured.begin(), E.captured.end(),
+                                  [](const Engine::Captured& a, const Engine::Captured& b) {
+                                    return a.last_use < b.last_use;
+                                  });
+      hipGraphExecDestroy(lru->exec);
+      E.captured.erase(lru);
+    }
     // Relaxed: other threads of the process (set-up and meshing workers) keep
     // allocating and freeing while this thread captures.
     SVIMX_CHECK_HIP(hipStreamBeginCapture(E.st, hipStreamCaptureModeRelaxed));
@@ -673,12 +709,15 @@ static int launch_main(Engine& E) {
       ++E.capture_fallbacks;
       return main_sequence(E);
     }
-    e = hipGraphInstantiate(&E.graph, g, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     SVIMX_CHECK_HIP(e);
-    E.graph_set = E.set;
+    E.captured.push_back(Engine::Captured{E.set, exec, 0});
+    hit = &E.captured.back();
   }
-  SVIMX_CHECK_HIP(hipGraphLaunch(E.graph, E.st));
+  hit->last_use = ++E.graph_clock;
+  SVIMX_CHECK_HIP(hipGraphLaunch(hit->exec, E.st));
   return SVIMX_OK;
 }
 
@@ -796,21 +835,63 @@ static int stop_test(const EngJob& e) {
   return SVIMX_STOP_NONE;
 }
 
+// Estimated steps left in the job's run: the volume still to go over the
+// volume lost per step since the run started (infinite for the first steps
+// of a run, so every job gets measured early); max_steps bounds it.
+static double steps_left(const EngJob& e) {
+  const double done = (double)(e.steps - e.n_start);
+  const double cap = (double)e.max_steps - done;
+  if (!e.has_target) return cap;
+  if (done < 4) return INFINITY;
+  const double rate = (e.vol_run0 - e.vol) / done;
+  if (!(rate > 0)) return INFINITY;
+  return std::min(cap, (e.vol - e.target) / rate);
+}
+
+// The jobs of this tick (E.set, increasing slot order) among the running
+// candidates, by the engine's schedule.
+static void select_jobs(Engine& E, const std::vector<int>& cand) {
+  E.set.clear();
+  if (E.policy == SVIMX_SCHED_LOCKSTEP || cand.size() <= 1) {
+    E.set = cand;
+    return;
+  }
+  std::vector<std::pair<double, int>> order;
+  order.reserve(cand.size());
+  for (int slot : cand) {
+    const EngJob& e = E.jobs[slot];
+    order.emplace_back(E.policy == SVIMX_SCHED_FIFO ? (double)e.seq : -steps_left(e), slot);
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first < b.first; });
+  const double cap = std::max((double)E.budget_nodes, (1.0 + E.slack) * (double)E.jobs[order[0].second].nb);
+  double used = 0;
+  for (const auto& o : order) {
+    const double nb = (double)E.jobs[o.second].nb;
+    if (E.set.empty() || used + nb <= cap) {
+      E.set.push_back(o.second);
+      used += nb;
+    }
+  }
+  std::sort(E.set.begin(), E.set.end());
+}
+
 static int tick(Engine& E, int* n_running) {
 #pragma clang fp contract(off)
   const double t0 = now_ms();
-  E.set.clear();
+  std::vector<int> cand;
   for (int slot = 0; slot < (int)E.jobs.size(); ++slot) {
     EngJob& e = E.jobs[slot];
     if (!e.used || !e.running) continue;
     const int why = stop_test(e);
     if (why) halt(E, slot, why);
-    else E.set.push_back(slot);
+    else cand.push_back(slot);
   }
-  if (E.set.empty()) {
+  if (cand.empty()) {
     *n_running = 0;
     return SVIMX_OK;
   }
+  select_jobs(E, cand);
   // one step of every job in the set
   int rc = upload(E);
   if (!rc) rc = launch_main(E);
@@ -1069,6 +1150,7 @@ int svimx_engine_add(void* engine, const svimx_job_desc_t* desc, int* job_id) {
   e.time = D.time;
   e.displacement = D.displacement;
   e.V0 = D.initial_volume;
+  e.seq = E->next_seq++;
   int rc = ensure_band_arrays(e);
   if (rc) {
     svimx_engine_remove(E, id);
@@ -1110,6 +1192,7 @@ int svimx_engine_run(void* engine, int id, double target_fraction, double max_ti
   }
   std::memcpy(e.res, res_of(*E, id), sizeof(e.res));
   e.vol = volume_of(e, e.res);
+  e.vol_run0 = e.vol;
   e.frac_valid = e.cur;
   e.running = 1;
   return SVIMX_OK;
@@ -1170,7 +1253,20 @@ int svimx_engine_remove(void* engine, int id) {
   hipFree(e.geo);
   e = EngJob();
   E->stopped.erase(std::remove(E->stopped.begin(), E->stopped.end(), id), E->stopped.end());
-  drop_graph(*E);
+  drop_graph(*E, id);
+  return SVIMX_OK;
+}
+
+int svimx_engine_schedule(void* engine, int policy, int64_t budget_nodes, double slack) {
+  Engine* E = static_cast<Engine*>(engine);
+  SVIMX_REQUIRE(E, SVIMX_EBADARG, "null engine");
+  SVIMX_REQUIRE(policy >= SVIMX_SCHED_LOCKSTEP && policy <= SVIMX_SCHED_LONGEST, SVIMX_EBADARG,
+                "unknown schedule %d", policy);
+  SVIMX_REQUIRE(budget_nodes >= 0 && slack >= 0 && std::isfinite(slack), SVIMX_EBADARG,
+                "budget_nodes and slack must be >= 0");
+  E->policy = policy;
+  E->budget_nodes = budget_nodes;
+  E->slack = slack;
   return SVIMX_OK;
 }
 

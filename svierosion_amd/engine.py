@@ -2,9 +2,10 @@
 
 Python face of ``svimx_engine_*`` (include/svimx.h).  A job is a prepared
 :class:`~svierosion_amd.evolve.ErosionSimulation` (level set built, law and
-N_r plan in place); the engine steps every running job once per tick with one
-launch per kernel and RK stage for the whole set, device-side dt and one host
-synchronisation per tick.  Each job's arithmetic is that of the single-job
+N_r plan in place); each engine tick steps the running jobs its schedule selects
+(all of them, or a priority order filled up to a band-node budget) with one
+launch per kernel and RK stage for the whole selection, device-side dt and one
+host synchronisation per tick.  Each job's arithmetic is that of the single-job
 path (same device bodies, block decomposition and reduction order), so a job's
 history and field are bit-identical alone, in any mix, and through
 ``ErosionSimulation.run``.  The step logic mirrors ErosionSimulation.run
@@ -28,6 +29,7 @@ from .levelset import MARGIN
 
 STOP_NAMES = {0: "running", 1: "target_volume", 2: "max_time", 3: "vanished", 4: "stalled", 5: "max_steps",
               6: "error"}
+SCHEDULES = {"lockstep": 0, "fifo": 1, "longest": 2}
 
 
 JobDesc, StepRec, JobStatus = _abi.JobDesc, _abi.StepRec, _abi.JobStatus
@@ -74,7 +76,7 @@ def job_desc(sim: ErosionSimulation) -> JobDesc:
 
 
 class BatchEngine:
-    """Lockstep stepping of many :class:`ErosionSimulation` jobs (see module doc)."""
+    """Stepping of many :class:`ErosionSimulation` jobs in shared ticks (see module doc)."""
 
     def __init__(self, max_jobs: int = 64, graphs: bool = True):
         self._h = C.c_void_p()
@@ -100,6 +102,13 @@ class BatchEngine:
     def use_graphs(self, on: bool) -> None:
         _abi.check(_abi.lib.svimx_engine_use_graphs(self._h, int(bool(on))))
 
+    def schedule(self, policy: str = "lockstep", budget_nodes: int = 0, slack: float = 0.0) -> None:
+        """Which running jobs a tick steps: ``"lockstep"`` (all), or ``"fifo"`` (the order
+        jobs were added) / ``"longest"`` (most estimated steps left first) filled while the
+        selected band nodes stay within max(budget_nodes, (1 + slack) x the first job's)."""
+        _abi.check(_abi.lib.svimx_engine_schedule(self._h, SCHEDULES[policy], int(budget_nodes), float(slack)),
+                   "engine schedule")
+
     # ----------------------------------------------------------------- jobs
     def add(self, sim: ErosionSimulation) -> int:
         import torch
@@ -119,7 +128,7 @@ class BatchEngine:
                                              float(tol)), "engine run")
 
     def tick(self) -> int:
-        """One step of every running job; returns how many are still running."""
+        """One step of each job the schedule selects; returns how many are still running."""
         n = C.c_int()
         with _abi.device_lock:  # no synchronous free elsewhere while a tick may capture
             rc = _abi.lib.svimx_engine_tick(self._h, C.byref(n))

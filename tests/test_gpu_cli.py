@@ -58,7 +58,8 @@ def test_erode_command_exits_2_when_the_target_is_not_reached(device, tmp_path, 
         p.max_steps = 3
         return p
     monkeypatch.setattr(cli, "params_from_args", three_steps)
-    rc = cli.main(["erode", src, "-r", "0.2", "--resolution", "32", "--quiet", "-o", str(tmp_path / "o.ply")])
+    rc = cli.main(["erode", src, "-r", "0.2", "--resolution", "32", "--target", "0.5", "--quiet",
+                   "-o", str(tmp_path / "o.ply")])
     assert rc == 2
     meta = json.loads((tmp_path / "o.json").read_text())
     assert meta["stop_reason"] == "max_steps"

@@ -47,6 +47,20 @@ def test_served_mix_equals_sequential(device):
     assert mix.latency(0.99) >= mix.latency(0.5) > 0
 
 
+@pytest.mark.parametrize("schedule", ["lockstep", "fifo", "longest"])
+def test_every_schedule_reproduces_sequential_erode(device, schedule):
+    """A tight band-node budget (each tick holds the first job and whatever fits
+    beside it in 20,000 band nodes) makes the priority schedules pass jobs over
+    in many ticks; every job's history, snapshots and mesh are unchanged."""
+    from svierosion_amd.batch import serve, serve_sequential
+    reqs = _requests()
+    rep = serve(reqs, max_jobs=8, workers=2, schedule=schedule, budget_nodes=20_000, slack=0.0, solo_tail=False)
+    seq = serve_sequential(reqs)
+    assert len(rep.done) == len(reqs)
+    _check_same(rep, seq)
+    assert rep.ticks >= max(c.steps for c in rep.completions)
+
+
 def test_budget_admits_one_job_at_a_time(device):
     from svierosion_amd.batch import serve, serve_sequential
     reqs = _requests()[:3]

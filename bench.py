@@ -248,7 +248,9 @@ def nr_shape_sweep(sim) -> int:
             _abi.check(_abi.lib.svimx_nr_sum(*args))
         b.record()
         torch.cuda.synchronize()
-        print(json.dumps({"shape": conv.shape_name, "ms": round(a.elapsed_time(b) / 5, 4)}), flush=True)
+        units = int(ls.results()[_abi.R_TILES])
+        print(json.dumps({"shape": conv.shape_name, "ms": round(a.elapsed_time(b) / 5, 4), "units": units,
+                          "outputs": units * rl * max(lines, 1)}), flush=True)
     return 0
 
 

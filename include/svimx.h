@@ -491,6 +491,11 @@ int svimx_engine_tick(void* engine, int* n_running);
  * stepped, a job that does not fit is passed over for this tick.  A job's
  * results do not depend on the schedule. */
 enum { SVIMX_SCHED_LOCKSTEP = 0, SVIMX_SCHED_FIFO = 1, SVIMX_SCHED_LONGEST = 2 };
+/* Up to max_ticks ticks in one call, ending early after the tick in which a
+ * job stopped or when none is running; *n_ticks = ticks taken.  For request
+ * loops that look at arrivals and stopped jobs every few ticks (no
+ * counterpart in the reference). */
+int svimx_engine_ticks(void* engine, int max_ticks, int* n_running, int* n_ticks);
 int svimx_engine_schedule(void* engine, int policy, int64_t budget_nodes, double slack);
 int svimx_engine_status(void* engine, int job_id, svimx_job_status_t* out);
 /* Copy step records [first, first + n) of job_id (records accumulate over runs). */

@@ -120,6 +120,7 @@ _SIGNATURES = {
     "svimx_engine_add": (_I, [_P, C.POINTER(JobDesc), C.POINTER(C.c_int)]),
     "svimx_engine_run": (_I, [_P, _I, _D, _D, _I64, _D]),
     "svimx_engine_tick": (_I, [_P, C.POINTER(C.c_int)]),
+    "svimx_engine_ticks": (_I, [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "svimx_engine_status": (_I, [_P, _I, C.POINTER(JobStatus)]),
     "svimx_engine_records": (_I, [_P, _I, _I64, _I64, C.POINTER(StepRec)]),
     "svimx_engine_remove": (_I, [_P, _I]),

@@ -156,20 +156,26 @@ class _Worker:
         self.pool.shutdown(wait=True)
 
 
-# Tick budget of the served schedule: measured on 12 Poisson requests of the
-# served workload (budgets 1M-4M band nodes, slack 0.25-1.0 all gave p50 and
-# p99 below one-at-a-time FIFO; 2M / 1.0 the lowest p99).
+# Tick budget of the served schedule, from a sweep over 12 Poisson requests of
+# the served workload at 3.08/s (one-at-a-time FIFO: p50 1.19 s, p99 2.43 s,
+# 8,750 erosions/hour): 2M band nodes / slack 0.5 gave p50 1.23 s, p99 2.01 s,
+# 9,930 erosions/hour; slack 0 or a 1M budget lowered p50 to 1.20-1.24 s and
+# raised p99 to 2.17-2.26 s; no budget (one job per tick) 1.35 / 2.62 s, the
+# engine's per-step cost for a job alone being ~10% above the single-job path.
 DEFAULT_BUDGET_NODES = 2_000_000
-DEFAULT_SLACK = 1.0
+DEFAULT_SLACK = 0.5
 
 
 def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, graphs: bool = True,
           keep_state: bool = False, verbose: bool = False, workers: int = 2, solo_tail: bool = True,
-          schedule: str = "fifo", budget_nodes: int = DEFAULT_BUDGET_NODES, slack: float = DEFAULT_SLACK) -> ServeReport:
+          schedule: str = "fifo", budget_nodes: int = DEFAULT_BUDGET_NODES, slack: float = DEFAULT_SLACK,
+          ticks_per_visit: int = 8) -> ServeReport:
     """Continuous batching of ``requests`` (arrival times in seconds from now).  With
     ``solo_tail`` the last running job finishes on the single-job path once no
     request is left to admit.  ``schedule``, ``budget_nodes`` and ``slack`` select
-    the jobs of each tick (:meth:`BatchEngine.schedule`)."""
+    the jobs of each tick (:meth:`BatchEngine.schedule`); the loop looks at
+    arrivals, set-up and meshing after every ``ticks_per_visit`` ticks or as soon
+    as a job stops."""
     import torch
     from .engine import STOP_NAMES, BatchEngine
     _abi.require_device("cuda")
@@ -257,10 +263,11 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
                 complete(idx, error=err)
             used -= need
             continue
-        # one step of every running job
+        # up to ticks_per_visit ticks (fewer when a job stops), then look at
+        # arrivals, set-up and meshing again
         if n_running > 0:
-            n_running = eng.tick()
-            ticks += 1
+            n_running, k = eng.ticks(ticks_per_visit)
+            ticks += k
         # jobs that reached a target: mesh them on the worker
         for jid in eng.stopped():
             idx, job, need, t_run, _ = live[jid]

@@ -25,7 +25,9 @@
 //
 // Host side.  The tick mirrors one iteration of ErosionSimulation.run
 // (svierosion/evolve.py:182-250) for every selected job: stop tests, stage 0
-// with statistics and device dt, stages 1-2, volume; then the regula falsi
+// with statistics and device dt, stages 1-2, volume (each stage's velocity
+// fused with the RK update that consumes it, as on the single-job path); then
+// the regula falsi
 // retries for the jobs that passed their target (in lockstep over that set),
 // then reinitialisation per job where the displacement reached
 // reinit_distance*h, followed by one batched volume pass.
@@ -76,6 +78,7 @@ struct JobDev {
   const int* col_start;
   int asplit, cfirst, csplit;  // the plan's N_r summation order
   const float* weights;
+  const float* colrec;
   float scale, negbv;
   int m, lds;
   int ylines, nyu;  // run mode: y-lines per unit, units per x-plane column
@@ -215,8 +218,8 @@ __global__ void __launch_bounds__(256) k_eng_nr_lines(const JobDev* __restrict__
     const JobDev& j = J[jslot];
     const int64_t nunit = j.ccounts[j.nchunks];
     if (lb == 0 && threadIdx.x == 0) j.results[SVIMX_R_TILES] = (double)nunit;
-    nr_lines_body<MP, RL, Y, kNrLineLanes>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nunit, j.cols,
-                                           j.col_start, j.m, j.asplit, j.weights, j.scale, j.negbv,
+    nr_lines_body<MP, RL, Y, kNrLineLanes, false>(j.occ, OccLayout::of(j.d.nx, j.d.ny, j.d.nz), j.nzr, j.list, nunit, j.cols,
+                                           j.col_start, j.m, j.asplit, j.colrec, j.scale, j.negbv,
                                            const_cast<float*>(j.field), xch, lb, G);
   }
 }
@@ -242,13 +245,23 @@ __global__ void __launch_bounds__(kBandThreads) k_eng_curvatures(const JobDev* _
   }
 }
 
-template <bool SVI>
-__global__ void __launch_bounds__(kBandThreads) k_eng_velocity(const JobDev* __restrict__ J, const Seg* S, int rel,
-                                                               int stats) {
+// The velocity fused with the RK stage that consumes it (svimx_velocity_rk's
+// body): MODE 1 stores L0 and the statistics of stage 0, MODE 2 and 3 write
+// stages 1 and 2 into buffer rel_out with the job's device dt.
+template <bool SVI, int MODE>
+__global__ void __launch_bounds__(kBandThreads) k_eng_velocity_rk(const JobDev* __restrict__ J, const Seg* S,
+                                                                  int rel_in, int rel_u0, int rel_out, int stats,
+                                                                  const double* __restrict__ dts) {
   ENG_FOR(S) {
     const JobDev& j = J[jslot];
-    velocity_body<SVI>(jbuf(j, rel), j.band, j.nb, j.d, j.h, j.law, j.field, j.k12, j.cut_lo, j.cut_hi, j.hi, stats,
-                  j.V, j.partials, lb, G);
+    RkFuse rk;
+    rk.u0 = jbuf(j, rel_u0);
+    rk.out = jbuf(j, rel_out);
+    rk.L0 = j.L0;
+    rk.inv_h = j.inv_h;
+    rk.dt = MODE >= 2 ? (float)dts[jslot] : 0.0f;
+    velocity_body<SVI, MODE>(jbuf(j, rel_in), j.band, j.nb, j.d, j.h, j.law, j.field, j.k12, j.cut_lo, j.cut_hi,
+                             j.hi, stats, j.V, j.partials, lb, G, rk);
   }
 }
 
@@ -461,6 +474,7 @@ static void fill_jobdev(const EngJob& e, JobDev& j) {
     j.m = p->m;
     j.lds = p->lds;
     j.weights = p->weights;
+    j.colrec = p->colrec;
     j.scale = p->scale;
     j.negbv = p->negbv;
   }
@@ -558,9 +572,10 @@ static unsigned grid_of(const Engine& E, int t) {
     }                                                                                                     \
   } while (0)
 
-// fractions + N_r (SVI jobs) or curvatures, then velocity of buffer cur+rel;
-// curvature jobs compute fractions only for the end-of-step volume
-static int prepare(Engine& E, int rel, int stats, int only_needed) {
+// fractions + N_r (SVI jobs) or curvatures of buffer cur+rel (the inputs of
+// its velocity); curvature jobs compute fractions only for the end-of-step
+// volume
+static int inputs(Engine& E, int rel, int only_needed) {
   ENG_LAUNCH(k_eng_clear_marks, T_MARKS, 256, only_needed);
   ENG_LAUNCH(k_eng_fractions, T_SVI, kBandThreads, rel, only_needed);
   ENG_LAUNCH(k_eng_unit_need, T_RUNS, 256);
@@ -598,9 +613,24 @@ static int prepare(Engine& E, int rel, int stats, int only_needed) {
     if (rc) return rc;
   }
   ENG_LAUNCH(k_eng_curvatures, T_CURV, kBandThreads, rel);
-  ENG_LAUNCH(k_eng_velocity<true>, T_SVI, kBandThreads, rel, stats);
-  ENG_LAUNCH(k_eng_velocity<false>, T_CURV, kBandThreads, rel, stats);
-  if (stats) ENG_LAUNCH(k_eng_stats_dt, T_JOBS, kFinalizeThreads, E.d_dts);
+  return SVIMX_OK;
+}
+
+// Velocity of buffer cur+rel fused with the RK stage that consumes it (as the
+// single-job path runs them): stage 0 stores L0 and the statistics and
+// derives dt; stages 1 and 2 write buffer cur+rel_out from u0 = cur.
+static int velocity_rk(Engine& E, int stage, int rel, int rel_out) {
+  if (stage == 0) {
+    ENG_LAUNCH((k_eng_velocity_rk<true, 1>), T_SVI, kBandThreads, rel, 0, rel, 1, E.d_dts);
+    ENG_LAUNCH((k_eng_velocity_rk<false, 1>), T_CURV, kBandThreads, rel, 0, rel, 1, E.d_dts);
+    ENG_LAUNCH(k_eng_stats_dt, T_JOBS, kFinalizeThreads, E.d_dts);
+  } else if (stage == 1) {
+    ENG_LAUNCH((k_eng_velocity_rk<true, 2>), T_SVI, kBandThreads, rel, 0, rel_out, 0, E.d_dts);
+    ENG_LAUNCH((k_eng_velocity_rk<false, 2>), T_CURV, kBandThreads, rel, 0, rel_out, 0, E.d_dts);
+  } else {
+    ENG_LAUNCH((k_eng_velocity_rk<true, 3>), T_SVI, kBandThreads, rel, 0, rel_out, 0, E.d_dts);
+    ENG_LAUNCH((k_eng_velocity_rk<false, 3>), T_CURV, kBandThreads, rel, 0, rel_out, 0, E.d_dts);
+  }
   return SVIMX_OK;
 }
 
@@ -617,19 +647,23 @@ static int rk(Engine& E, int stage, int rel_in, int rel_u0, int rel_out) {
   return SVIMX_OK;
 }
 
-// stages 1 and 2 after stage 0 (or 3) wrote b = cur+1; then the volume of b
+// stages 1 and 2 after stage 0 (or its replay) wrote b = cur+1; then the
+// volume of b
 static int tail_and_volume(Engine& E) {
-  int rc = prepare(E, 1, 0, 0);
-  if (!rc) rc = rk(E, 1, 1, 0, 2);
-  if (!rc) rc = prepare(E, 2, 0, 0);
-  if (!rc) rc = rk(E, 2, 2, 0, 1);
+  int rc = inputs(E, 1, 0);
+  if (!rc) rc = velocity_rk(E, 1, 1, 2);
+  if (!rc) rc = inputs(E, 2, 0);
+  if (!rc) rc = velocity_rk(E, 2, 2, 1);
   if (!rc) rc = volume(E, 1);
   return rc;
 }
 
+// stage 0 as velocity + L0 and statistics, then its update b = u0 + dt L0
+// (the stage-3 replay of rk_node)
 static int main_sequence(Engine& E) {
-  int rc = prepare(E, 0, 1, 1);
-  if (!rc) rc = rk(E, 0, 0, 0, 1);
+  int rc = inputs(E, 0, 1);
+  if (!rc) rc = velocity_rk(E, 0, 0, 0);
+  if (!rc) rc = rk(E, 3, 0, 0, 1);
   if (!rc) rc = tail_and_volume(E);
   return rc;
 }
@@ -1204,6 +1238,20 @@ int svimx_engine_tick(void* engine, int* n_running) {
   Engine* E = static_cast<Engine*>(engine);
   SVIMX_REQUIRE(E && n_running, SVIMX_EBADARG, "null argument");
   return tick(*E, n_running);
+  SVIMX_TRY_END
+}
+
+int svimx_engine_ticks(void* engine, int max_ticks, int* n_running, int* n_ticks) {
+  SVIMX_TRY_BEGIN
+  Engine* E = static_cast<Engine*>(engine);
+  SVIMX_REQUIRE(E && n_running && n_ticks && max_ticks > 0, SVIMX_EBADARG, "null argument or max_ticks < 1");
+  *n_ticks = 0;
+  do {
+    const int rc = tick(*E, n_running);
+    if (rc) return rc;
+    ++*n_ticks;
+  } while (*n_ticks < max_ticks && *n_running > 0 && E->stopped.empty());
+  return SVIMX_OK;
   SVIMX_TRY_END
 }
 

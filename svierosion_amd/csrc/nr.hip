@@ -22,6 +22,7 @@
 // owning x-layer x0+w; source x-planes stream through double-buffered LDS and
 // each wave sums the columns of its (wave-uniform) offset dx = X - (x0+w).
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "common.h"
@@ -104,12 +105,12 @@ template <int MP, int RL, int Y>
 __global__ void __launch_bounds__(256)
     k_nr_lines(const float* __restrict__ occ, OccLayout OL, int nzr, const int32_t* __restrict__ list,
                const int64_t* __restrict__ total, const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-               int m, int asplit, const float* __restrict__ w, float scale, float negbv, float* __restrict__ out,
+               int m, int asplit, const float* __restrict__ colrec, float scale, float negbv, float* __restrict__ out,
                double* __restrict__ results) {
   const int64_t nunit = *total;
   if (blockIdx.x == 0 && threadIdx.x == 0 && results) results[SVIMX_R_TILES] = (double)nunit;
   __shared__ float xch[nr_lines_xch<RL, Y, kNrLineLanes>()];
-  nr_lines_body<MP, RL, Y, kNrLineLanes>(occ, OL, nzr, list, nunit, cols, col_start, m, asplit, w, scale, negbv,
+  nr_lines_body<MP, RL, Y, kNrLineLanes>(occ, OL, nzr, list, nunit, cols, col_start, m, asplit, colrec, scale, negbv,
                                          out, xch, blockIdx.x, gridDim.x);
 }
 
@@ -216,7 +217,9 @@ bool nr_shape_allowed(const NrPlan* p, const NrShape& s) {
   if (s.rl != 4 && s.rl != 8 && s.rl != 16) return false;
   if (s.lines != 1 && s.lines != 2 && s.lines != 4) return false;
   if (s.lds) return s.lines == 1 && p->lds_ok;
-  if (s.lines > 1) return p->cols_contig && p->m + s.lines - 1 <= kOccPad && !(s.lines == 4 && s.rl > 8);
+  // the multi-line kernels index the occupancy by 32-bit float4 offsets (lds_ok)
+  if (s.lines > 1)
+    return p->cols_contig && p->lds_ok && p->m + s.lines - 1 <= kOccPad && !(s.lines == 4 && s.rl > 8);
   return true;
 }
 
@@ -327,7 +330,16 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
   p->nweights = (int)wts.size();
   p->scale = (float)(h * h * h);
   p->negbv = (float)(-1.0 / (4.0 / 3.0 * M_PI * radius * radius * radius));
+  std::vector<float> rec((cols.size() + 1) * kNrRecStride, 0.0f);
+  if (run_mode)
+    for (size_t c = 0; c < cols.size(); ++c) {
+      const int k = -cols[c].dz0;
+      const int kbits = k;
+      std::memcpy(&rec[c * kNrRecStride], &kbits, sizeof(int));
+      for (int d = 0; d <= k; ++d) rec[c * kNrRecStride + 1 + d] = wts[cols[c].woff + k + d];
+    }
   hipError_t e = hipMalloc(&p->cols, sizeof(NrColumn) * (cols.size() + 1));
+  if (e == hipSuccess) e = hipMalloc(&p->colrec, sizeof(float) * rec.size());
   if (e == hipSuccess) e = hipMalloc(&p->col_start, sizeof(int) * col_start.size());
   if (e == hipSuccess) e = hipMalloc(&p->weights, sizeof(float) * (wts.size() + 1));
   if (e == hipSuccess) e = hipMalloc(&p->marks, cap_marks);
@@ -341,12 +353,13 @@ int svimx_nr_plan_create(const double* weights, int m, double h, double radius, 
     e = hipMemcpy(p->col_start, col_start.data(), sizeof(int) * col_start.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess && !wts.empty())
     e = hipMemcpy(p->weights, wts.data(), sizeof(float) * wts.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->colrec, rec.data(), sizeof(float) * rec.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(p->marks, 0, cap_marks);
   if (e == hipSuccess) e = hipMemset(p->list, 0, sizeof(int32_t) * cap_marks);
   if (e == hipSuccess) e = hipMemset(p->ccounts, 0, sizeof(int64_t) * (cap_chunks + 2));
   if (e != hipSuccess) {
     set_error("N_r plan allocation failed: %s", hipGetErrorString(e));
-    hipFree(p->cols); hipFree(p->col_start); hipFree(p->weights); hipFree(p->marks); hipFree(p->dil);
+    hipFree(p->cols); hipFree(p->col_start); hipFree(p->weights); hipFree(p->colrec); hipFree(p->marks); hipFree(p->dil);
     hipFree(p->list); hipFree(p->count); hipFree(p->ccounts);
     delete p;
     return SVIMX_EDEVICE;
@@ -400,6 +413,7 @@ int svimx_nr_plan_destroy(void* plan) {
   hipFree(p->cols);
   hipFree(p->col_start);
   hipFree(p->weights);
+  hipFree(p->colrec);
   hipFree(p->marks);
   hipFree(p->dil);
   hipFree(p->list);
@@ -493,7 +507,7 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
     const int grid_lds = (int)std::min<int64_t>((p->nruns + 63) / 64 + kXcds - 1, 256 * 10) / kXcds * kXcds;
 #define SVIMX_LINES(MPV, RLV, YV)                                                                              \
   hipLaunchKernelGGL((k_nr_lines<MPV, RLV, YV>), dim3(grid), dim3(256), 0, st, occupancy, OL, p->nzr, p->list, \
-                     total, p->cols, p->col_start, p->m, p->asplit, p->weights, p->scale, p->negbv, nr_out,   \
+                     total, p->cols, p->col_start, p->m, p->asplit, p->colrec, p->scale, p->negbv, nr_out,    \
                      w.results)
 #define SVIMX_RUNS(MPV, RLV)                                                                                       \
   if (p->lds)                                                                                                      \

@@ -132,6 +132,47 @@ __device__ __forceinline__ void taps_of(int k, const float* __restrict__ wc, con
   }
 }
 
+// Taps of a column from its record r (r[0]: k as int bits, r[1 + d]: the
+// weight at |dz| = d): the same products, in the same order, as
+// column_taps over the column's 2k+1 weights.
+template <int K, int MP, int RL, int WIN>
+__device__ __forceinline__ void column_taps_rec(const float (&r)[MP + 2], const float (&win)[WIN],
+                                                float (&part)[RL]) {
+  if constexpr (K <= MP) {
+#pragma unroll
+    for (int t = 0; t <= 2 * K; ++t) {
+      const float wt = r[1 + (t < K ? K - t : t - K)];
+#pragma unroll
+      for (int j = 0; j < RL; ++j) part[j] += wt * win[j + MP - K + t];
+    }
+  }
+}
+
+template <int MP, int RL>
+__device__ __forceinline__ void taps_rec(const float (&r)[MP + 2], const float (&win)[RL + 2 * MP],
+                                         float (&part)[RL]) {
+  switch (__float_as_int(r[0])) {
+    case 0: column_taps_rec<0, MP, RL>(r, win, part); break;
+    case 1: column_taps_rec<1, MP, RL>(r, win, part); break;
+    case 2: column_taps_rec<2, MP, RL>(r, win, part); break;
+    case 3: column_taps_rec<3, MP, RL>(r, win, part); break;
+    case 4: column_taps_rec<4, MP, RL>(r, win, part); break;
+    case 5: column_taps_rec<5, MP, RL>(r, win, part); break;
+    case 6: column_taps_rec<6, MP, RL>(r, win, part); break;
+    case 7: column_taps_rec<7, MP, RL>(r, win, part); break;
+    case 8: column_taps_rec<8, MP, RL>(r, win, part); break;
+    case 9: column_taps_rec<9, MP, RL>(r, win, part); break;
+    case 10: column_taps_rec<10, MP, RL>(r, win, part); break;
+    case 11: column_taps_rec<11, MP, RL>(r, win, part); break;
+    case 12: column_taps_rec<12, MP, RL>(r, win, part); break;
+    case 13: column_taps_rec<13, MP, RL>(r, win, part); break;
+    case 14: column_taps_rec<14, MP, RL>(r, win, part); break;
+    case 15: column_taps_rec<15, MP, RL>(r, win, part); break;
+    case 16: column_taps_rec<16, MP, RL>(r, win, part); break;
+    default: break;
+  }
+}
+
 constexpr int kNrLineLanes = 2;  // waves per unit group of the multi-line kernels (1 or 2)
 
 // Order of the run-mode sums read through the L1 (every such shape, so they
@@ -179,44 +220,154 @@ __device__ __forceinline__ void nr_window(const float* __restrict__ occ, const O
 }
 
 // Sum of the |dx| groups [ab, ae) of the unit's Y lines, folded group by
-// group from 0.0 (one half of the canonical order).
-template <int MP, int RL, int Y>
+// group from 0.0 (one half of the canonical order).  The windows of the
+// groups form one sequence (group a, window t = 0 .. ncol + Y - 2).  With PF
+// the loads of the next window are in flight while the taps of the current
+// one run (two register sets, the loop unrolled twice so their roles
+// alternate without copies): -5% on 4-node runs of 4 lines at m = 8 (0.218 ->
+// 0.205 ms at the 1.3M-triangle scan), where the second set still leaves 3
+// waves per SIMD; wider windows would spill, so the kernels enable it for
+// MP <= 8 and 4-node runs only.  The column records of a window's lines are
+// loaded before any of their taps.  Window offsets are 32-bit float4
+// indices (plans whose occupancy has 2^31 float4s or more take the one-line
+// shapes).
+struct NrPos {
+  int a, t, c0, ncol, dyf;
+};
+
+template <int MP, int RL, int Y, bool PF>
 __device__ __forceinline__ void nr_lines_half(const float* __restrict__ occ, const OccLayout& OL,
                                               const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-                                              const float* __restrict__ w, int x, int y0, int z0, int m, int ab,
+                                              const float* __restrict__ colrec, int x, int y0, int z0, int m, int ab,
                                               int ae, float (&sum)[Y][RL]) {
-  constexpr int WIN = RL + 2 * MP;
+  constexpr int WIN = RL + 2 * MP, Q = WIN / 4;
+  const float4* __restrict__ occ4 = reinterpret_cast<const float4*>(occ);
+  const int32_t sx4 = (int32_t)(OL.pitch_y * OL.pitch_z / 4), sz4 = (int32_t)(OL.pitch_z / 4);
+  const int32_t ub4 = (int32_t)(OL.at(x, y0, z0 - MP) / 4);
   float acc[Y][RL], part[Y][RL];
 #pragma unroll
   for (int yy = 0; yy < Y; ++yy)
 #pragma unroll
     for (int j = 0; j < RL; ++j) acc[yy][j] = part[yy][j] = 0.0f;
-  bool first_group = true;
-  for (int a = ab; a < ae; ++a) {
-    const int c0 = col_start[m + a], ncol = col_start[m + a + 1] - c0;
-    if (ncol == 0) continue;
-    if (!first_group) {
-#pragma unroll
-      for (int yy = 0; yy < Y; ++yy)
-#pragma unroll
-        for (int j = 0; j < RL; ++j) {
-          acc[yy][j] += part[yy][j];
-          part[yy][j] = 0.0f;
-        }
+  auto group = [&](NrPos& p) {  // first window of the first group >= p.a with columns
+    while (p.a < ae && col_start[m + p.a + 1] == col_start[m + p.a]) ++p.a;
+    p.t = 0;
+    if (p.a < ae) {
+      p.c0 = col_start[m + p.a];
+      p.ncol = col_start[m + p.a + 1] - p.c0;
+      p.dyf = cols[p.c0].dy;
     }
-    first_group = false;
-    const int dyf = cols[c0].dy;
-    for (int t = 0; t < ncol + Y - 1; ++t) {
-      float win[WIN];
-      nr_window<MP, RL>(occ, OL, x, a, y0 + dyf + t, z0, win);
+  };
+  auto advance = [&](NrPos p) {
+    if (++p.t >= p.ncol + Y - 1) {
+      ++p.a;
+      group(p);
+    }
+    return p;
+  };
+  auto issue = [&](const NrPos& p, float4 (&P)[Q], float4 (&M)[Q]) {
+    const int32_t o = ub4 + (p.dyf + p.t) * sz4;
+    const float4* __restrict__ src = occ4 + (uint32_t)(o + p.a * sx4);  // the window's float4s at immediate offsets
+#pragma unroll
+    for (int q = 0; q < Q; ++q) P[q] = src[q];
+    if (p.a > 0) {
+      const float4* __restrict__ mir = occ4 + (uint32_t)(o - p.a * sx4);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) M[q] = mir[q];
+    }
+  };
+  bool first_group = true;
+  auto taps = [&](const NrPos& p, const float4 (&P)[Q], const float4 (&M)[Q]) {
+    float win[WIN];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      win[4 * q] = P[q].x;
+      win[4 * q + 1] = P[q].y;
+      win[4 * q + 2] = P[q].z;
+      win[4 * q + 3] = P[q].w;
+    }
+    if (p.a > 0) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        win[4 * q] += M[q].x;
+        win[4 * q + 1] += M[q].y;
+        win[4 * q + 2] += M[q].z;
+        win[4 * q + 3] += M[q].w;
+      }
+    }
+    if (p.t == 0) {
+      if (!first_group) {
+#pragma unroll
+        for (int yy = 0; yy < Y; ++yy)
+#pragma unroll
+          for (int j = 0; j < RL; ++j) {
+            acc[yy][j] += part[yy][j];
+            part[yy][j] = 0.0f;
+          }
+      }
+      first_group = false;
+    }
+    if constexpr (PF) {
+      // the records of the window's columns (one per line), all loads issued first
+      float r[Y][MP + 2];
 #pragma unroll
       for (int yy = 0; yy < Y; ++yy) {
-        const int ci = t - yy;
-        if (ci >= 0 && ci < ncol) {
-          const NrColumn col = cols[c0 + ci];
-          taps_of<MP, RL>(-col.dz0, w + col.woff, win, part[yy]);
+        const int ci = p.t - yy;
+        if (ci >= 0 && ci < p.ncol) {
+          const float* rec = colrec + (int64_t)(p.c0 + ci) * kNrRecStride;
+#pragma unroll
+          for (int i = 0; i < MP + 2; ++i) r[yy][i] = rec[i];
         }
       }
+#pragma unroll
+      for (int yy = 0; yy < Y; ++yy) {
+        const int ci = p.t - yy;
+        if (ci >= 0 && ci < p.ncol) taps_rec<MP, RL>(r[yy], win, part[yy]);
+      }
+    } else {
+#pragma unroll
+      for (int yy = 0; yy < Y; ++yy) {
+        const int ci = p.t - yy;
+        if (ci >= 0 && ci < p.ncol) {
+          const float* rec = colrec + (int64_t)(p.c0 + ci) * kNrRecStride;
+          float r[MP + 2];
+#pragma unroll
+          for (int i = 0; i < MP + 2; ++i) r[i] = rec[i];
+          taps_rec<MP, RL>(r, win, part[yy]);
+        }
+      }
+    }
+  };
+  NrPos cur{ab, 0, 0, 0, 0};
+  group(cur);
+  if constexpr (!PF) {
+    for (; cur.a < ae; ++cur.a) {
+      cur.c0 = col_start[m + cur.a];
+      cur.ncol = col_start[m + cur.a + 1] - cur.c0;
+      if (cur.ncol == 0) continue;
+      cur.dyf = cols[cur.c0].dy;
+      for (cur.t = 0; cur.t < cur.ncol + Y - 1; ++cur.t) {
+        float4 PA[Q], MA[Q];
+        issue(cur, PA, MA);
+        taps(cur, PA, MA);
+      }
+    }
+  } else if (cur.a < ae) {
+    float4 PA[Q], MA[Q], PB[Q], MB[Q];
+    issue(cur, PA, MA);
+    while (true) {
+      NrPos nxt = advance(cur);
+      bool more = nxt.a < ae;
+      if (more) issue(nxt, PB, MB);
+      taps(cur, PA, MA);
+      if (!more) break;
+      cur = nxt;
+      nxt = advance(cur);
+      more = nxt.a < ae;
+      if (more) issue(nxt, PA, MA);
+      taps(cur, PB, MB);
+      if (!more) break;
+      cur = nxt;
     }
   }
 #pragma unroll
@@ -229,11 +380,16 @@ __device__ __forceinline__ void nr_lines_half(const float* __restrict__ occ, con
 template <int RL, int Y, int S>
 constexpr int nr_lines_xch() { return S == 2 ? Y * RL * 128 : 1; }
 
-template <int MP, int RL, int Y, int S>
+// PF: software-pipelined window loads (see nr_lines_half), for kernels whose
+// register budget allows them.
+template <int MP, int RL>
+constexpr bool nr_prefetch() { return MP <= 8 && RL == 4; }
+
+template <int MP, int RL, int Y, int S, bool PF = nr_prefetch<MP, RL>()>
 __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, const OccLayout& OL, int nzr,
                                               const int32_t* __restrict__ list, int64_t nunit,
                                               const NrColumn* __restrict__ cols, const int* __restrict__ col_start,
-                                              int m, int asplit, const float* __restrict__ w, float scale,
+                                              int m, int asplit, const float* __restrict__ colrec, float scale,
                                               float negbv, float* __restrict__ out, float* __restrict__ xch,
                                               int64_t lb, int64_t G) {
   static_assert(S == 1 || (S == 2 && Y % 2 == 0), "the two halves of a unit store half of its lines each");
@@ -250,8 +406,8 @@ __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, con
       const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
       const int z0 = zr * RL;
       float lo[Y][RL], hi[Y][RL];
-      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, m, 0, asplit, lo);
-      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, m, asplit, m + 1, hi);
+      nr_lines_half<MP, RL, Y, PF>(occ, OL, cols, col_start, colrec, x, y0, z0, m, 0, asplit, lo);
+      nr_lines_half<MP, RL, Y, PF>(occ, OL, cols, col_start, colrec, x, y0, z0, m, asplit, m + 1, hi);
       if (!live) continue;
 #pragma unroll
       for (int yy = 0; yy < Y; ++yy) {
@@ -286,7 +442,7 @@ __device__ __forceinline__ void nr_lines_body(const float* __restrict__ occ, con
       const int x = (int)(xyu / nyu), y0 = (int)(xyu - (uint32_t)x * nyu) * Y;
       const int z0 = zr * RL;
       float own[Y][RL];
-      nr_lines_half<MP, RL, Y>(occ, OL, cols, col_start, w, x, y0, z0, m, half ? asplit : 0,
+      nr_lines_half<MP, RL, Y, PF>(occ, OL, cols, col_start, colrec, x, y0, z0, m, half ? asplit : 0,
                                half ? m + 1 : asplit, own);
 #pragma unroll
       for (int yl = 0; yl < YH; ++yl)

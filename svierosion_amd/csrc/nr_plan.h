@@ -29,6 +29,10 @@ struct NrShape {
 constexpr NrShape kNrTuneShapes[kNrTuneCands] = {{4, 1, 0}, {8, 1, 0}, {16, 1, 0}, {4, 2, 0}, {8, 2, 0}, {16, 2, 0},
                                                  {4, 4, 0}, {8, 4, 0}, {4, 1, 1}, {8, 1, 1}, {16, 1, 1}};
 constexpr int kRunMaxM = 16;               // largest ball half-width handled in run mode (= kOccPad)
+// Column record of the multi-line kernels: the column's k (int bits) and its
+// weights at |dz| = 0..k (the taps are mirror-symmetric in z), so one scalar
+// load brings a column's tap count and weights together.
+constexpr int kNrRecStride = kRunMaxM + 2;
 
 struct NrColumn {
   int dx, dy;     // column offset
@@ -68,6 +72,7 @@ struct NrPlan {
   int cfirst;         // col_start[m]: the first column of dx = 0
   int csplit;         // col_start[m + asplit]: the first column of the hi groups
   float* weights;     // device; run mode: per column the taps dz = -k..k
+  float* colrec;      // device, ncols x kNrRecStride column records (run mode)
   uint8_t* marks;     // device, nmarks
   uint8_t* dil;       // device, nruns bytes (run mode: need flag of each unit)
   int32_t* list;      // device, nmarks

@@ -85,6 +85,7 @@ class BatchEngine:
         _abi.check(_abi.lib.svimx_engine_create(None, int(max_jobs), C.byref(self._h)), "engine")
         self.sims: dict[int, ErosionSimulation] = {}
         self._synced: dict[int, int] = {}
+        self._stop_buf, self._stop_n = (C.c_int * 256)(), C.c_int()
         self.use_graphs(graphs)
 
     def close(self) -> None:
@@ -135,6 +136,15 @@ class BatchEngine:
         _abi.check(rc, "engine tick")
         return n.value
 
+    def ticks(self, max_ticks: int) -> tuple[int, int]:
+        """Up to ``max_ticks`` ticks, ending after one in which a job stopped; returns
+        (jobs still running, ticks taken)."""
+        n, k = C.c_int(), C.c_int()
+        with _abi.device_lock:
+            rc = _abi.lib.svimx_engine_ticks(self._h, int(max_ticks), C.byref(n), C.byref(k))
+        _abi.check(rc, "engine ticks")
+        return n.value, k.value
+
     def status(self, job: int) -> JobStatus:
         st = JobStatus()
         _abi.check(_abi.lib.svimx_engine_status(self._h, job, C.byref(st)))
@@ -181,13 +191,12 @@ class BatchEngine:
 
     def stopped(self) -> list[int]:
         """Jobs that stopped since the last call (each reported once)."""
-        buf = (C.c_int * 256)()
-        n = C.c_int()
+        buf, n = self._stop_buf, self._stop_n
         out = []
         while True:
-            _abi.check(_abi.lib.svimx_engine_stopped(self._h, buf, 256, C.byref(n)))
+            _abi.check(_abi.lib.svimx_engine_stopped(self._h, buf, len(buf), C.byref(n)))
             out.extend(buf[:n.value])
-            if n.value < 256:
+            if n.value < len(buf):
                 return out
 
     def run_all(self) -> None:

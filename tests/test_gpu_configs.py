@@ -186,7 +186,7 @@ def test_fragment_mix_prefix_batched_equals_sequential(device):
     from tests.test_gpu_serve import _check_same
     reqs = [Request(m, p) for m, p in fragment_mix(6)]
     assert {r.params.law for r in reqs} == {"svi", "mean", "gauss"}
-    mix = serve(reqs, workers=2)
+    mix = serve(reqs, workers=2, schedule="lockstep")  # the batch schedule of erode_batch
     seq = serve_sequential(reqs)
     assert len(mix.done) == len(seq.done) == 6
     _check_same(mix, seq)

@@ -123,7 +123,7 @@ def _same_history(a, b) -> bool:
                                     for x, y in zip(a, b))
 
 
-def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, slack: float):
+def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, slack: float, progress: bool = False):
     """The fragment mix (first n_jobs of the 64-fragment set) eroded batched and one at a time."""
     from svierosion_amd.batch import Request, serve, serve_sequential
     from svierosion_amd.workloads import fragment_mix
@@ -131,9 +131,9 @@ def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, sla
     jobs = fragment_mix(n_jobs)
     reqs = [Request(m, p) for m, p in jobs]
     gen_s = time.time() - t0
-    rb = serve(reqs, workers=workers, schedule=schedule, budget_nodes=budget_nodes, slack=slack)
+    rb = serve(reqs, workers=workers, schedule=schedule, budget_nodes=budget_nodes, slack=slack, verbose=progress)
     print(f"[bench] batched {n_jobs} jobs: {rb.summary()}", file=sys.stderr, flush=True)
-    rs = serve_sequential(reqs)
+    rs = serve_sequential(reqs, verbose=progress)
     print(f"[bench] sequential {n_jobs} jobs: {rs.summary()}", file=sys.stderr, flush=True)
     same = all(cb.error is None and cs.error is None and _same_history(cb.result.history, cs.result.history)
                for cb, cs in zip(rb.completions, rs.completions))
@@ -153,14 +153,15 @@ def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, sla
             "engine_ticks": rb.ticks, "mesh_generation_s": round(gen_s, 2)}, rs
 
 
-def served_leg(n_req: int, rate: float, workers: int, schedule: str, budget_nodes: int, slack: float):
+def served_leg(n_req: int, rate: float, workers: int, schedule: str, budget_nodes: int, slack: float,
+               progress: bool = False):
     """Poisson arrivals of the served workload (first n_req requests), continuous batching vs FIFO."""
     from svierosion_amd.batch import serve, serve_sequential
     from svierosion_amd.workloads import poisson_requests
     reqs = poisson_requests(n_req, rate=rate)
-    rv = serve(reqs, workers=workers, schedule=schedule, budget_nodes=budget_nodes, slack=slack)
+    rv = serve(reqs, workers=workers, schedule=schedule, budget_nodes=budget_nodes, slack=slack, verbose=progress)
     print(f"[bench] served {n_req} requests at {rate:.4f}/s: {rv.summary()}", file=sys.stderr, flush=True)
-    rq = serve_sequential(reqs)
+    rq = serve_sequential(reqs, verbose=progress)
     print(f"[bench] sequential FIFO: {rq.summary()}", file=sys.stderr, flush=True)
     same = all(a.error is None and b.error is None and _same_history(a.result.history, b.result.history)
                for a, b in zip(rv.completions, rq.completions))
@@ -271,6 +272,7 @@ def main(argv=None):
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="arrivals per second of the served leg (default: --serve-load x the sequential rate)")
     ap.add_argument("--workers", type=int, default=2, help="set-up / meshing threads of the batched forms")
+    ap.add_argument("--progress", action="store_true", help="print each completed job of the batched legs (stderr)")
     ap.add_argument("--batch-schedule", choices=["lockstep", "longest", "fifo"], default="lockstep")
     ap.add_argument("--serve-schedule", choices=["fifo", "longest", "lockstep"], default="fifo")
     ap.add_argument("--budget-nodes", type=int, default=None, help="band nodes a tick may hold (schedule budget)")
@@ -386,11 +388,12 @@ def main(argv=None):
     budget = DEFAULT_BUDGET_NODES if args.budget_nodes is None else args.budget_nodes
     slack = DEFAULT_SLACK if args.slack is None else args.slack
     if args.batch_jobs > 0:
-        batched, rs = batched_leg(args.batch_jobs, args.workers, args.batch_schedule, budget, slack)
+        batched, rs = batched_leg(args.batch_jobs, args.workers, args.batch_schedule, budget, slack, args.progress)
         seq_rate = len(rs.done) / rs.makespan if rs.makespan > 0 else None
     if args.serve_requests > 0:
         rate = args.serve_rate or args.serve_load * (seq_rate if seq_rate else 0.5)
-        served = served_leg(args.serve_requests, rate, args.workers, args.serve_schedule, budget, slack)
+        served = served_leg(args.serve_requests, rate, args.workers, args.serve_schedule, budget, slack,
+                            args.progress)
     line = {"metric": "RK3 steps/s + HBM GB/s @res400 1.3M-tri; batched erosions/hour, p99 latency",
             "value": round(value, 3), "unit": "RK3 steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
             "ms_per_step": round(1000 * elapsed / max(steps, 1), 3), "higher_is_better": True, "scaling": "weak",

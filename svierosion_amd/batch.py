@@ -36,6 +36,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import math
+import sys
 import threading
 import time
 from collections import deque
@@ -205,7 +206,7 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
         c.result, c.error = result, error
         if verbose:
             print(f"[serve] request {idx} done at {c.finished:.2f}s "
-                  f"({'ok' if error is None else error})", flush=True)
+                  f"({'ok' if error is None else error})", file=sys.stderr, flush=True)
 
     while pending or prep or fin or live:
         now = clock()
@@ -339,7 +340,7 @@ def serve_sequential(requests: list, keep_state: bool = False, verbose: bool = F
         c.finished = time.perf_counter() - t0
         comps.append(c)
         if verbose:
-            print(f"[sequential] request {idx} done at {c.finished:.2f}s", flush=True)
+            print(f"[sequential] request {idx} done at {c.finished:.2f}s", file=sys.stderr, flush=True)
     comps.sort(key=lambda c: c.index)
     return ServeReport(comps, time.perf_counter() - t0)
 

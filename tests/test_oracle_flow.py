@@ -111,6 +111,29 @@ def test_reinitialize_preserves_the_interface(h, tol):
     assert np.abs(ph - ex)[shell].max() < 0.2 * h
 
 
+def test_float32_reinitialisation_error_against_float64():
+    """The FP32 reinitialisation (the precision of the HIP path, which equals this FP32
+    oracle to 1e-4 h) against the same algorithm in FP64 on the distorted sphere, 20
+    iterations: 99.9% of the nodes within 1.5 h of the zero set agree to 1e-5 h and the
+    level-set volume to 1e-7 relative.  The few worst nodes (2 of 15,140 above 1e-4 h,
+    max 3.5e-4 h) start within ~1e-5 h of zero, where the subcell distance to the
+    interface is ill-conditioned."""
+    phi64, origin, h, shape = sphere_phi(1.0, 0.05, 0.6, distort=True, dtype=np.float64)
+    a = F.LevelSet(phi64.copy(), shape, h, width=6.0)
+    b = F.LevelSet(phi64.astype(np.float32), shape, h, width=6.0)
+    a.reinitialize(iterations=20, cfl=0.5, dilation=3)
+    b.reinitialize(iterations=20, cfl=0.5, dilation=3)
+    assert a.flat.dtype == np.float64 and b.flat.dtype == np.float32
+    near = np.abs(a.flat) < 1.5 * h
+    d = np.abs(b.flat.astype(np.float64) - a.flat)[near] / h
+    rel = abs(a.volume() - b.volume()) / a.volume()
+    print(f"FP32 vs FP64 reinit: 99.9th pct {np.percentile(d, 99.9):.2e} h, max {d.max():.2e} h, "
+          f"{int((d > 1e-4).sum())} of {d.size} above 1e-4 h, volume {rel:.2e} relative")
+    assert np.percentile(d, 99.9) < 1e-5
+    assert d.max() < 1e-3
+    assert rel < 1e-7
+
+
 # ------------------------------------------------------------------ simulation: sphere ODEs
 def _interface_radius(ls, origin, shape):
     h = ls.h

@@ -268,6 +268,8 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
         # arrivals, set-up and meshing again
         if n_running > 0:
             n_running, k = eng.ticks(ticks_per_visit)
+            if verbose and ticks // 5000 != (ticks + k) // 5000:
+                print(f"[serve] {ticks + k} ticks at {clock():.2f}s, {n_running} jobs running", file=sys.stderr, flush=True)
             ticks += k
         # jobs that reached a target: mesh them on the worker
         for jid in eng.stopped():

@@ -71,10 +71,16 @@ constexpr int kMaxReduceBlocks = 2048;  // grid cap of reducing kernels (grid-st
 constexpr int kXcds = 8;                // MI355X: workgroups go round-robin over 8 XCDs
 
 // Grid of a reducing kernel over n items: grid-stride with at most kMaxReduceBlocks
-// blocks, so the finalisation reads a bounded number of partials.
+// blocks, so the finalisation reads a bounded number of partials.  Always a
+// multiple of the XCD count: the velocity pass gives each XCD a contiguous
+// slice of the band only on such grids, and in the batch engine every job's
+// segment of these grids then starts on a multiple of 8, so a virtual block's
+// local index mod 8 is its XCD (surplus blocks write identity partials).
 __host__ __device__ inline int reduce_grid(int64_t n) {
   int64_t b = (n + kBandThreads - 1) / kBandThreads;
-  return (int)(b < 1 ? 1 : (b > kMaxReduceBlocks ? kMaxReduceBlocks : b));
+  b = b < 1 ? 1 : b;
+  b = (b + kXcds - 1) / kXcds * kXcds;
+  return (int)(b > kMaxReduceBlocks ? kMaxReduceBlocks : b);
 }
 
 // Block remap giving XCD x (blocks go round-robin over the 8 XCDs, each with

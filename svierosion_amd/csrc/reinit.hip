@@ -396,6 +396,9 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
                 "bad reinit parameters");
   Workspace w;
   if (!carve_workspace(ws, ws_bytes, n, w)) return SVIMX_EBADARG;
+  // the work set holds at most every node: larger buffers are used up to n
+  // (the compactions and the workspace's chunk counts are sized for n)
+  work_cap = std::min<int64_t>(work_cap, n);
   hipStream_t st = as_stream(stream);
   const dim3 tb(256);
   ReduceSpec spec;

@@ -115,3 +115,25 @@ def test_vertex_fields_follow_the_reference_definition(device, law_kw):
     err_v = np.abs(vd["speed"] - ref).max() / max(np.abs(ref).max(), 1e-30)
     print(f"{law_kw['law']}: {len(mesh.vertices)} vertices, field error {err_q:.2e}, speed error {err_v:.2e}")
     assert err_v < 1e-5
+
+
+def test_small_grid_job_in_the_engine_equals_erode(device):
+    """A job whose band is a large share of its grid (cube res 64: 262k band nodes of
+    90^3) in the batch engine equals erode(): the engine grows its reinitialisation
+    work buffers by a quarter over the request, past the node count, and the
+    reinitialisation uses them up to the grid size (it refused the compaction
+    before)."""
+    from svierosion_amd import ErosionParams, erode
+    from svierosion_amd.batch import Request, serve
+    from svierosion_amd.shapes import cube
+    p = ErosionParams(law="svi", radius=0.1, k=1, s=1, t=0.93, resolution=64, target_fraction=0.9,
+                      snapshots=(0.95,))
+    rep = serve([Request(cube(1.0), p)], solo_tail=False)  # every step in the engine
+    assert rep.completions[0].error is None, rep.completions[0].error
+    eng = rep.completions[0].result
+    run = erode(cube(1.0), p)
+    assert eng.stop_reason == run.stop_reason == "target_volume"
+    assert [(r.volume, r.dt, r.reinit) for r in eng.history] == [(r.volume, r.dt, r.reinit) for r in run.history]
+    assert any(r.reinit for r in eng.history)
+    assert [f for f, _, _ in eng.snapshots] == [f for f, _, _ in run.snapshots]
+    assert np.array_equal(eng.mesh.vertices, run.mesh.vertices) and np.array_equal(eng.mesh.faces, run.mesh.faces)

@@ -233,7 +233,7 @@ int svimx_nr_sum(void* stream, void* plan, const float* occupancy, float* nr_out
  * nx*ny*nz; on entry it holds the nb nodes of the current band).
  * `inside_outside` is the count of inside nodes outside the current band.
  * work_idx/work_info hold work_cap nodes, geo 8*work_cap floats; mask_a/mask_b
- * are dense byte fields.
+ * are dense byte fields.  margin >= 2 (the sweeps read 2 nodes along each axis).
  * Synchronises once, at the end.  A work set above work_cap returns
  * SVIMX_ECAPACITY with info[9] = its size and leaves phi and the band as they
  * were.  info (host, 10 int64): new band count, inside nodes outside the new

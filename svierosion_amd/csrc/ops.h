@@ -413,53 +413,74 @@ __device__ __forceinline__ ReinitGeo reinit_geometry(const float* __restrict__ f
 // -sgn(phi0)(|grad phi| - 1) with ENO2 differences and the subcell fix
 // (svierosion/kernels.py:363-379).
 // The 13 values the right-hand side reads: the node and +-1, +-2 along each
-// axis, loaded apart from the arithmetic so a caller can put the loads of
-// several nodes in flight before computing any of them.
+// axis (axis 0 = x, 2 = z), loaded apart from the arithmetic so a caller can
+// put the loads of several nodes in flight before computing any of them.
 struct Stencil13 {
   float c, m2[3], m1[3], p1[3], p2[3];
 };
-__device__ __forceinline__ Stencil13 load_stencil13(const float* __restrict__ f, int64_t n, const Dims& d) {
-  const int64_t st[3] = {d.sx(), d.sy(), 1};
+
+// Loads through byte offsets of type Off from the field's base: with uint32_t
+// offsets (grids below 2^30 nodes) every load is one scalar-base + 32-bit
+// vector-offset instruction and the z neighbours share the node's offset
+// through the instruction's immediate field.
+template <typename Off>
+__device__ __forceinline__ Stencil13 load_stencil13_at(const float* __restrict__ f, Off node, Off sx4, Off sy4) {
+  const char* B = reinterpret_cast<const char*>(f);
+  const Off o = node * (Off)4;
+  const float* p = reinterpret_cast<const float*>(B + o);
   Stencil13 S;
-  S.c = f[n];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int64_t s = st[a];
-    S.m2[a] = f[n - 2 * s];
-    S.m1[a] = f[n - s];
-    S.p1[a] = f[n + s];
-    S.p2[a] = f[n + 2 * s];
-  }
+  S.c = p[0];
+  S.m2[2] = p[-2];
+  S.m1[2] = p[-1];
+  S.p1[2] = p[1];
+  S.p2[2] = p[2];
+  S.m2[1] = *reinterpret_cast<const float*>(B + (o - 2 * sy4));
+  S.m1[1] = *reinterpret_cast<const float*>(B + (o - sy4));
+  S.p1[1] = *reinterpret_cast<const float*>(B + (o + sy4));
+  S.p2[1] = *reinterpret_cast<const float*>(B + (o + 2 * sy4));
+  S.m2[0] = *reinterpret_cast<const float*>(B + (o - 2 * sx4));
+  S.m1[0] = *reinterpret_cast<const float*>(B + (o - sx4));
+  S.p1[0] = *reinterpret_cast<const float*>(B + (o + sx4));
+  S.p2[0] = *reinterpret_cast<const float*>(B + (o + 2 * sx4));
   return S;
 }
 
-__device__ __forceinline__ float reinit_rhs_of(const Stencil13& S, float h, uint32_t flags, const float x[6]) {
-  const float ih = rcp(h), ihh = ih * ih;
+// minmod(a, b) as the median of (a, b, 0), one v_med3_f32: the smaller
+// magnitude when a and b share a sign, 0 otherwise.  It equals the
+// sign-product form (a b > 0 ? ...) except where a b underflows to 0 for two
+// nonzero values of one sign (|a b| < 1e-38: second differences of a
+// distance field are never that small unless one is exactly 0).
+__device__ __forceinline__ float minmod_med3(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.0f); }
+
+// Right-hand side at one node.  s = sgn(phi0) in {+1, -1, 0} (info bits 6-7),
+// ih = 1/h, ihh = 1/h^2.  The Godunov upwind choice of the reference,
+//   outside: max(max(Dm,0)^2, min(Dp,0)^2)   inside: max(min(Dm,0)^2, max(Dp,0)^2),
+// is max(s Dm, -s Dp, 0)^2 (squaring is monotone on non-negative floats, so
+// the maximum may be taken before it); s is folded into the factors of each
+// one-sided difference (a sign flip of one factor flips the rounded result
+// exactly), so each axis costs one v_max3_f32 after the differences.
+// sqrt is the hardware v_sqrt_f32 (1 ulp).
+__device__ __forceinline__ float reinit_rhs_lean(const Stencil13& S, float s, float ih, float ihh, uint32_t flags,
+                                                 const float x[6]) {
   const float c = S.c;
-  uint32_t sb = (flags >> 6) & 3u;
-  float sgn = sb == 1u ? 1.0f : (sb == 2u ? -1.0f : 0.0f);
-  bool outside = sgn > 0.0f;
+  const float sih = s * ih, sihh = s * ihh, sc = s * c;
   float g2 = 0.0f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float m2 = S.m2[a], m1 = S.m1[a], p1 = S.p1[a], p2 = S.p2[a];
-    float mm_p = minmod(p1 - 2.0f * c + m1, p2 - 2.0f * p1 + c);
-    float mm_m = minmod(p1 - 2.0f * c + m1, c - 2.0f * m1 + m2);
-    bool cp = (flags >> (2 * a)) & 1u;
-    bool cm = (flags >> (2 * a + 1)) & 1u;
-    float xp = x[2 * a], xm = x[2 * a + 1];
-    float Dp = cp ? (-c * rcp(xp) - 0.5f * xp * mm_p * ihh) : ((p1 - c - 0.5f * mm_p) * ih);
-    float Dm = cm ? (c * rcp(xm) + 0.5f * xm * mm_m * ihh) : ((c - m1 + 0.5f * mm_m) * ih);
-    float pos = fmaxf(sq(fmaxf(Dm, 0.0f)), sq(fminf(Dp, 0.0f)));
-    float neg = fmaxf(sq(fminf(Dm, 0.0f)), sq(fmaxf(Dp, 0.0f)));
-    g2 = g2 + (outside ? pos : neg);
+    const float d0 = p1 - 2.0f * c + m1;
+    const float mm_p = minmod_med3(d0, p2 - 2.0f * p1 + c);
+    const float mm_m = minmod_med3(d0, c - 2.0f * m1 + m2);
+    const bool cp = (flags >> (2 * a)) & 1u;
+    const bool cm = (flags >> (2 * a + 1)) & 1u;
+    const float xp = x[2 * a], xm = x[2 * a + 1];
+    // s * Dp and s * Dm
+    const float sDp = cp ? (-sc * rcp(xp) - 0.5f * xp * mm_p * sihh) : ((p1 - c - 0.5f * mm_p) * sih);
+    const float sDm = cm ? (sc * rcp(xm) + 0.5f * xm * mm_m * sihh) : ((c - m1 + 0.5f * mm_m) * sih);
+    const float g = fmaxf(fmaxf(sDm, -sDp), 0.0f);  // one v_max3_f32
+    g2 = __fadd_rn(g2, __fmul_rn(g, g));
   }
-  return -sgn * (sqrtf(g2) - 1.0f);
-}
-
-__device__ __forceinline__ float reinit_rhs(const float* __restrict__ f, int64_t n, const Dims& d,
-                                            float h, uint32_t flags, const float x[6]) {
-  return reinit_rhs_of(load_stencil13(f, n, d), h, flags, x);
+  return -s * (__builtin_amdgcn_sqrtf(g2) - 1.0f);
 }
 
 // ---------------------------------------------------------------- point-triangle distance

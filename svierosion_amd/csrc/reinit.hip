@@ -263,31 +263,33 @@ __global__ void k_reinit_geo(const float* __restrict__ phi, const int32_t* __res
   info[i] = (uint8_t)(g.flags & 0xFFu);
 }
 
-// mode 0: out = in + dtau R(in);  mode 1: out = (out + in + dtau R(in)) / 2.
-// Each thread takes kSweepNodes work nodes kBandThreads apart (coalesced) and
+// Mode 0: out = in + dtau R(in);  mode 1: out = (out + in + dtau R(in)) / 2.
+// Each thread takes kSweepNodes work nodes kSweepThreads apart (coalesced) and
 // issues the stencil loads of all of them before computing any, so a wave
 // keeps kSweepNodes x 13 gathers in flight.  The launch is sized for the
-// capacity; blocks past the device work count exit.
+// capacity; blocks past the device work count exit.  Off = uint32_t addresses
+// the field by 32-bit byte offsets (grids below 2^30 nodes), int64_t beyond.
 constexpr int kSweepNodes = 2;
-__global__ void __launch_bounds__(256) k_reinit_sweep(int mode, const float* __restrict__ in, float* out,
-                                                      const int32_t* __restrict__ idx, const long long* ctr,
-                                                      int64_t cap, Dims d, float h, float dtau0,
-                                                      const uint8_t* __restrict__ info,
-                                                      const float* __restrict__ geo) {
-  const int64_t n = work_count(ctr, cap);
+constexpr int kSweepThreads = 256;
+constexpr int kSweepPerBlock = kSweepThreads * kSweepNodes;
+template <int Mode, typename Off>
+__global__ void __launch_bounds__(kSweepThreads)
+    k_reinit_sweep(const float* __restrict__ in, float* out, const int32_t* __restrict__ idx, const long long* ctr,
+                   int64_t cap, Off sx4, Off sy4, float ih, float ihh, float dtau0,
+                   const uint8_t* __restrict__ info, const float* __restrict__ geo) {
+  const uint32_t n = (uint32_t)work_count(ctr, cap);  // < 2^31
   // XCD-contiguous slices of the (sorted) work list keep stencil neighbours in
   // one L2: the blocks needed for n, taken in XCD order
-  const int64_t need = (n + blockDim.x * kSweepNodes - 1) / (blockDim.x * kSweepNodes);
-  const int64_t g8 = (need + kXcds - 1) / kXcds;  // blocks per XCD
-  const int64_t xcd = blockIdx.x % kXcds, lbx = blockIdx.x / kXcds;
+  const uint32_t need = (n + kSweepPerBlock - 1) / kSweepPerBlock;
+  const uint32_t g8 = (need + kXcds - 1) / kXcds;  // blocks per XCD
+  const uint32_t xcd = blockIdx.x % kXcds, lbx = blockIdx.x / kXcds;
   if (lbx >= g8) return;  // block-uniform
-  const int64_t lb = xcd * g8 + lbx;
-  const int64_t base = lb * blockDim.x * kSweepNodes + threadIdx.x;
-  int64_t node[kSweepNodes];
+  const uint32_t base = (xcd * g8 + lbx) * kSweepPerBlock + threadIdx.x;
+  int32_t node[kSweepNodes];
   uint32_t word[kSweepNodes];
 #pragma unroll
   for (int k = 0; k < kSweepNodes; ++k) {
-    const int64_t i = base + (int64_t)k * blockDim.x;
+    const uint32_t i = base + k * kSweepThreads;
     node[k] = i < n ? idx[i] : -1;
     word[k] = i < n ? info[i] : 0u;
   }
@@ -296,27 +298,46 @@ __global__ void __launch_bounds__(256) k_reinit_sweep(int mode, const float* __r
 #pragma unroll
   for (int k = 0; k < kSweepNodes; ++k) {
     if (node[k] >= 0) {
-      S[k] = load_stencil13(in, node[k], d);
-      prev[k] = mode == 0 ? 0.0f : out[node[k]];
+      S[k] = load_stencil13_at<Off>(in, (Off)node[k], sx4, sy4);
+      prev[k] = Mode == 0 ? 0.0f : out[node[k]];
     }
   }
 #pragma unroll
   for (int k = 0; k < kSweepNodes; ++k) {
     if (node[k] < 0) continue;
-    const int64_t i = base + (int64_t)k * blockDim.x;
+    const uint32_t i = base + k * kSweepThreads;
     float dtau = dtau0;
     float x[6] = {0, 0, 0, 0, 0, 0};
     if (word[k] & 63u) {
-      const float* g = geo + 8 * i;
+      const float* g = geo + 8 * (size_t)i;
       dtau = g[0];
       for (int q = 0; q < 6; ++q) x[q] = g[1 + q];
     }
-    const float r = reinit_rhs_of(S[k], h, word[k], x);
-    if (mode == 0) {
+    const uint32_t sb = (word[k] >> 6) & 3u;
+    const float s = sb == 1u ? 1.0f : (sb == 2u ? -1.0f : 0.0f);
+    const float r = reinit_rhs_lean(S[k], s, ih, ihh, word[k], x);
+    if (Mode == 0) {
       out[node[k]] = S[k].c + dtau * r;
     } else {
       out[node[k]] = 0.5f * (prev[k] + S[k].c + dtau * r);
     }
+  }
+}
+
+template <typename Off>
+static void launch_sweeps(hipStream_t st, int iterations, float* phi, float* scratch, const int32_t* work_idx,
+                          const long long* ctr, int64_t work_cap, const Dims& d, float h, float dtau0,
+                          const uint8_t* info_b, const float* geo) {
+  // blocks for the largest work set, kXcds times the per-XCD share (idle ones exit)
+  const int64_t need = (work_cap + kSweepPerBlock - 1) / kSweepPerBlock;
+  const int gs = (int)(((need + kXcds - 1) / kXcds) * kXcds);
+  const Off sx4 = (Off)(d.sx() * 4), sy4 = (Off)(d.sy() * 4);
+  const float ih = 1.0f / h, ihh = ih * ih;
+  for (int it = 0; it < iterations; ++it) {
+    hipLaunchKernelGGL((k_reinit_sweep<0, Off>), dim3(gs), dim3(kSweepThreads), 0, st, phi, scratch, work_idx, ctr,
+                       work_cap, sx4, sy4, ih, ihh, dtau0, info_b, geo);
+    hipLaunchKernelGGL((k_reinit_sweep<1, Off>), dim3(gs), dim3(kSweepThreads), 0, st, scratch, phi, work_idx, ctr,
+                       work_cap, sx4, sy4, ih, ihh, dtau0, info_b, geo);
   }
 }
 
@@ -392,8 +413,8 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
                     info,
                 SVIMX_EBADARG, "null reinit buffer");
   SVIMX_REQUIRE(phi != scratch && phi != phi_copy && scratch != phi_copy, SVIMX_EBADARG, "aliased reinit buffers");
-  SVIMX_REQUIRE(iterations >= 0 && dilation >= 0 && margin >= 0 && work_cap >= 0, SVIMX_EBADARG,
-                "bad reinit parameters");
+  SVIMX_REQUIRE(iterations >= 0 && dilation >= 0 && margin >= 2 && work_cap >= 0, SVIMX_EBADARG,
+                "bad reinit parameters (the sweeps' stencils reach 2 nodes: margin >= 2)");
   Workspace w;
   if (!carve_workspace(ws, ws_bytes, n, w)) return SVIMX_EBADARG;
   // the work set holds at most every node: larger buffers are used up to n
@@ -455,17 +476,11 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
                        (float)h, (float)cfl, info_b, geo);
     SVIMX_CHECK_LAUNCH();
     const float dtau0 = (float)cfl * fmaxf((float)h, 1e-3f * (float)h);
-    // blocks for the largest work set, kXcds times the per-XCD share (idle ones exit)
-    const int64_t need = (work_cap + 256 * kSweepNodes - 1) / (256 * kSweepNodes);
-    const int gs = (int)(((need + kXcds - 1) / kXcds) * kXcds);
-    for (int it = 0; it < iterations; ++it) {
-      hipLaunchKernelGGL(k_reinit_sweep, dim3(gs), tb, 0, st, 0, phi, scratch, work_idx, ctr, work_cap, d, (float)h,
-                         dtau0, info_b, geo);
-      SVIMX_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_reinit_sweep, dim3(gs), tb, 0, st, 1, scratch, phi, work_idx, ctr, work_cap, d, (float)h,
-                         dtau0, info_b, geo);
-      SVIMX_CHECK_LAUNCH();
-    }
+    if (n * 4 < ((int64_t)1 << 32))
+      launch_sweeps<uint32_t>(st, iterations, phi, scratch, work_idx, ctr, work_cap, d, (float)h, dtau0, info_b, geo);
+    else
+      launch_sweeps<int64_t>(st, iterations, phi, scratch, work_idx, ctr, work_cap, d, (float)h, dtau0, info_b, geo);
+    SVIMX_CHECK_LAUNCH();
   }
   hipLaunchKernelGGL(k_rebuild_work, dim3(nblk), tb, 0, st, phi, scratch, phi_copy, occupancy, work_idx, ctr,
                      work_cap, d, capf, thr, w.partials);

@@ -70,6 +70,14 @@ class LevelSet:
 
     @property
     def band(self) -> torch.Tensor:
+        """Flat indices of the band nodes, sorted, as int64 (the reference's
+        ``LevelSet.band`` type, svierosion/levelset.py:54-86): a copy of the
+        resident int32 list that the kernels read (``band32``)."""
+        return self._band[: self.nb].long()
+
+    @property
+    def band32(self) -> torch.Tensor:
+        """The resident int32 band list itself (a view, no copy)."""
         return self._band[: self.nb]
 
     def results(self) -> np.ndarray:
@@ -100,7 +108,7 @@ class LevelSet:
 
     def interface_nodes(self, idx: torch.Tensor | None = None) -> torch.Tensor:
         """Band nodes having a 6-neighbour of the opposite (or zero) sign."""
-        idx = self.band if idx is None else _idx32(idx, self.device)
+        idx = self.band32 if idx is None else _idx32(idx, self.device)
         out = torch.empty(idx.numel(), dtype=torch.uint8, device=self.device)
         _abi.check(_abi.lib.svimx_op_interface_mask(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx), idx.numel(),
                                                     *self.dims, _abi.ptr(out)))
@@ -135,7 +143,7 @@ class LevelSet:
 
     # ------------------------------------------------- volume fractions
     def cell_fractions(self, idx: torch.Tensor | None = None) -> torch.Tensor:
-        idx = self.band if idx is None else _idx32(idx, self.device)
+        idx = self.band32 if idx is None else _idx32(idx, self.device)
         out = torch.empty(idx.numel(), dtype=torch.float32, device=self.device)
         _abi.check(_abi.lib.svimx_op_cell_fractions(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx),
                                                     idx.numel(), *self.dims, _abi.ptr(out)))

@@ -98,3 +98,13 @@ class TriMesh:
 
     def copy(self) -> "TriMesh":
         return TriMesh(self.vertices.copy(), self.faces.copy(), {k: v.copy() for k, v in self.vertex_data.items()})
+
+
+def __getattr__(name):
+    """``read_ply`` / ``write_ply`` live in ``plyio`` (which imports TriMesh from
+    here); they resolve lazily so both import paths of the reference work
+    (``from svierosion.meshio import read_ply, write_ply``, svierosion/meshio.py:264,313)."""
+    if name in ("read_ply", "write_ply"):
+        from . import plyio
+        return getattr(plyio, name)
+    raise AttributeError(name)

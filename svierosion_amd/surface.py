@@ -45,7 +45,7 @@ def extract_surface(ls: LevelSet, project: bool = True, max_shift: float = 0.5) 
     """Zero set of the level set as an outward-oriented, watertight triangle mesh."""
     if ls.nb == 0:
         return TriMesh(np.zeros((0, 3)), np.zeros((0, 3), dtype=np.int64))
-    verts, faces = device_mesh(ls.flat, ls.band, ls.nb, ls.dims, max_shift if project else 0.0)
+    verts, faces = device_mesh(ls.flat, ls.band32, ls.nb, ls.dims, max_shift if project else 0.0)
     grid = ls.grid
     mesh = TriMesh(np.asarray(grid.origin) + grid.spacing * verts.astype(np.float64), faces.astype(np.int64))
     if len(faces) and mesh.signed_volume() < 0:

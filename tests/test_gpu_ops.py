@@ -32,6 +32,7 @@ def test_band_rebuild_matches(device):
     ls, ref, g = _pair()
     assert ls.nb == ref.band.size
     assert np.array_equal(ls.band.cpu().numpy(), ref.band)
+    assert ls.band.dtype == torch.int64 and ls.band32.dtype == torch.int32  # the reference's band type
     assert np.array_equal(ls.flat.cpu().numpy(), ref.flat)
     n_in = int((ref.flat < 0).sum()) - int((ref.flat[ref.band] < 0).sum())
     assert ls.inside_outside == n_in

@@ -38,6 +38,18 @@ def test_ply_roundtrip_binary_and_ascii():
             assert np.allclose(r.vertex_data["quality"], q, atol=1e-6)
 
 
+def test_reference_import_paths():
+    """The reference's import paths: read_ply / write_ply from the package and from
+    its meshio module (svierosion/__init__.py:25, svierosion/meshio.py:264,313)."""
+    import svierosion_amd
+    from svierosion_amd import meshio, plyio
+    from svierosion_amd.meshio import read_ply, write_ply
+    assert read_ply is plyio.read_ply and write_ply is plyio.write_ply
+    assert svierosion_amd.read_ply is read_ply and svierosion_amd.TriMesh is meshio.TriMesh
+    with pytest.raises(AttributeError):
+        meshio.not_a_function
+
+
 def test_ply_quads_are_triangulated():
     from svierosion_amd.plyio import read_ply
     txt = ("ply\nformat ascii 1.0\nelement vertex 4\nproperty float x\nproperty float y\n"

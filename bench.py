@@ -118,12 +118,33 @@ def cpu_baseline(ls, law, steps=6):
                        f"(oracle/cpu/svicpu.cpp) on {threads} threads: {el:.1f} s")}
 
 
+def write_records(path, leg: str, reqs, report) -> None:
+    """Append one JSON line per job of a batched or served leg: arrival,
+    admission and finish times, latency, steps, law and grid."""
+    if not path:
+        return
+    with open(path, "a") as fh:
+        for c in report.completions:
+            r = reqs[c.index]
+            res = c.result
+            fh.write(json.dumps({"leg": leg, "index": c.index, "law": r.params.law, "k": r.params.k,
+                                 "s": r.params.s, "resolution": r.params.resolution,
+                                 "target": r.params.target_fraction, "triangles": int(len(r.mesh.faces)),
+                                 "grid": list(res.grid.shape) if res is not None else None,
+                                 "arrival": round(c.arrival, 4), "admitted": round(c.admitted, 4),
+                                 "finished": round(c.finished, 4), "latency": round(c.latency, 4),
+                                 "steps": c.steps, "error": c.error,
+                                 "timings": {k: round(v, 4) for k, v in res.timings.items()} if res else None})
+                     + "\n")
+
+
 def _same_history(a, b) -> bool:
     return len(a) == len(b) and all(x.volume == y.volume and x.dt == y.dt and x.reinit == y.reinit
                                     for x, y in zip(a, b))
 
 
-def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, slack: float, progress: bool = False):
+def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, slack: float, progress: bool = False,
+                records: str | None = None):
     """The fragment mix (first n_jobs of the 64-fragment set) eroded batched and one at a time."""
     from svierosion_amd.batch import Request, serve, serve_sequential
     from svierosion_amd.workloads import fragment_mix
@@ -135,6 +156,8 @@ def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, sla
     print(f"[bench] batched {n_jobs} jobs: {rb.summary()}", file=sys.stderr, flush=True)
     rs = serve_sequential(reqs, verbose=progress)
     print(f"[bench] sequential {n_jobs} jobs: {rs.summary()}", file=sys.stderr, flush=True)
+    write_records(records, "batched", reqs, rb)
+    write_records(records, "batched_sequential", reqs, rs)
     same = all(cb.error is None and cs.error is None and _same_history(cb.result.history, cs.result.history)
                for cb, cs in zip(rb.completions, rs.completions))
     laws = {}
@@ -154,7 +177,7 @@ def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, sla
 
 
 def served_leg(n_req: int, rate: float, workers: int, schedule: str, budget_nodes: int, slack: float,
-               progress: bool = False):
+               progress: bool = False, records: str | None = None):
     """Poisson arrivals of the served workload (first n_req requests), continuous batching vs FIFO."""
     from svierosion_amd.batch import serve, serve_sequential
     from svierosion_amd.workloads import poisson_requests
@@ -163,6 +186,8 @@ def served_leg(n_req: int, rate: float, workers: int, schedule: str, budget_node
     print(f"[bench] served {n_req} requests at {rate:.4f}/s: {rv.summary()}", file=sys.stderr, flush=True)
     rq = serve_sequential(reqs, verbose=progress)
     print(f"[bench] sequential FIFO: {rq.summary()}", file=sys.stderr, flush=True)
+    write_records(records, "served", reqs, rv)
+    write_records(records, "served_sequential", reqs, rq)
     same = all(a.error is None and b.error is None and _same_history(a.result.history, b.result.history)
                for a, b in zip(rv.completions, rq.completions))
     return {"workload": (f"first {n_req} of 512 requests (fragments as in the mix, res log-uniform 64-400, "
@@ -273,6 +298,8 @@ def main(argv=None):
                     help="arrivals per second of the served leg (default: --serve-load x the sequential rate)")
     ap.add_argument("--workers", type=int, default=2, help="set-up / meshing threads of the batched forms")
     ap.add_argument("--progress", action="store_true", help="print each completed job of the batched legs (stderr)")
+    ap.add_argument("--records", default=None, metavar="PATH",
+                    help="append one JSON line per job of the batched and served legs to PATH")
     ap.add_argument("--batch-schedule", choices=["lockstep", "longest", "fifo"], default="lockstep")
     ap.add_argument("--serve-schedule", choices=["fifo", "longest", "lockstep"], default="fifo")
     ap.add_argument("--budget-nodes", type=int, default=None, help="band nodes a tick may hold (schedule budget)")
@@ -388,12 +415,13 @@ def main(argv=None):
     budget = DEFAULT_BUDGET_NODES if args.budget_nodes is None else args.budget_nodes
     slack = DEFAULT_SLACK if args.slack is None else args.slack
     if args.batch_jobs > 0:
-        batched, rs = batched_leg(args.batch_jobs, args.workers, args.batch_schedule, budget, slack, args.progress)
+        batched, rs = batched_leg(args.batch_jobs, args.workers, args.batch_schedule, budget, slack, args.progress,
+                                  args.records)
         seq_rate = len(rs.done) / rs.makespan if rs.makespan > 0 else None
     if args.serve_requests > 0:
         rate = args.serve_rate or args.serve_load * (seq_rate if seq_rate else 0.5)
         served = served_leg(args.serve_requests, rate, args.workers, args.serve_schedule, budget, slack,
-                            args.progress)
+                            args.progress, args.records)
     line = {"metric": "RK3 steps/s + HBM GB/s @res400 1.3M-tri; batched erosions/hour, p99 latency",
             "value": round(value, 3), "unit": "RK3 steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
             "ms_per_step": round(1000 * elapsed / max(steps, 1), 3), "higher_is_better": True, "scaling": "weak",

@@ -94,81 +94,70 @@ __device__ __forceinline__ WaveSplit wave_split(int64_t n, int64_t lb, int64_t G
 }
 
 // ---------------------------------------------------------------- run-mode summation
+// A column's sum over its 2k+1 z taps in the symmetric order that every
+// run-mode shape uses: the weights are mirror-symmetric in z (checked when the
+// plan is built), so for d = k .. 1 the pair win[j - d] + win[j + d] takes
+// one FMA with the weight at |dz| = d, and the centre tap comes last: k adds
+// and k + 1 FMAs per output instead of 2k + 1 FMAs.  Kernels that apply
+// several columns to one window form the pair sums once (column_taps_ps);
+// the sums are the same floats either way, so every shape agrees bit for bit.
 template <int K, int MP, int RL, int WIN>
 __device__ __forceinline__ void column_taps(const float* __restrict__ wc, const float (&win)[WIN],
                                             float (&part)[RL]) {
   if constexpr (K <= MP) {
 #pragma unroll
-    for (int t = 0; t <= 2 * K; ++t) {
-      const float wt = wc[t];
+    for (int d = K; d >= 1; --d) {
+      const float wt = wc[K + d];
 #pragma unroll
-      for (int j = 0; j < RL; ++j) part[j] += wt * win[j + MP - K + t];
+      for (int j = 0; j < RL; ++j) part[j] = __builtin_fmaf(wt, win[j + MP - d] + win[j + MP + d], part[j]);
     }
+    const float w0 = wc[K];
+#pragma unroll
+    for (int j = 0; j < RL; ++j) part[j] = __builtin_fmaf(w0, win[j + MP], part[j]);
   }
 }
 
-template <int MP, int RL>
-__device__ __forceinline__ void taps_of(int k, const float* __restrict__ wc, const float (&win)[RL + 2 * MP],
-                                        float (&part)[RL]) {
-  switch (k) {
-    case 0: column_taps<0, MP, RL>(wc, win, part); break;
-    case 1: column_taps<1, MP, RL>(wc, win, part); break;
-    case 2: column_taps<2, MP, RL>(wc, win, part); break;
-    case 3: column_taps<3, MP, RL>(wc, win, part); break;
-    case 4: column_taps<4, MP, RL>(wc, win, part); break;
-    case 5: column_taps<5, MP, RL>(wc, win, part); break;
-    case 6: column_taps<6, MP, RL>(wc, win, part); break;
-    case 7: column_taps<7, MP, RL>(wc, win, part); break;
-    case 8: column_taps<8, MP, RL>(wc, win, part); break;
-    case 9: column_taps<9, MP, RL>(wc, win, part); break;
-    case 10: column_taps<10, MP, RL>(wc, win, part); break;
-    case 11: column_taps<11, MP, RL>(wc, win, part); break;
-    case 12: column_taps<12, MP, RL>(wc, win, part); break;
-    case 13: column_taps<13, MP, RL>(wc, win, part); break;
-    case 14: column_taps<14, MP, RL>(wc, win, part); break;
-    case 15: column_taps<15, MP, RL>(wc, win, part); break;
-    case 16: column_taps<16, MP, RL>(wc, win, part); break;
-    default: break;
-  }
-}
-
-// Taps of a column from its record r (r[0]: k as int bits, r[1 + d]: the
-// weight at |dz| = d): the same products, in the same order, as
-// column_taps over the column's 2k+1 weights.
-template <int K, int MP, int RL, int WIN>
-__device__ __forceinline__ void column_taps_rec(const float (&r)[MP + 2], const float (&win)[WIN],
-                                                float (&part)[RL]) {
+// All taps of one window in the multi-line kernels: the window feeds the
+// columns of lines yy = ylo .. yhi (one column each, records r[yy]), whose
+// half-heights are at most K.  For d = K .. 1 the pair sums of the window are
+// formed once and each column takes its FMA with the weight at |dz| = d --
+// 0 where d exceeds the column's own half-height (records are zero-filled
+// past k), and fma(0, p, s) = s exactly for the finite, non-negative pair
+// sums and partials here -- then the centre taps.  Per column these are the
+// operations of column_taps in its order, so the sums equal the one-line
+// kernels' bit for bit.  ALL: every line of the window has a column (the
+// windows inside a group), no per-line test.
+template <int K, bool ALL, int MP, int RL, int Y, int WIN>
+__device__ __forceinline__ void window_taps_k(const float (&r)[Y][MP + 2], int ylo, int yhi,
+                                              const float (&win)[WIN], float (&part)[Y][RL]) {
   if constexpr (K <= MP) {
 #pragma unroll
-    for (int t = 0; t <= 2 * K; ++t) {
-      const float wt = r[1 + (t < K ? K - t : t - K)];
+    for (int d = K; d >= 1; --d) {
+      float pr[RL];
 #pragma unroll
-      for (int j = 0; j < RL; ++j) part[j] += wt * win[j + MP - K + t];
+      for (int j = 0; j < RL; ++j) pr[j] = win[j + MP - d] + win[j + MP + d];
+#pragma unroll
+      for (int yy = 0; yy < Y; ++yy)
+        if (ALL || (yy >= ylo && yy <= yhi))
+#pragma unroll
+          for (int j = 0; j < RL; ++j) part[yy][j] = __builtin_fmaf(r[yy][1 + d], pr[j], part[yy][j]);
     }
+#pragma unroll
+    for (int yy = 0; yy < Y; ++yy)
+      if (ALL || (yy >= ylo && yy <= yhi))
+#pragma unroll
+        for (int j = 0; j < RL; ++j) part[yy][j] = __builtin_fmaf(r[yy][1], win[j + MP], part[yy][j]);
   }
 }
 
-template <int MP, int RL>
-__device__ __forceinline__ void taps_rec(const float (&r)[MP + 2], const float (&win)[RL + 2 * MP],
-                                         float (&part)[RL]) {
-  switch (__float_as_int(r[0])) {
-    case 0: column_taps_rec<0, MP, RL>(r, win, part); break;
-    case 1: column_taps_rec<1, MP, RL>(r, win, part); break;
-    case 2: column_taps_rec<2, MP, RL>(r, win, part); break;
-    case 3: column_taps_rec<3, MP, RL>(r, win, part); break;
-    case 4: column_taps_rec<4, MP, RL>(r, win, part); break;
-    case 5: column_taps_rec<5, MP, RL>(r, win, part); break;
-    case 6: column_taps_rec<6, MP, RL>(r, win, part); break;
-    case 7: column_taps_rec<7, MP, RL>(r, win, part); break;
-    case 8: column_taps_rec<8, MP, RL>(r, win, part); break;
-    case 9: column_taps_rec<9, MP, RL>(r, win, part); break;
-    case 10: column_taps_rec<10, MP, RL>(r, win, part); break;
-    case 11: column_taps_rec<11, MP, RL>(r, win, part); break;
-    case 12: column_taps_rec<12, MP, RL>(r, win, part); break;
-    case 13: column_taps_rec<13, MP, RL>(r, win, part); break;
-    case 14: column_taps_rec<14, MP, RL>(r, win, part); break;
-    case 15: column_taps_rec<15, MP, RL>(r, win, part); break;
-    case 16: column_taps_rec<16, MP, RL>(r, win, part); break;
+template <bool ALL, int MP, int RL, int Y>
+__device__ __forceinline__ void window_taps(int k, const float (&r)[Y][MP + 2], int ylo, int yhi,
+                                            const float (&win)[RL + 2 * MP], float (&part)[Y][RL]) {
+  switch (k) {
+#define SVIMX_WT(K) case K: window_taps_k<K, ALL, MP, RL, Y>(r, ylo, yhi, win, part); break;
+    SVIMX_WT(0) SVIMX_WT(1) SVIMX_WT(2) SVIMX_WT(3) SVIMX_WT(4) SVIMX_WT(5) SVIMX_WT(6) SVIMX_WT(7) SVIMX_WT(8)
+    SVIMX_WT(9) SVIMX_WT(10) SVIMX_WT(11) SVIMX_WT(12) SVIMX_WT(13) SVIMX_WT(14) SVIMX_WT(15) SVIMX_WT(16)
+#undef SVIMX_WT
     default: break;
   }
 }
@@ -307,36 +296,24 @@ __device__ __forceinline__ void nr_lines_half(const float* __restrict__ occ, con
       }
       first_group = false;
     }
-    if constexpr (PF) {
-      // the records of the window's columns (one per line), all loads issued first
-      float r[Y][MP + 2];
+    // the records of the window's columns (lines ylo .. yhi, one column
+    // each), all loads issued first
+    const int ylo = max(0, p.t - p.ncol + 1), yhi = min(Y - 1, p.t);
+    float r[Y][MP + 2];
+    int kmax = 0;
 #pragma unroll
-      for (int yy = 0; yy < Y; ++yy) {
-        const int ci = p.t - yy;
-        if (ci >= 0 && ci < p.ncol) {
-          const float* rec = colrec + (int64_t)(p.c0 + ci) * kNrRecStride;
+    for (int yy = 0; yy < Y; ++yy) {
+      if (yy >= ylo && yy <= yhi) {
+        const float* rec = colrec + (int64_t)(p.c0 + p.t - yy) * kNrRecStride;
 #pragma unroll
-          for (int i = 0; i < MP + 2; ++i) r[yy][i] = rec[i];
-        }
-      }
-#pragma unroll
-      for (int yy = 0; yy < Y; ++yy) {
-        const int ci = p.t - yy;
-        if (ci >= 0 && ci < p.ncol) taps_rec<MP, RL>(r[yy], win, part[yy]);
-      }
-    } else {
-#pragma unroll
-      for (int yy = 0; yy < Y; ++yy) {
-        const int ci = p.t - yy;
-        if (ci >= 0 && ci < p.ncol) {
-          const float* rec = colrec + (int64_t)(p.c0 + ci) * kNrRecStride;
-          float r[MP + 2];
-#pragma unroll
-          for (int i = 0; i < MP + 2; ++i) r[i] = rec[i];
-          taps_rec<MP, RL>(r, win, part[yy]);
-        }
+        for (int i = 0; i < MP + 2; ++i) r[yy][i] = rec[i];
+        kmax = max(kmax, __float_as_int(r[yy][0]));
       }
     }
+    if (ylo == 0 && yhi == Y - 1)
+      window_taps<true, MP, RL, Y>(kmax, r, ylo, yhi, win, part);
+    else
+      window_taps<false, MP, RL, Y>(kmax, r, ylo, yhi, win, part);
   };
   NrPos cur{ab, 0, 0, 0, 0};
   group(cur);

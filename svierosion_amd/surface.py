@@ -54,7 +54,10 @@ def extract_surface(ls: LevelSet, project: bool = True, max_shift: float = 0.5) 
 
 
 def device_mesh(phi_flat: torch.Tensor, band: torch.Tensor, nb: int, dims, max_shift: float = 0.0):
-    """(vertices in index coordinates float32 (V, 3), faces int32 (F, 3)) of the zero set over ``band``."""
+    """(vertices in index coordinates float32 (V, 3), faces int32 (F, 3)) of the zero set over ``band``
+    (flat node indices of any integer type; the kernels read int32)."""
+    if band.dtype != torch.int32 or not band.is_contiguous():
+        band = band[: int(nb)].to(dtype=torch.int32).contiguous()
     xyz, faces = C.c_void_p(), C.c_void_p()
     nv, nf = C.c_int64(), C.c_int64()
     st = _abi.stream()

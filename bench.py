@@ -280,6 +280,49 @@ def nr_shape_sweep(sim) -> int:
     return 0
 
 
+def nr_fft_timing(ls, sim, reps: int = 10) -> int:
+    """The reference's N_r method (FFT over the whole grid, svierosion/svi.py:84-93) on
+    the workload's occupancy: ms per evaluation (HIP events on the library's stream
+    over `reps` evaluations) and the largest difference from the direct summation at
+    the nodes the step reads.  One JSON line."""
+    import torch
+    from svierosion_amd import _abi
+    from svierosion_amd.svi import BallConvolution
+    g = ls.grid
+    fft = BallConvolution(g, sim.conv.radius, device=ls.device, method="fft")
+    st = _abi.stream()
+    ls.fractions(sim.conv.plan, sim.cutoff[1])  # occupancy of the current field, and the step's N_r marks
+    occ = ls.occupancy
+    out = torch.empty(g.size, dtype=torch.float32, device=ls.device)
+    ws = fft._workspace()
+    nodes = torch.arange(g.size, dtype=torch.int32, device=ls.device)
+    _abi.check(_abi.lib.svimx_nr_reset(st, fft.plan))
+    _abi.check(_abi.lib.svimx_nr_mark_nodes(st, fft.plan, _abi.ptr(nodes), nodes.numel()))
+    for _ in range(2):
+        fft.exec_marked(occ, out, ws)
+    cur = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(cur)
+    for _ in range(reps):
+        fft.exec_marked(occ, out, ws)
+    b.record(cur)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    # the direct summation at the units the current field's step reads
+    _abi.check(_abi.lib.svimx_nr_select(st, sim.conv.plan))
+    direct = torch.full((g.size,), float("nan"), dtype=torch.float32, device=ls.device)
+    _abi.check(_abi.lib.svimx_nr_sum(st, sim.conv.plan, _abi.ptr(ls.occupancy), _abi.ptr(direct), _abi.ptr(ls.ws),
+                                     ls.ws.numel()))
+    torch.cuda.synchronize()
+    ok = ~torch.isnan(direct)
+    err = float((out[ok] - direct[ok]).abs().max())
+    n = int(g.size)
+    print(json.dumps({"nr_fft": True, "grid": list(g.shape), "nodes": n, "ms_per_evaluation": round(ms, 4),
+                      "direct_nodes_compared": int(ok.sum()), "max_abs_diff_vs_direct": err,
+                      "half_spectrum_bytes_per_node_pass": 8 * (g.shape[2] // 2 + 1) / g.shape[2]}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,6 +351,8 @@ def main(argv=None):
                     help="first run a plain device copy of this many MiB (profiler byte calibration)")
     ap.add_argument("--full-erosion", choices=["scan400", "fragment256", "cube64"], default=None,
                     help="diagnostic: run erode() to the workload's target volume end to end and exit")
+    ap.add_argument("--nr-fft", action="store_true",
+                    help="time the FFT N_r method on the workload and compare it with the direct sum, then exit")
     ap.add_argument("--nr-shape-sweep", action="store_true",
                     help="diagnostic: time every N_r summation shape on the workload's field "
                          "(5 launches each, in the autotuner's order) and exit")
@@ -335,6 +380,8 @@ def main(argv=None):
     print(f"[bench] workload ready: {info}", file=sys.stderr, flush=True)
     if args.nr_shape_sweep:
         return nr_shape_sweep(sim)
+    if args.nr_fft:
+        return nr_fft_timing(ls, sim)
     sim.run(target_fraction=None, max_steps=args.warmup)
     torch.cuda.synchronize()
     if not args.no_events:

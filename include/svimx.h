@@ -134,6 +134,22 @@ int svimx_velocity_rk(void* stream, int stage, const float* phi_in, const float*
                       const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const svimx_law_t* law,
                       const float* field, const float* k12, double cut_lo, double cut_hi, double dt, float* L0,
                       void* ws, int64_t ws_bytes);
+/* svimx_velocity_rk with each band entry's closest point given (points: nb x 3
+ * float32, index coordinates, e.g. from svimx_closest_points_newton) instead of
+ * the one-step projection of the node: the reference's velocity with
+ * newton_steps > 1 (svierosion/evolve.py:118, svierosion/levelset.py:129-151). */
+int svimx_velocity_rk_points(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                             const int32_t* band, int64_t nb, int nx, int ny, int nz, double h,
+                             const svimx_law_t* law, const float* field, const float* k12, double cut_lo,
+                             double cut_hi, double dt, float* L0, const float* points, void* ws, int64_t ws_bytes);
+/* Closest points of n nodes (idx) on the zero set of phi, index coordinates
+ * (n x 3 float32): the capped projection with the central gradient, then
+ * newton_steps - 1 capped Newton steps with the tricubic value and gradient of
+ * phi, each clamped to where tricubic stencils fit.  Replaces
+ * LevelSet.closest_points (svierosion/levelset.py:129-151) for any newton_steps
+ * in [1, 64]. */
+int svimx_closest_points_newton(void* stream, const float* phi, const int32_t* idx, int64_t n, int nx, int ny,
+                                int nz, double h, int newton_steps, float* out_p);
 /* svimx_velocity without a workspace: no counterpart in the reference.  For
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  The block partials of

@@ -246,11 +246,16 @@ class LevelSet:
         out_band_inside = int((f < 0).sum()) - int((f[self.band] < 0).sum())
         return (out_band_inside + float(self.cell_fractions().astype(np.float64).sum())) * self.h ** 3
 
-    def closest_points(self, idx):
-        return S.closest_points(self.flat, idx, self.shape, self.h)
+    def closest_points(self, idx, newton_steps=1):
+        return S.closest_points(self.flat, idx, self.shape, self.h, newton_steps)
 
-    def reinitialize(self, iterations=12, cfl=0.45, dilation=3):
-        """Min (2010) subcell-fixed PDE reinitialisation.  svierosion/levelset.py:154-183."""
+    def reinitialize(self, iterations=12, cfl=0.45, dilation=3, geometry64=False):
+        """Min (2010) subcell-fixed PDE reinitialisation.  svierosion/levelset.py:154-183.
+
+        geometry64: form the crossings, subcell distances and local dtau in
+        float64 from the field's values and round them to the field's dtype once
+        (the HIP path's precision form); the reference forms them in the field's
+        dtype."""
         mask = np.zeros(self.flat.size, dtype=bool)
         mask[self.band] = True
         mask = dilate(mask.reshape(self.shape), dilation)
@@ -262,7 +267,11 @@ class LevelSet:
             mask[tuple(sl)] = False
         idx = np.nonzero(mask.reshape(-1))[0]
         f = self.flat
-        geo = S.reinit_geometry(f, idx, self.strides, self.h, cfl)
+        if geometry64 and f.dtype != np.float64:
+            geo = S.reinit_geometry(f.astype(np.float64), idx, self.strides, self.h, cfl)
+            geo = [g.astype(f.dtype) if g.dtype == np.float64 else g for g in geo]
+        else:
+            geo = S.reinit_geometry(f, idx, self.strides, self.h, cfl)
         dtau = geo[1]
         for _ in range(iterations):
             u0 = f[idx].copy()
@@ -311,8 +320,9 @@ class Simulation:
     """
 
     def __init__(self, ls: LevelSet, law, cfl=0.5, feedback_cfl=1.0, parabolic_cfl=0.25,
-                 reinit_distance=1.5, reinit_iterations=20, cutoff=(4.0, 5.5)):
+                 reinit_distance=1.5, reinit_iterations=20, cutoff=(4.0, 5.5), newton_steps=1):
         self.ls, self.law = ls, law
+        self.newton_steps = int(newton_steps)
         self.cfl, self.feedback_cfl, self.parabolic_cfl = float(cfl), float(feedback_cfl), float(parabolic_cfl)
         self.reinit_distance, self.reinit_iterations = float(reinit_distance), int(reinit_iterations)
         self.cutoff = tuple(float(c) for c in cutoff)
@@ -349,7 +359,7 @@ class Simulation:
         ls = self.ls
         idx = ls.band
         phi_b = ls.flat[idx]
-        cp = ls.closest_points(idx)
+        cp = ls.closest_points(idx, self.newton_steps)
         info = {}
         near = np.abs(phi_b) < phi_b.dtype.type(ls.h)
         if self.law.requires_svi:

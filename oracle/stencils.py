@@ -153,9 +153,11 @@ def cell_fractions(flat, idx, strides):
 
 
 # ------------------------------------------------------------------ closest points / interpolation
-def closest_points(flat, idx, shape, h):
-    """First capped Newton projection, clamped to [1, n-3.001].  svierosion/kernels.py:185-208,
-    svierosion/levelset.py:129-151 (newton_steps = 1)."""
+def closest_points(flat, idx, shape, h, newton_steps=1):
+    """Capped Newton projection, clamped to [1, n-3.001].  svierosion/kernels.py:185-208,
+    svierosion/levelset.py:129-151: the first step with the central gradient at the
+    node, each further step with the tricubic value and gradient at the current point,
+    its length capped at 1.25 |value|/h + 0.25 cells."""
     nx, ny, nz = shape
     dt = flat.dtype.type
     c = flat[idx]
@@ -167,8 +169,15 @@ def closest_points(flat, idx, shape, h):
     ijk = unravel(idx, shape).astype(flat.dtype)
     hi = [dt(n - 3.001) for n in shape]
     p = np.stack([np.minimum(np.maximum(ijk[:, a] - scale * g, 1.0), hi[a])
-                  for a, g in enumerate((gx, gy, gz))], 1)
-    return p.astype(flat.dtype)
+                  for a, g in enumerate((gx, gy, gz))], 1).astype(flat.dtype)
+    for _ in range(newton_steps - 1):  # svierosion/levelset.py:139-145
+        val, grad = tricubic_with_gradient(flat, shape, p)
+        g2 = np.maximum((grad * grad).sum(1), 1e-30)
+        length = np.abs(val) / np.sqrt(g2)
+        cap = 1.25 * np.abs(val) / dt(h) + 0.25
+        step = (val / g2 * np.minimum(cap / np.maximum(length, 1e-30), 1.0))[:, None] * grad
+        p = np.minimum(np.maximum(p - step, 1.0), np.array(hi, dtype=flat.dtype)).astype(flat.dtype)
+    return p
 
 
 def cubic_weights(t):
@@ -193,6 +202,35 @@ def tricubic(flat, shape, p):
             for c in range(4):
                 out = out + (wab * wz[c]) * flat[b0 + off + c]
     return out
+
+
+def cubic_dweights(t):
+    """Derivatives of the Lagrange cubic weights.  svierosion/kernels.py:140-143."""
+    return (-(3 * t * t - 6 * t + 2) / 6, (3 * t * t - 4 * t - 1) / 2,
+            -(3 * t * t - 2 * t - 2) / 2, (3 * t * t - 1) / 6)
+
+
+def tricubic_with_gradient(flat, shape, p):
+    """Tricubic value and gradient (per cell) at index points p (m, 3).  svierosion/kernels.py:232-253."""
+    nx, ny, nz = shape
+    fl = np.floor(p)
+    t = (p - fl).astype(flat.dtype)
+    base = fl.astype(np.int64) - 1
+    b0 = base[:, 0] * (ny * nz) + base[:, 1] * nz + base[:, 2]
+    w = [cubic_weights(t[:, a]) for a in range(3)]
+    dw = [cubic_dweights(t[:, a]) for a in range(3)]
+    val = np.zeros(len(p), dtype=flat.dtype)
+    g = [np.zeros(len(p), dtype=flat.dtype) for _ in range(3)]
+    for a in range(4):
+        for b in range(4):
+            off = a * ny * nz + b * nz
+            for c in range(4):
+                v = flat[b0 + off + c]
+                val = val + w[0][a] * w[1][b] * w[2][c] * v
+                g[0] = g[0] + dw[0][a] * w[1][b] * w[2][c] * v
+                g[1] = g[1] + w[0][a] * dw[1][b] * w[2][c] * v
+                g[2] = g[2] + w[0][a] * w[1][b] * dw[2][c] * v
+    return val, np.stack(g, 1)
 
 
 def trilinear(flat, shape, p):

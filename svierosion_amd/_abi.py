@@ -80,6 +80,8 @@ _SIGNATURES = {
                             _P, _I64]),
     "svimx_velocity_rk": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _D,
                                _P, _P, _I64]),
+    "svimx_velocity_rk_points": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D,
+                                      _D, _P, _P, _P, _I64]),
     "svimx_velocity_static": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _I, _P,
                                    _P]),
     "svimx_rk_stage": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P]),
@@ -131,6 +133,7 @@ _SIGNATURES = {
     "svimx_op_godunov": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
     "svimx_op_cell_fractions": (_I, [_P, _P, _P, _I64, _I, _I, _I, _P]),
     "svimx_op_closest_points": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P]),
+    "svimx_closest_points_newton": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _I, _P]),
     "svimx_op_tricubic": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P]),
     "svimx_op_tricubic_gradient": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P, _P]),
     "svimx_op_trilinear": (_I, [_P, _P, _I, _I, _I, _P, _I64, _P]),

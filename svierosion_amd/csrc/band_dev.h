@@ -201,11 +201,16 @@ __device__ __forceinline__ void curvatures_body(const float* __restrict__ phi, c
 // update into rk.out.  V G, u0 and phi are the same floats as in the separate
 // passes (rk_node), combined by the same operations: the fields are
 // bit-identical to the unfused kernels.
+//
+// rk.cp, when set, holds each band entry's closest point (3 floats per entry,
+// svimx_closest_points_newton with newton_steps > 1); else the body projects
+// the node itself (one step).
 struct RkFuse {
   const float* u0 = nullptr;
   float* out = nullptr;
   float* L0 = nullptr;
   float inv_h = 0.0f, dt = 0.0f;
+  const float* cp = nullptr;
 };
 
 template <bool SVI, int MODE = 0>
@@ -250,10 +255,14 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
     float v = 0.0f;
     if constexpr (SVI) {
       if (x < cut_hi) {
-        int i, j, k;
-        unflat(d, n, i, j, k);
         float p[3];
-        closest_point(g, i, j, k, h, hi.v, p);
+        if (rk.cp) {
+          p[0] = rk.cp[3 * q]; p[1] = rk.cp[3 * q + 1]; p[2] = rk.cp[3 * q + 2];
+        } else {
+          int i, j, k;
+          unflat(d, n, i, j, k);
+          closest_point(g, i, j, k, h, hi.v, p);
+        }
         float N = tricubic(field, d, p);
         float vraw = svi_speed(L, N);
         v = vraw * cutoff_factor(x, cut_lo, cut_hi);
@@ -269,10 +278,14 @@ __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, con
         }
       }
     } else {
-      int i, j, k;
-      unflat(d, n, i, j, k);
       float p[3];
-      closest_point(g, i, j, k, h, hi.v, p);
+      if (rk.cp) {
+        p[0] = rk.cp[3 * q]; p[1] = rk.cp[3 * q + 1]; p[2] = rk.cp[3 * q + 2];
+      } else {
+        int i, j, k;
+        unflat(d, n, i, j, k);
+        closest_point(g, i, j, k, h, hi.v, p);
+      }
       const float2 kk = trilinear2(reinterpret_cast<const float2*>(k12), d, p);
       const float k1 = kk.x, k2 = kk.y;
       float vraw, D;

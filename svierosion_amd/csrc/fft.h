@@ -17,8 +17,9 @@ __host__ __device__ constexpr int fft_kb(int R) { return (kFftTileElems / R + kF
 struct FftPlan {
   Dims d{0, 0, 0};
   int len[3] = {0, 0, 0};
-  float2* C = nullptr;       // complex work array, nx*ny*nz
-  float* khat = nullptr;     // real spectrum of the ball kernel, nx*ny*nz
+  int nzh = 0;               // kept z frequencies, nz / 2 + 1
+  float2* C = nullptr;       // complex work array, nx*ny*nzh (the half spectrum)
+  float* khat = nullptr;     // real spectrum of the ball kernel, nx*ny*nzh
   float2* tw[3] = {nullptr, nullptr, nullptr};  // twiddles exp(-2 pi i t / L) per axis
 };
 

@@ -272,6 +272,24 @@ __device__ __forceinline__ float tricubic_grad(const float* __restrict__ f, cons
   return val;
 }
 
+// Further capped Newton steps from a first projection p (svierosion/levelset.py:
+// 139-145): the tricubic value and gradient of f at p, a step val/|g|^2 g of at
+// most 1.25 |val|/h + 0.25 cells, then the clamp to where tricubic stencils fit.
+__device__ __forceinline__ void newton_refine(const float* __restrict__ f, const Dims& d, float h, const float hi[3],
+                                              float p[3], int extra) {
+  for (int s = 0; s < extra; ++s) {
+    float g[3];
+    const float val = tricubic_grad(f, d, p, g);
+    const float g2 = fmaxf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2], 1e-30f);
+    const float av = fabsf(val);
+    const float length = av / sqrtf(g2);
+    const float cap = 1.25f * av / h + 0.25f;
+    const float scale = (val / g2) * fminf(cap / fmaxf(length, 1e-30f), 1.0f);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = fminf(fmaxf(p[a] - scale * g[a], 1.0f), hi[a]);
+  }
+}
+
 // Trilinear interpolation at index point p (svierosion/kernels.py:257-270).
 __device__ __forceinline__ float trilinear(const float* __restrict__ f, const Dims& d, const float p[3]) {
   float fx = floorf(p[0]), fy = floorf(p[1]), fz = floorf(p[2]);
@@ -347,25 +365,27 @@ __device__ __forceinline__ void principal_curvatures(const float* __restrict__ f
 }
 
 // ---------------------------------------------------------------- reinitialisation
-__device__ __forceinline__ float minmod(float a, float b) {
-  return (a * b > 0.0f) ? (fabsf(a) < fabsf(b) ? a : b) : 0.0f;
+template <typename Real>
+__device__ __forceinline__ Real minmod(Real a, Real b) {
+  return (a * b > Real(0)) ? (fabs(a) < fabs(b) ? a : b) : Real(0);
 }
 
 // Quadratic-ENO distance from a node (p0) to the zero towards a neighbour (p1)
-// (Min 2010; svierosion/kernels.py:312-331).
-__device__ __forceinline__ float interface_distance(float p0, float p1, float dd, float h) {
-  float den = p0 - p1;
-  float lin = h * p0 / (den != 0.0f ? den : 1e-30f);
-  float c2 = 0.5f * dd / (h * h);
-  float c1 = (p1 - p0) / h;
-  float c0 = 0.5f * (p0 + p1) - c2 * h * h / 4.0f;
-  float disc = c1 * c1 - 4.0f * c2 * c0;
-  float sqd = sqrtf(fmaxf(disc, 0.0f));
-  float q = -0.5f * (c1 + (c1 >= 0.0f ? sqd : -sqd));
-  float xi = c0 / (q != 0.0f ? q : 1e-30f);
-  float quad = h / 2.0f + xi;
-  bool ok = (disc >= 0.0f) && (fabsf(dd) > 1e-10f * h * h) && (quad >= 0.0f) && (quad <= h);
-  return fminf(fmaxf(ok ? quad : lin, 0.0f), h);
+// (Min 2010; svierosion/kernels.py:312-331), in the arithmetic of Real.
+template <typename Real>
+__device__ __forceinline__ Real interface_distance(Real p0, Real p1, Real dd, Real h) {
+  const Real den = p0 - p1;
+  const Real lin = h * p0 / (den != Real(0) ? den : Real(1e-30));
+  const Real c2 = Real(0.5) * dd / (h * h);
+  const Real c1 = (p1 - p0) / h;
+  const Real c0 = Real(0.5) * (p0 + p1) - c2 * h * h / Real(4);
+  const Real disc = c1 * c1 - Real(4) * c2 * c0;
+  const Real sqd = sqrt(fmax(disc, Real(0)));
+  const Real q = Real(-0.5) * (c1 + (c1 >= Real(0) ? sqd : -sqd));
+  const Real xi = c0 / (q != Real(0) ? q : Real(1e-30));
+  const Real quad = h / Real(2) + xi;
+  const bool ok = (disc >= Real(0)) && (fabs(dd) > Real(1e-10) * h * h) && (quad >= Real(0)) && (quad <= h);
+  return fmin(fmax(ok ? quad : lin, Real(0)), h);
 }
 
 // Reinit geometry of one node: flags bit (2a) = crossing towards +axis a,
@@ -377,36 +397,44 @@ struct ReinitGeo {
   uint32_t flags;
 };
 
-__device__ __forceinline__ ReinitGeo reinit_geometry(const float* __restrict__ f, int64_t n,
-                                                     const Dims& d, float h, float cfl) {
+// The crossings, subcell distances and local dtau are formed in the
+// arithmetic of Real from the float32 field and rounded to float32 once.
+// The reinitialisation uses Real = double: next to the zero set the
+// quadratic root is ill-conditioned (a node within ~1e-5 h of zero), and
+// float32 roots there moved the reinitialised field by up to 3.5e-4 h from
+// the same algorithm in FP64, against 8e-5 h with FP64 roots.
+template <typename Real>
+__device__ __forceinline__ ReinitGeo reinit_geometry(const float* __restrict__ f, int64_t n, const Dims& d,
+                                                     float hf, float cflf) {
   ReinitGeo g;
   const int64_t st[3] = {d.sx(), d.sy(), 1};
-  float c = f[n];
-  float hmin = h;
+  const Real h = (Real)hf, cfl = (Real)cflf;
+  const Real c = (Real)f[n];
+  Real hmin = h;
   uint32_t flags = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    int64_t s = st[a];
-    float m2 = f[n - 2 * s], m1 = f[n - s], p1 = f[n + s], p2 = f[n + 2 * s];
-    float d_m1 = c - 2.0f * m1 + m2;
-    float d_0 = p1 - 2.0f * c + m1;
-    float d_p1 = p2 - 2.0f * p1 + c;
-    bool cp = (c * p1) < 0.0f;
-    bool cm = (c * m1) < 0.0f;
+    const int64_t s = st[a];
+    const Real m2 = (Real)f[n - 2 * s], m1 = (Real)f[n - s], p1 = (Real)f[n + s], p2 = (Real)f[n + 2 * s];
+    const Real d_m1 = c - Real(2) * m1 + m2;
+    const Real d_0 = p1 - Real(2) * c + m1;
+    const Real d_p1 = p2 - Real(2) * p1 + c;
+    const bool cp = (c * p1) < Real(0);
+    const bool cm = (c * m1) < Real(0);
     // a distance is read only where its crossing flag is set (most work
     // nodes have none): the root is solved only there
-    float xp = cp ? interface_distance(c, p1, minmod(d_0, d_p1), h) : h;
-    float xm = cm ? interface_distance(c, m1, minmod(d_0, d_m1), h) : h;
-    if (cp) hmin = fminf(hmin, xp);
-    if (cm) hmin = fminf(hmin, xm);
+    const Real xp = cp ? interface_distance<Real>(c, p1, minmod<Real>(d_0, d_p1), h) : h;
+    const Real xm = cm ? interface_distance<Real>(c, m1, minmod<Real>(d_0, d_m1), h) : h;
+    if (cp) hmin = fmin(hmin, xp);
+    if (cm) hmin = fmin(hmin, xm);
     flags |= (cp ? 1u : 0u) << (2 * a);
     flags |= (cm ? 1u : 0u) << (2 * a + 1);
-    g.x[2 * a] = fmaxf(xp, 1e-6f * h);
-    g.x[2 * a + 1] = fmaxf(xm, 1e-6f * h);
+    g.x[2 * a] = (float)fmax(xp, Real(1e-6) * h);
+    g.x[2 * a + 1] = (float)fmax(xm, Real(1e-6) * h);
   }
-  flags |= (c > 0.0f ? 1u : (c < 0.0f ? 2u : 0u)) << 6;
+  flags |= (c > Real(0) ? 1u : (c < Real(0) ? 2u : 0u)) << 6;
   g.flags = flags;
-  g.dtau = cfl * fmaxf(hmin, 1e-3f * h);
+  g.dtau = (float)(cfl * fmax(hmin, Real(1e-3) * h));
   return g;
 }
 

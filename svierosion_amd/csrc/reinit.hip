@@ -254,7 +254,7 @@ __global__ void k_reinit_geo(const float* __restrict__ phi, const int32_t* __res
   const int64_t n = work_count(ctr, cap);
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  ReinitGeo g = reinit_geometry(phi, idx[i], d, h, cfl);
+  ReinitGeo g = reinit_geometry<double>(phi, idx[i], d, h, cfl);
   if (g.flags & 63u) {
     float* o = geo + 8 * i;
     o[0] = g.dtau;

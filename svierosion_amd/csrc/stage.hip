@@ -68,7 +68,8 @@ __global__ void k_op_fractions(const float* phi, const int32_t* idx, int64_t n, 
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) out[q] = cell_fraction(central(phi, idx[q], d));
 }
-__global__ void k_op_closest(const float* phi, const int32_t* idx, int64_t n, Dims d, float h, Hi3 hi, float* out) {
+__global__ void k_op_closest(const float* phi, const int32_t* idx, int64_t n, Dims d, float h, Hi3 hi, int extra,
+                             float* out) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   int64_t node = idx[q];
@@ -76,6 +77,7 @@ __global__ void k_op_closest(const float* phi, const int32_t* idx, int64_t n, Di
   unflat(d, node, i, j, k);
   float p[3];
   closest_point(central(phi, node, d), i, j, k, h, hi.v, p);
+  newton_refine(phi, d, h, hi.v, p, extra);
   out[3 * q] = p[0];
   out[3 * q + 1] = p[1];
   out[3 * q + 2] = p[2];
@@ -120,7 +122,7 @@ __global__ void k_op_reinit_geo(const float* phi, const int32_t* idx, int64_t n,
                                 float* geo) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
-  ReinitGeo g = reinit_geometry(phi, idx[q], d, h, cfl);
+  ReinitGeo g = reinit_geometry<double>(phi, idx[q], d, h, cfl);
   geo[8 * q] = g.dtau;
   for (int s = 0; s < 6; ++s) geo[8 * q + 1 + s] = g.x[s];
   geo[8 * q + 7] = (float)g.flags;
@@ -321,10 +323,10 @@ int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t 
   SVIMX_TRY_END
 }
 
-int svimx_velocity_rk(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
-                      const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const svimx_law_t* law,
-                      const float* field, const float* k12, double cut_lo, double cut_hi, double dt, float* L0,
-                      void* ws, int64_t ws_bytes) {
+static int velocity_rk(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                       const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const svimx_law_t* law,
+                       const float* field, const float* k12, double cut_lo, double cut_hi, double dt, float* L0,
+                       const float* cp, void* ws, int64_t ws_bytes) {
   SVIMX_TRY_BEGIN
   Dims d{nx, ny, nz};
   Workspace w;
@@ -346,6 +348,7 @@ int svimx_velocity_rk(void* stream, int stage, const float* phi_in, const float*
     rk.L0 = L0;
     rk.inv_h = (float)(1.0 / h);
     rk.dt = (float)dt;
+    rk.cp = cp;
     const LawF lf = law_to_f(*law);
     const float clo = (float)(cut_lo * h), chi = (float)(cut_hi * h);
 #define SVIMX_VRK(SVI_, MODE_)                                                                                   \
@@ -363,6 +366,23 @@ int svimx_velocity_rk(void* stream, int stage, const float* phi_in, const float*
   const ReduceSpec spec = velocity_stats_spec();
   return launch_finalize(st, w.partials, nblk, w.results, spec);
   SVIMX_TRY_END
+}
+
+int svimx_velocity_rk(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                      const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const svimx_law_t* law,
+                      const float* field, const float* k12, double cut_lo, double cut_hi, double dt, float* L0,
+                      void* ws, int64_t ws_bytes) {
+  return velocity_rk(stream, stage, phi_in, u0, phi_out, band, nb, nx, ny, nz, h, law, field, k12, cut_lo, cut_hi,
+                     dt, L0, nullptr, ws, ws_bytes);
+}
+
+int svimx_velocity_rk_points(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                             const int32_t* band, int64_t nb, int nx, int ny, int nz, double h,
+                             const svimx_law_t* law, const float* field, const float* k12, double cut_lo,
+                             double cut_hi, double dt, float* L0, const float* points, void* ws, int64_t ws_bytes) {
+  SVIMX_REQUIRE(points || nb <= 0, SVIMX_EBADARG, "svimx_velocity_rk_points: null points");
+  return velocity_rk(stream, stage, phi_in, u0, phi_out, band, nb, nx, ny, nz, h, law, field, k12, cut_lo, cut_hi,
+                     dt, L0, points, ws, ws_bytes);
 }
 
 int svimx_velocity_static(void* stream, const float* phi, const int32_t* band, int64_t nb, int nx, int ny, int nz,
@@ -416,7 +436,17 @@ int svimx_op_closest_points(void* stream, const float* phi, const int32_t* idx, 
                             double h, float* out_p) {
   if (n <= 0) return SVIMX_OK;
   Dims d{nx, ny, nz};
-  hipLaunchKernelGGL(k_op_closest, GRID1(n), phi, idx, n, d, (float)h, hi_of(d), out_p);
+  hipLaunchKernelGGL(k_op_closest, GRID1(n), phi, idx, n, d, (float)h, hi_of(d), 0, out_p);
+  SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+}
+int svimx_closest_points_newton(void* stream, const float* phi, const int32_t* idx, int64_t n, int nx, int ny,
+                                int nz, double h, int newton_steps, float* out_p) {
+  SVIMX_REQUIRE(newton_steps >= 1 && newton_steps <= 64, SVIMX_EBADARG, "newton_steps must be in [1, 64]");
+  SVIMX_REQUIRE(phi && (idx || n <= 0) && (out_p || n <= 0), SVIMX_EBADARG, "null argument");
+  if (n <= 0) return SVIMX_OK;
+  Dims d{nx, ny, nz};
+  hipLaunchKernelGGL(k_op_closest, GRID1(n), phi, idx, n, d, (float)h, hi_of(d), newton_steps - 1, out_p);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
 }

@@ -113,6 +113,9 @@ class BatchEngine:
     # ----------------------------------------------------------------- jobs
     def add(self, sim: ErosionSimulation) -> int:
         import torch
+        if sim.newton_steps != 1:
+            raise ValueError("the batch engine projects each node once (newton_steps = 1); "
+                             "run this simulation with ErosionSimulation.run")
         torch.cuda.current_stream().synchronize()  # the job's buffers are ready
         desc = job_desc(sim)
         jid = C.c_int()

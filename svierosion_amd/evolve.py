@@ -77,9 +77,10 @@ class ErosionSimulation:
         self.cutoff = tuple(float(c) for c in cutoff)
         if self.cutoff[1] > levelset.width:
             raise ValueError("cutoff must lie inside the band")
-        if int(newton_steps) != 1:
-            raise NotImplementedError("the fused velocity kernel uses one projection step")
-        self.newton_steps = 1
+        if int(newton_steps) < 1:
+            raise ValueError("newton_steps must be at least 1")
+        self.newton_steps = int(newton_steps)
+        self.cp = None  # newton_steps > 1: the band's closest points, 3 floats per entry
         self._law_abi = law.to_abi()
         dev = levelset.device
         n = levelset.grid.size
@@ -149,6 +150,8 @@ class ErosionSimulation:
         dev = self.ls.device
         self.V = torch.empty(cap, dtype=torch.float32, device=dev)
         self.L0 = torch.empty(cap, dtype=torch.float32, device=dev)
+        if self.newton_steps > 1:
+            self.cp = torch.empty(3 * cap, dtype=torch.float32, device=dev)
 
     # ---------------------------------------------------------- stages
     def _plan(self):
@@ -168,6 +171,15 @@ class ErosionSimulation:
                 with self._span("fractions"):
                     ls.fractions(self._plan(), self.cutoff[1])
                 self._frac_valid = buf
+            if self.newton_steps > 1:
+                # the refined closest points of every band node (svierosion/levelset.py:
+                # 139-145); their tricubic stencils join the N_r marks of the fractions pass
+                _abi.check(_abi.lib.svimx_closest_points_newton(st, _abi.ptr(ls.flat), _abi.ptr(ls._band), ls.nb,
+                                                                *ls.dims, ls.h, self.newton_steps,
+                                                                _abi.ptr(self.cp)), "closest points")
+                if self.conv is not None:
+                    _abi.check(_abi.lib.svimx_nr_mark_points(st, self.conv.plan, _abi.ptr(self.cp), ls.nb),
+                               "N_r marks")
             if self.conv is not None:
                 with self._span("nr_select"):
                     _abi.check(_abi.lib.svimx_nr_select(st, self.conv.plan), "N_r select")
@@ -191,12 +203,15 @@ class ErosionSimulation:
         """Fused velocity + RK stage on bufs[src] (inputs already prepared): stage 0
         stores L0 and the statistics, stages 1 and 2 write bufs[dst]."""
         ls = self.ls
+        args = (_abi.stream(), stage, _abi.ptr(ls.bufs[src]), _abi.ptr(ls.bufs[u0]), _abi.ptr(ls.bufs[dst]),
+                _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h, self._law_abi, _abi.ptr(self.field), _abi.ptr(self.k12),
+                self.cutoff[0], self.cutoff[1], float(dt), _abi.ptr(self.L0))
         with self._span("velocity_rk"):
-            _abi.check(_abi.lib.svimx_velocity_rk(_abi.stream(), stage, _abi.ptr(ls.bufs[src]), _abi.ptr(ls.bufs[u0]),
-                                                  _abi.ptr(ls.bufs[dst]), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
-                                                  self._law_abi, _abi.ptr(self.field), _abi.ptr(self.k12),
-                                                  self.cutoff[0], self.cutoff[1], float(dt), _abi.ptr(self.L0),
-                                                  _abi.ptr(ls.ws), ls.ws.numel()), "velocity + rk stage")
+            if self.newton_steps > 1:
+                rc = _abi.lib.svimx_velocity_rk_points(*args, _abi.ptr(self.cp), _abi.ptr(ls.ws), ls.ws.numel())
+            else:
+                rc = _abi.lib.svimx_velocity_rk(*args, _abi.ptr(ls.ws), ls.ws.numel())
+            _abi.check(rc, "velocity + rk stage")
         if stage > 0 and self._frac_valid == dst:
             self._frac_valid = None
 
@@ -308,6 +323,8 @@ class ErosionSimulation:
             tol: float = 1e-6, callback=None) -> RunResult:
         """Evolve until the volume equals ``target_fraction`` of the initial volume (or
         ``max_time`` / ``max_steps``); re-entrant with decreasing targets."""
+        if self.newton_steps > 1 and not self.fused:
+            raise ValueError("newton_steps > 1 runs on the fused velocity pass (fused = True)")
         ls = self.ls
         V0 = self.initial_volume
         target = None if target_fraction is None else target_fraction * V0

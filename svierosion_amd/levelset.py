@@ -160,13 +160,14 @@ class LevelSet:
 
     # ----------------------------------------------------- projection
     def closest_points(self, idx: torch.Tensor, newton_steps: int = 1) -> torch.Tensor:
-        """First capped Newton projection onto the zero set, index coordinates (m, 3)."""
-        if newton_steps != 1:
-            raise NotImplementedError("only the single projection step of the erosion path is implemented")
+        """Capped Newton projection onto the zero set, index coordinates (m, 3): the
+        first step with the central gradient at the node, each further step with the
+        tricubic value and gradient at the current point (svierosion/levelset.py:129-151)."""
         idx = _idx32(idx, self.device)
         out = torch.empty((idx.numel(), 3), dtype=torch.float32, device=self.device)
-        _abi.check(_abi.lib.svimx_op_closest_points(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx), idx.numel(),
-                                                    *self.dims, self.h, _abi.ptr(out)))
+        _abi.check(_abi.lib.svimx_closest_points_newton(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(idx),
+                                                        idx.numel(), *self.dims, self.h, int(newton_steps),
+                                                        _abi.ptr(out)), "closest points")
         return out
 
     def clamp_to_interior(self, p: torch.Tensor) -> torch.Tensor:

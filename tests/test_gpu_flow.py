@@ -206,3 +206,60 @@ def test_fused_velocity_rk_is_bit_identical(device, law_name):
                                                                        for r in b.history]
     assert torch.equal(ba, bb) and torch.equal(fa, fb)
     print(f"{law_name}: {len(a.history)} steps identical fused and unfused", flush=True)
+
+
+def test_closest_points_newton_steps_match_oracle(device):
+    """LevelSet.closest_points with extra Newton steps (svierosion/levelset.py:129-151)
+    on a field that is not a distance function (the distorted sphere): the device
+    points equal the oracle's to 1e-4 cells for 1, 2 and 3 steps, and the extra steps
+    bring the tricubic value at the points closer to zero."""
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.levelset import LevelSet, tricubic
+    from oracle import stencils as S
+    phi, origin, h, shape = sphere_phi(1.0, 0.05, 0.6, distort=True)
+    phi32 = phi.astype(np.float32)
+    ls = LevelSet(torch.from_numpy(phi32).cuda(), Grid(origin, h, shape), width=6.0)
+    ref = F.LevelSet(phi32, shape, h, width=6.0)  # the same band and clamped field
+    idx = ls.band
+    idx_np = idx.cpu().numpy()
+    res = []
+    for k in (1, 2, 3):
+        p = ls.closest_points(idx, newton_steps=k)
+        want = S.closest_points(ref.flat, idx_np, shape, h, newton_steps=k)
+        err = float(np.abs(p.cpu().numpy() - want).max())
+        val = float(tricubic(ls.flat, shape, p).abs().max()) / h
+        res.append(val)
+        print(f"newton_steps={k}: points {err:.2e} cells from the oracle, max |phi(p)| {val:.2e} h", flush=True)
+        assert err < 1e-4
+    assert res[1] < 0.5 * res[0] and res[2] < 0.2 * res[1]  # oracle: 4.55 h, 0.85 h, 0.083 h
+
+
+@pytest.mark.parametrize("law", ["svi", "mean"])
+def test_newton_steps_history_matches_oracle(device, law):
+    """ErosionSimulation(newton_steps=2) (svierosion/evolve.py:118): closest points
+    refined by a tricubic Newton step, their N_r stencils marked for the direct sum,
+    the velocity read at them; per-step fraction within 1e-4 and dt within 1e-3 of
+    the oracle's run with the same newton_steps, same reinitialisations and stop."""
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.flows import MeanCurvatureLaw, SVIPowerLaw
+    h = 0.05 if law == "svi" else 0.1
+    r = 0.3
+    ls, g, phi = _gpu_sphere(1.0, h, (r + 10 * h) if law == "svi" else 12 * h)
+    if law == "svi":
+        sim = ErosionSimulation(ls, SVIPowerLaw(r, 1, 1, 0.93), newton_steps=2)
+        ref_law, target = F.SVILaw(r, 1, 1, 0.93), 0.85
+    else:
+        sim = ErosionSimulation(ls, MeanCurvatureLaw(1.0, 0.0, 0.0), newton_steps=2)
+        ref_law, target = F.CurvatureLaw("mean", 1.0, delta=0.0), 0.7
+    res = sim.run(target_fraction=target, max_steps=2000)
+    ref = F.Simulation(F.LevelSet(phi.astype(np.float32), g.shape, h), ref_law, newton_steps=2)
+    reason, steps, _, frac, hist = ref.run(target_fraction=target, max_steps=2000)
+    assert res.stop_reason == reason == "target_volume"
+    assert len(res.history) == len(hist)
+    err = max(abs(a.fraction - b.fraction) for a, b in zip(res.history, hist))
+    derr = max(abs(a.dt - b.dt) / b.dt for a, b in zip(res.history, hist))
+    print(f"{law} newton_steps=2: {steps} steps, max fraction difference {err:.2e}, dt {derr:.2e} relative",
+          flush=True)
+    for a, b in zip(res.history, hist):
+        assert a.reinit == b.reinit
+    assert err < 1e-4 and derr < 1e-3

@@ -103,17 +103,19 @@ def test_reinit_geometry_matches(device):
     _abi.check(_abi.lib.svimx_op_reinit_geometry(_abi.stream(), _abi.ptr(ls.flat), _abi.ptr(it), idx.size,
                                                  *g.shape, g.spacing, 0.45, _abi.ptr(geo)))
     geo = geo.cpu().numpy().reshape(-1, 8)
-    want = S.reinit_geometry(ref.flat, idx, ref.strides, g.spacing, 0.45)
-    assert np.abs(geo[:, 0] - want[1]).max() < 1e-5 * g.spacing
+    # the library forms the geometry in FP64 from the float32 field (rounded to
+    # float32 once): the oracle's geometry of the same values in float64
+    want = S.reinit_geometry(ref.flat.astype(np.float64), idx, ref.strides, g.spacing, 0.45)
+    assert np.abs(geo[:, 0] - want[1]).max() < 1e-6 * g.spacing
     flags = geo[:, 7].astype(np.int64)
     for a in range(3):
         cp, cm, xp, xm = want[2 + 4 * a: 6 + 4 * a]
         assert np.array_equal(((flags >> (2 * a)) & 1).astype(bool), cp)
         assert np.array_equal(((flags >> (2 * a + 1)) & 1).astype(bool), cm)
         sel = cp
-        assert np.abs(geo[sel, 1 + 2 * a] - xp[sel]).max(initial=0) < 2e-5 * g.spacing
+        assert np.abs(geo[sel, 1 + 2 * a] - xp[sel]).max(initial=0) < 1e-6 * g.spacing
         sel = cm
-        assert np.abs(geo[sel, 2 + 2 * a] - xm[sel]).max(initial=0) < 2e-5 * g.spacing
+        assert np.abs(geo[sel, 2 + 2 * a] - xm[sel]).max(initial=0) < 1e-6 * g.spacing
 
 
 def test_fractions_volume_and_interface(device):
@@ -208,13 +210,40 @@ def test_nr_sphere_closed_form(device, ratio, tol):
 
 
 def test_reinitialize_matches_oracle(device):
+    """The HIP reinitialisation against the FP32 oracle with the same precision form
+    (geometry in FP64), within 1e-4 h near the zero set."""
     ls, ref, g = _pair(distort=True, R=1.0, h=0.08, pad=12 * 0.08)
     ls.reinitialize(40)
-    ref.reinitialize(40)
+    ref.reinitialize(40, geometry64=True)
     assert ls.nb == ref.band.size or abs(ls.nb - ref.band.size) <= 2
     got = ls.flat.cpu().numpy()
     near = np.abs(ref.flat) < 3 * g.spacing
-    assert np.abs(got - ref.flat)[near].max() < 1e-4 * g.spacing
+    err = np.abs(got - ref.flat)[near].max() / g.spacing
+    print(f"reinit vs FP32 oracle: {err:.2e} h", flush=True)
+    assert err < 1e-4
+
+
+@pytest.mark.parametrize("h", [0.05, 0.025])
+def test_reinitialize_against_fp64(device, h):
+    """The precision form of the reinitialisation: the HIP path (float32 sweeps, the
+    subcell geometry next to the zero set formed in FP64) against the same algorithm
+    entirely in FP64 (the oracle on the float64 field), 20 iterations on the
+    distorted sphere: 99.9% of the nodes within 1.5 h of the zero set within 1e-5 h,
+    every one within 2e-4 h.  The float32-geometry form of the reference reaches
+    3.5e-4 h at h = 0.05 (tests/test_oracle_flow.py)."""
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.levelset import LevelSet
+    phi64, origin, h, shape = sphere_phi(1.0, h, 0.6 if h == 0.05 else 0.4, distort=True, dtype=np.float64)
+    a = F.LevelSet(phi64.copy(), shape, h, width=6.0)
+    a.reinitialize(iterations=20, cfl=0.5, dilation=3)
+    ls = LevelSet(torch.from_numpy(phi64.astype(np.float32)).cuda(), Grid(origin, h, shape), width=6.0)
+    ls.reinitialize(20, cfl=0.5, dilation=3)
+    near = np.abs(a.flat) < 1.5 * h
+    d = np.abs(ls.flat.cpu().numpy().astype(np.float64) - a.flat)[near] / h
+    print(f"HIP reinit vs FP64 (h = {h}): 99.9th pct {np.percentile(d, 99.9):.2e} h, max {d.max():.2e} h "
+          f"over {d.size} nodes", flush=True)
+    assert np.percentile(d, 99.9) < 1e-5
+    assert d.max() < 2e-4
 
 
 def test_reinitialize_preserves_interface(device):

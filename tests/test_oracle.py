@@ -147,6 +147,51 @@ def test_tricubic_exact_for_cubics():
     assert np.abs(g - lin(p[:, 0], p[:, 1], p[:, 2])).max() < 1e-9
 
 
+def test_tricubic_gradient_exact_for_cubics():
+    """The tricubic interpolant reproduces polynomials of degree <= 3 per axis: its
+    value and its gradient (per cell) equal the analytic ones."""
+    shape = (12, 13, 14)
+    ax = [np.arange(n, dtype=np.float64) for n in shape]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    fn = lambda x, y, z: 0.3 * x ** 3 - 0.1 * x * y * z + 0.05 * y ** 2 * z + z ** 3 / 7 - 2 * y + 1
+    grad = lambda x, y, z: np.stack([0.9 * x ** 2 - 0.1 * y * z, -0.1 * x * z + 0.1 * y * z - 2,
+                                     -0.1 * x * y + 0.05 * y ** 2 + 3 * z ** 2 / 7], 1)
+    rng = np.random.default_rng(4)
+    p = rng.uniform(1, 8.9, size=(500, 3))
+    val, g = S.tricubic_with_gradient(fn(X, Y, Z).reshape(-1), shape, p)
+    assert np.abs(val - fn(*p.T)).max() < 1e-9
+    assert np.abs(g - grad(*p.T)).max() < 1e-9
+
+
+def test_newton_steps_converge_onto_a_quadric_zero_set():
+    """Extra capped Newton steps (svierosion/levelset.py:139-145) on a field that is
+    not a distance function, phi = (|x - c|^2 - R^2) / (2R) (its zero set is the
+    sphere |x - c| = R): the tricubic interpolant of this quadratic is exact, so each
+    step is a Newton step on the exact field and the distance of the points to the
+    sphere falls quadratically; on a plane one step is already exact and further
+    steps stay there."""
+    shape = (40, 40, 40)
+    h = 0.05
+    c = np.array([19.3, 20.1, 19.7])
+    R = 11.0
+    ax = [np.arange(n, dtype=np.float64) for n in shape]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    phi = (h * ((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 - R * R) / (2 * R)).reshape(-1)
+    idx = np.nonzero(np.abs(phi) < 3 * h)[0]
+    errs = []
+    for k in (1, 2, 3, 4):
+        p = S.closest_points(phi, idx, shape, h, newton_steps=k)
+        errs.append(np.abs(np.linalg.norm(p - c, axis=1) - R).max())
+    assert errs[0] > 1e-3 and errs[1] < 0.1 * errs[0] and errs[2] < 0.1 * errs[1] and errs[3] < 1e-9, errs
+    nrm = np.array([0.2, 0.4, -0.89])
+    nrm /= np.linalg.norm(nrm)
+    plane = (h * (nrm[0] * X + nrm[1] * Y + nrm[2] * Z - 13.3)).reshape(-1)
+    idx = np.nonzero(np.abs(plane) < 3 * h)[0]
+    idx = idx[(S.unravel(idx, shape) > 5).all(1) & (S.unravel(idx, shape) < 34).all(1)]
+    p = S.closest_points(plane, idx, shape, h, newton_steps=3)
+    assert np.abs(p @ nrm - 13.3).max() < 1e-9
+
+
 def test_principal_curvatures_of_sphere():
     phi, origin, h, shape = sphere_phi(1.0, 0.04, 0.4)
     flat = phi.reshape(-1)

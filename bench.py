@@ -154,9 +154,9 @@ def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, sla
     gen_s = time.time() - t0
     rb = serve(reqs, workers=workers, schedule=schedule, budget_nodes=budget_nodes, slack=slack, verbose=progress)
     print(f"[bench] batched {n_jobs} jobs: {rb.summary()}", file=sys.stderr, flush=True)
+    write_records(records, "batched", reqs, rb)
     rs = serve_sequential(reqs, verbose=progress)
     print(f"[bench] sequential {n_jobs} jobs: {rs.summary()}", file=sys.stderr, flush=True)
-    write_records(records, "batched", reqs, rb)
     write_records(records, "batched_sequential", reqs, rs)
     same = all(cb.error is None and cs.error is None and _same_history(cb.result.history, cs.result.history)
                for cb, cs in zip(rb.completions, rs.completions))
@@ -177,16 +177,18 @@ def batched_leg(n_jobs: int, workers: int, schedule: str, budget_nodes: int, sla
 
 
 def served_leg(n_req: int, rate: float, workers: int, schedule: str, budget_nodes: int, slack: float,
-               progress: bool = False, records: str | None = None):
-    """Poisson arrivals of the served workload (first n_req requests), continuous batching vs FIFO."""
+               progress: bool = False, records: str | None = None, replay_fifo: bool = False):
+    """Poisson arrivals of the served workload (first n_req requests), continuous batching vs FIFO
+    (live, or with replay_fifo the requests back to back and the FIFO queue replayed from their
+    measured service times)."""
     from svierosion_amd.batch import serve, serve_sequential
     from svierosion_amd.workloads import poisson_requests
     reqs = poisson_requests(n_req, rate=rate)
     rv = serve(reqs, workers=workers, schedule=schedule, budget_nodes=budget_nodes, slack=slack, verbose=progress)
     print(f"[bench] served {n_req} requests at {rate:.4f}/s: {rv.summary()}", file=sys.stderr, flush=True)
-    rq = serve_sequential(reqs, verbose=progress)
-    print(f"[bench] sequential FIFO: {rq.summary()}", file=sys.stderr, flush=True)
     write_records(records, "served", reqs, rv)
+    rq = serve_sequential(reqs, verbose=progress, wait_for_arrivals=not replay_fifo)
+    print(f"[bench] sequential FIFO: {rq.summary()}", file=sys.stderr, flush=True)
     write_records(records, "served_sequential", reqs, rq)
     same = all(a.error is None and b.error is None and _same_history(a.result.history, b.result.history)
                for a, b in zip(rv.completions, rq.completions))
@@ -200,7 +202,72 @@ def served_leg(n_req: int, rate: float, workers: int, schedule: str, budget_node
             "sequential_latency_p50_s": round(rq.latency(0.5), 3),
             "sequential_latency_p99_s": round(rq.latency(0.99), 3),
             "speedup": round(rv.erosions_per_hour() / max(rq.erosions_per_hour(), 1e-9), 3),
-            "identical_histories": bool(same), "peak_device_gb": round(rv.peak_device_bytes / 1e9, 3)}
+            "identical_histories": bool(same), "peak_device_gb": round(rv.peak_device_bytes / 1e9, 3),
+            "fifo": "replayed from back-to-back service times" if replay_fifo else "live"}
+
+
+def served_part(args) -> int:
+    """One part of a long served measurement, for request counts whose two legs do not
+    fit one run: ``--serve-part served`` runs the continuous-batching leg over the first
+    --serve-requests requests; ``--serve-part fifo:A:B`` erodes requests A..B-1 back to
+    back (their FIFO service times).  Both append per-request records to --records;
+    --summarize-records combines the parts (the FIFO queue replayed in arrival order
+    from the service times)."""
+    from svierosion_amd.batch import ServeReport, serve, serve_sequential
+    from svierosion_amd.workloads import poisson_requests
+    from svierosion_amd.batch import DEFAULT_BUDGET_NODES, DEFAULT_SLACK
+    if not args.records or not args.serve_rate:
+        raise SystemExit("--serve-part needs --records and --serve-rate")
+    reqs = poisson_requests(args.serve_requests, rate=args.serve_rate)
+    if args.serve_part == "served":
+        budget = DEFAULT_BUDGET_NODES if args.budget_nodes is None else args.budget_nodes
+        slack = DEFAULT_SLACK if args.slack is None else args.slack
+        rv = serve(reqs, workers=args.workers, schedule=args.serve_schedule, budget_nodes=budget, slack=slack,
+                   verbose=args.progress)
+        write_records(args.records, "served", reqs, rv)
+        print(json.dumps({"serve_part": "served", "rate_per_s": args.serve_rate, **rv.summary()}), flush=True)
+        return 0
+    _, a, b = args.serve_part.split(":")
+    part = [reqs[i] for i in range(int(a), min(int(b), len(reqs)))]
+    rq = serve_sequential(part, verbose=args.progress, wait_for_arrivals=False)
+    for c in rq.completions:  # back to the workload's request indices
+        c.index += int(a)
+    write_records(args.records, "served_sequential", reqs, rq)
+    print(json.dumps({"serve_part": args.serve_part, "requests": len(part),
+                      "service_s": round(sum(c.finished - c.admitted for c in rq.completions), 3)}), flush=True)
+    return 0
+
+
+def summarize_records(paths) -> int:
+    """Served vs FIFO from per-request records (any number of files): the served
+    leg's latencies as measured; the FIFO leg replayed in arrival order from each
+    request's measured service time (start = max(arrival, previous finish)); the
+    histories compared by final volume and step count.  One JSON line."""
+    rows = [json.loads(l) for p in paths for l in open(p) if l.strip()]
+    sv = {r["index"]: r for r in rows if r["leg"] == "served"}
+    sq = {r["index"]: r for r in rows if r["leg"] == "served_sequential"}
+    common = sorted(set(sv) & set(sq))
+    clock, fifo = 0.0, {}
+    for i in sorted(common, key=lambda i: sq[i]["arrival"]):
+        start = max(sq[i]["arrival"], clock)
+        clock = start + (sq[i]["finished"] - sq[i]["admitted"])
+        fifo[i] = clock - sq[i]["arrival"]
+    lat_s = np.array([sv[i]["latency"] for i in common])
+    lat_f = np.array([fifo[i] for i in common])
+    mk_s = max(sv[i]["finished"] for i in common)
+    laws = {}
+    for i in common:
+        laws[sv[i]["law"]] = laws.get(sv[i]["law"], 0) + 1
+    print(json.dumps({"requests": len(common), "laws": laws,
+                      "steps_total": int(sum(sv[i]["steps"] for i in common)),
+                      "served": {"makespan_s": round(mk_s, 2), "erosions_per_hour": round(3600 * len(common) / mk_s, 1),
+                                 "latency_p50_s": round(float(np.quantile(lat_s, 0.5)), 3),
+                                 "latency_p99_s": round(float(np.quantile(lat_s, 0.99)), 3)},
+                      "fifo": {"makespan_s": round(clock, 2), "erosions_per_hour": round(3600 * len(common) / clock, 1),
+                               "latency_p50_s": round(float(np.quantile(lat_f, 0.5)), 3),
+                               "latency_p99_s": round(float(np.quantile(lat_f, 0.99)), 3)},
+                      "same_steps": all(sv[i]["steps"] == sq[i]["steps"] for i in common)}), flush=True)
+    return 0
 
 
 def full_erosion(args) -> int:
@@ -345,6 +412,12 @@ def main(argv=None):
                     help="append one JSON line per job of the batched and served legs to PATH")
     ap.add_argument("--batch-schedule", choices=["lockstep", "longest", "fifo"], default="lockstep")
     ap.add_argument("--serve-schedule", choices=["fifo", "longest", "lockstep"], default="fifo")
+    ap.add_argument("--serve-part", default=None, metavar="served|fifo:A:B",
+                    help="run one part of a long served measurement and exit (records to --records)")
+    ap.add_argument("--summarize-records", nargs="+", default=None, metavar="PATH",
+                    help="combine per-request records of the served parts into one comparison and exit")
+    ap.add_argument("--replay-fifo", action="store_true",
+                    help="served leg: run the FIFO baseline back to back and replay the queue from its service times")
     ap.add_argument("--budget-nodes", type=int, default=None, help="band nodes a tick may hold (schedule budget)")
     ap.add_argument("--slack", type=float, default=None, help="tick budget over the first job's band, as a fraction")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -361,8 +434,12 @@ def main(argv=None):
         print("bench.py measures one GPU: this workload is a single erosion job on one MI355X "
               "(multi-GPU sharding is by job and is not run here); use --gpus 1", file=sys.stderr)
         return 2
+    if args.summarize_records:
+        return summarize_records(args.summarize_records)
     import torch
     torch.cuda.set_device(0)
+    if args.serve_part:
+        return served_part(args)
     if args.full_erosion:
         return full_erosion(args)
     if args.copy_calibration:
@@ -468,7 +545,7 @@ def main(argv=None):
     if args.serve_requests > 0:
         rate = args.serve_rate or args.serve_load * (seq_rate if seq_rate else 0.5)
         served = served_leg(args.serve_requests, rate, args.workers, args.serve_schedule, budget, slack,
-                            args.progress, args.records)
+                            args.progress, args.records, args.replay_fifo)
     line = {"metric": "RK3 steps/s + HBM GB/s @res400 1.3M-tri; batched erosions/hour, p99 latency",
             "value": round(value, 3), "unit": "RK3 steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
             "ms_per_step": round(1000 * elapsed / max(steps, 1), 3), "higher_is_better": True, "scaling": "weak",

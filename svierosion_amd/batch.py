@@ -322,29 +322,43 @@ def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, g
     return ServeReport([comps[i] for i in sorted(comps)], makespan, int(peak), ticks, timers)
 
 
-def serve_sequential(requests: list, keep_state: bool = False, verbose: bool = False) -> ServeReport:
-    """The same requests served one at a time with ``erode`` (first come, first served)."""
+def serve_sequential(requests: list, keep_state: bool = False, verbose: bool = False,
+                     wait_for_arrivals: bool = True) -> ServeReport:
+    """The same requests served one at a time with ``erode`` (first come, first served).
+
+    With ``wait_for_arrivals=False`` the requests run back to back and the FIFO
+    schedule is replayed from the measured service times: request i starts at
+    max(arrival_i, finish_{i-1}) and finishes a service time later -- the same
+    single-server queue, without the idle waits for arrivals."""
     order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
     t0 = time.perf_counter()
     comps = []
+    clock = 0.0  # replayed FIFO clock
     for idx in order:
         r = requests[idx]
-        wait = r.arrival - (time.perf_counter() - t0)
-        if wait > 0:
-            time.sleep(wait)
-        start = time.perf_counter() - t0
-        c = Completion(idx, r.arrival, start, math.nan)
+        if wait_for_arrivals:
+            wait = r.arrival - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+        s0 = time.perf_counter()
+        c = Completion(idx, r.arrival, s0 - t0, math.nan)
         try:
             c.result = erode(r.mesh, r.params, keep_state=keep_state)
             c.steps = len(c.result.history)
         except Exception as ex:
             c.error = f"{type(ex).__name__}: {ex}"
-        c.finished = time.perf_counter() - t0
+        s1 = time.perf_counter()
+        if wait_for_arrivals:
+            c.finished = s1 - t0
+        else:
+            c.admitted = max(r.arrival, clock)
+            c.finished = clock = c.admitted + (s1 - s0)
         comps.append(c)
         if verbose:
-            print(f"[sequential] request {idx} done at {c.finished:.2f}s", file=sys.stderr, flush=True)
+            print(f"[sequential] request {idx} done at {c.finished:.2f}s ({s1 - s0:.2f} s of service)",
+                  file=sys.stderr, flush=True)
     comps.sort(key=lambda c: c.index)
-    return ServeReport(comps, time.perf_counter() - t0)
+    return ServeReport(comps, (time.perf_counter() - t0) if wait_for_arrivals else clock)
 
 
 def erode_batch(meshes: list, params, max_jobs: int = 64, graphs: bool = True, keep_state: bool = False,

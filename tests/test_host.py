@@ -327,3 +327,30 @@ def test_sdf_prepare_native_matches_numpy():
     tri = shapes.cube(1.0)
     g = make_grid(tri, ErosionParams(law="mean", resolution=48))
     assert sdf._prepare_native(tri, g, 2.0) is None  # 48-cell edges: split on the NumPy path
+
+
+def test_fifo_replay_follows_the_single_server_queue(monkeypatch):
+    """serve_sequential(wait_for_arrivals=False): requests run back to back and the FIFO
+    queue is replayed from their service times -- start = max(arrival, previous finish),
+    finish = start + service (no GPU: erode is replaced by a timed stand-in)."""
+    import time as _time
+    from types import SimpleNamespace
+    from svierosion_amd import batch
+    service = {0: 0.03, 1: 0.01, 2: 0.02, 3: 0.01}
+
+    def fake_erode(mesh, params, keep_state=False):
+        _time.sleep(service[params])
+        return SimpleNamespace(history=[0] * (params + 1))
+
+    monkeypatch.setattr(batch, "erode", fake_erode)
+    arrivals = [0.0, 0.005, 0.5, 0.51]
+    reqs = [batch.Request(None, i, a) for i, a in enumerate(arrivals)]
+    rep = batch.serve_sequential(reqs, wait_for_arrivals=False)
+    c = rep.completions
+    assert [x.index for x in c] == [0, 1, 2, 3]
+    assert c[0].admitted == 0.0 and c[1].admitted == pytest.approx(c[0].finished)
+    assert c[2].admitted == 0.5 and c[3].admitted == pytest.approx(c[2].finished)
+    for x in c:
+        assert x.finished - x.admitted == pytest.approx(service[x.index], abs=0.02)
+        assert x.steps == x.index + 1
+    assert rep.makespan == pytest.approx(c[3].finished)

@@ -342,8 +342,18 @@ def serve_sequential(requests: list, keep_state: bool = False, verbose: bool = F
                 time.sleep(wait)
         s0 = time.perf_counter()
         c = Completion(idx, r.arrival, s0 - t0, math.nan)
+        progress = None
+        if verbose:  # a line every ~30 s inside long erosions
+            last = [s0]
+
+            def progress(sim, rec, idx=idx, last=last):
+                now = time.perf_counter()
+                if now - last[0] > 30.0:
+                    last[0] = now
+                    print(f"[sequential] request {idx}: step {rec.step}, fraction {rec.fraction:.5f}",
+                          file=sys.stderr, flush=True)
         try:
-            c.result = erode(r.mesh, r.params, keep_state=keep_state)
+            c.result = erode(r.mesh, r.params, keep_state=keep_state, callback=progress)
             c.steps = len(c.result.history)
         except Exception as ex:
             c.error = f"{type(ex).__name__}: {ex}"

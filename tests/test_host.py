@@ -338,7 +338,7 @@ def test_fifo_replay_follows_the_single_server_queue(monkeypatch):
     from svierosion_amd import batch
     service = {0: 0.03, 1: 0.01, 2: 0.02, 3: 0.01}
 
-    def fake_erode(mesh, params, keep_state=False):
+    def fake_erode(mesh, params, keep_state=False, callback=None):
         _time.sleep(service[params])
         return SimpleNamespace(history=[0] * (params + 1))
 

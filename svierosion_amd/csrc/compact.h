@@ -188,4 +188,81 @@ int compact_async(hipStream_t st, Pred pred, const int64_t* n_dev, int64_t nmax,
   return SVIMX_OK;
 }
 
+// ---------------------------------------------------------------- counts -> offsets
+// Exclusive scan of n per-item counts (int32 or int64) into int64 offsets,
+// offs[n] = the total, in the same chunks as the compaction: each block sums
+// its chunk with coalesced loads, one block scans the chunk sums, each block
+// then scans its chunk from its base (wave prefix sums and one LDS exchange
+// per round).  Every pass is parallel over the chunks.
+template <typename T>
+__global__ void __launch_bounds__(kCompactThreads) k_chunk_sum(const T* __restrict__ cnt, int64_t n,
+                                                                int64_t* __restrict__ sums) {
+  __shared__ long long wsum[kCompactThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
+  long long s = 0;
+  for (int r = 0; r < kCompactPerThread; ++r) {
+    const int64_t e = base + (int64_t)r * kCompactThreads + threadIdx.x;
+    if (e < n) s += (long long)cnt[e];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int w = 0; w < kCompactThreads / 64; ++w) t += wsum[w];
+    sums[blockIdx.x] = (int64_t)t;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCompactThreads) k_chunk_scan(const T* __restrict__ cnt, int64_t n,
+                                                                 const int64_t* __restrict__ base_of,
+                                                                 int64_t* __restrict__ offs) {
+  __shared__ long long wtot[kCompactThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t base = (int64_t)blockIdx.x * kCompactChunk;
+  long long run = (long long)base_of[blockIdx.x];
+  for (int r = 0; r < kCompactPerThread; ++r) {
+    const int64_t e = base + (int64_t)r * kCompactThreads + threadIdx.x;
+    const long long v = e < n ? (long long)cnt[e] : 0;
+    long long inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    __syncthreads();  // wtot of the previous round is read
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    long long before = 0, tot = 0;
+    for (int w = 0; w < kCompactThreads / 64; ++w) {
+      before += w < wave ? wtot[w] : 0;
+      tot += wtot[w];
+    }
+    if (e < n) offs[e] = (int64_t)(run + before + inc - v);
+    run += tot;
+  }
+}
+
+// offs[0..n) = exclusive prefix of cnt, offs[n] = total (stream-ordered, no
+// host synchronisation; a temporary of (chunks + 1) int64 on the stream).
+template <typename T>
+int scan_to_offsets(hipStream_t st, const T* cnt, int64_t n, int64_t* offs) {
+  const int64_t chunks = (n + kCompactChunk - 1) / kCompactChunk;
+  int64_t* sums = nullptr;
+  SVIMX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&sums), sizeof(int64_t) * (chunks + 1), st));
+  if (chunks > 0) {
+    hipLaunchKernelGGL(k_chunk_sum<T>, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, cnt, n, sums);
+    SVIMX_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, sums, chunks);
+  SVIMX_CHECK_LAUNCH();
+  if (chunks > 0) {
+    hipLaunchKernelGGL(k_chunk_scan<T>, dim3((unsigned)chunks), dim3(kCompactThreads), 0, st, cnt, n, sums, offs);
+    SVIMX_CHECK_LAUNCH();
+  }
+  SVIMX_CHECK_HIP(hipMemcpyAsync(offs + n, sums + chunks, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+  SVIMX_CHECK_HIP(hipFreeAsync(sums, st));
+  return SVIMX_OK;
+}
+
 }  // namespace svimx

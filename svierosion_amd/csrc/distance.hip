@@ -16,6 +16,7 @@
 #include <cmath>
 
 #include "common.h"
+#include "compact.h"
 
 namespace svimx {
 
@@ -54,30 +55,6 @@ __global__ void k_bucket_count(const float* __restrict__ local, const int32_t* _
       for (int k = lo[2]; k <= hi[2]; ++k) atomicAdd(&counts[B.id(i, j, k)], 1);
 }
 
-// Exclusive scan of n int32 counts into int64 offsets, offs[n] = total (one block).
-__global__ void __launch_bounds__(1024) k_bucket_scan(const int32_t* cnt, int64_t n, int64_t* offs) {
-  __shared__ int64_t sums[1024];
-  const int t = threadIdx.x;
-  const int64_t per = (n + 1023) / 1024;
-  const int64_t lo = t * per < n ? t * per : n;
-  const int64_t hi = lo + per < n ? lo + per : n;
-  int64_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += cnt[i];
-  sums[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int64_t v = t >= o ? sums[t - o] : 0;
-    __syncthreads();
-    sums[t] += v;
-    __syncthreads();
-  }
-  int64_t run = sums[t] - s;
-  for (int64_t i = lo; i < hi; ++i) {
-    offs[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) offs[n] = sums[1023];
-}
 
 __global__ void k_bucket_fill(const float* __restrict__ local, const int32_t* __restrict__ org, int64_t ntri,
                               BucketGrid B, const int64_t* __restrict__ offs, int32_t* cursor, int32_t* items) {
@@ -193,8 +170,10 @@ int svimx_point_mesh_distance(void* stream, const float* tri_local, const int32_
   SVIMX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_bucket_count, gt, tb, 0, st, tri_local, tri_org, ntri, B, counts.p);
   SVIMX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, st, counts.p, nbk, offs.p);
-  SVIMX_CHECK_LAUNCH();
+  {
+    const int rc = scan_to_offsets(st, counts.p, nbk, offs.p);
+    if (rc) return rc;
+  }
   int64_t nitems = 0;
   SVIMX_CHECK_HIP(hipMemcpyAsync(&nitems, offs.p + nbk, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SVIMX_CHECK_HIP(hipStreamSynchronize(st));

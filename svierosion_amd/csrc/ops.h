@@ -73,24 +73,34 @@ constexpr float kThird = 1.0f / 3.0f, kSixth = 1.0f / 6.0f, kFiveSixths = 5.0f /
 constexpr float kSevenSixths = 7.0f / 6.0f, kElevenSixths = 11.0f / 6.0f, kThirteenTwelfths = 13.0f / 12.0f;
 
 // Jiang-Peng WENO5 derivative from five one-sided differences
-// (svierosion/kernels.py:83-97).
-__device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5) {
+// (svierosion/kernels.py:83-97); `big` is the largest of v1^2 .. v5^2 (the
+// caller forms the maxima of the two derivatives of an axis from one shared
+// tree: max is exact in any order).  The normalised weights a_k / sum a with
+// a_k = d_k / q_k, q_k = (s_k + eps)^2, are d_k prod_{j!=k} q_j over their
+// sum: one reciprocal instead of four (v_rcp_f32 issues at a quarter of the
+// rate).  The q_k are scaled by 2^20 first, which leaves the weights as they
+// are: q_k >= 1e-20 (eps >= 1e-10), so the products of two stay normal
+// floats, and |v| <= 12 in a field clamped to +-6h (s_k < 5e3) keeps them far
+// from overflow.
+__device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5, float big) {
   float p1 = v1 * kThird - v2 * kSevenSixths + v3 * kElevenSixths;
   float p2 = -v2 * kSixth + v3 * kFiveSixths + v4 * kThird;
   float p3 = v3 * kThird + v4 * kFiveSixths - v5 * kSixth;
   float s1 = kThirteenTwelfths * sq(v1 - 2.0f * v2 + v3) + 0.25f * sq(v1 - 4.0f * v2 + 3.0f * v3);
   float s2 = kThirteenTwelfths * sq(v2 - 2.0f * v3 + v4) + 0.25f * sq(v2 - v4);
   float s3 = kThirteenTwelfths * sq(v3 - 2.0f * v4 + v5) + 0.25f * sq(3.0f * v3 - 4.0f * v4 + v5);
-  float big = fmaxf(fmaxf(fmaxf(v1 * v1, v2 * v2), fmaxf(v3 * v3, v4 * v4)), v5 * v5);
-  float eps = 1e-6f * big + 1e-10f;  // (s + eps)^2 stays a normal float
-  float a1 = 0.1f * rcp(sq(s1 + eps));
-  float a2 = 0.6f * rcp(sq(s2 + eps));
-  float a3 = 0.3f * rcp(sq(s3 + eps));
+  float eps = 1e-6f * big + 1e-10f;
+  constexpr float kScale = 1048576.0f;  // 2^20
+  const float q1 = sq(s1 + eps) * kScale, q2 = sq(s2 + eps) * kScale, q3 = sq(s3 + eps) * kScale;
+  const float a1 = 0.1f * (q2 * q3), a2 = 0.6f * (q1 * q3), a3 = 0.3f * (q1 * q2);
   return (a1 * p1 + a2 * p2 + a3 * p3) * rcp(a1 + a2 + a3);
 }
 
 // Godunov |grad phi| for phi_t = V |grad phi|, V >= 0, WENO5 one-sided
-// derivatives (svierosion/kernels.py:146-169).
+// derivatives (svierosion/kernels.py:146-169).  The upwind term
+// max(min(D-, 0)^2, max(D+, 0)^2) is max(-D-, D+, 0)^2 (one v_max3_f32:
+// squaring is monotone on non-negative floats); the square root is the
+// hardware v_sqrt_f32 (1 ulp).
 __device__ __forceinline__ float godunov_inward(const float* __restrict__ phi, int64_t n,
                                                 const Dims& d, float inv_h) {
   const float c = phi[n];
@@ -103,11 +113,13 @@ __device__ __forceinline__ float godunov_inward(const float* __restrict__ phi, i
     float p1 = phi[n + s], p2 = phi[n + 2 * s], p3 = phi[n + 3 * s];
     float d0 = (m2 - m3) * inv_h, d1 = (m1 - m2) * inv_h, d2 = (c - m1) * inv_h;
     float d3 = (p1 - c) * inv_h, d4 = (p2 - p1) * inv_h, d5 = (p3 - p2) * inv_h;
-    float dm = weno5(d0, d1, d2, d3, d4);
-    float dp = weno5(d5, d4, d3, d2, d1);
-    g2 = g2 + fmaxf(sq(fminf(dm, 0.0f)), sq(fmaxf(dp, 0.0f)));
+    const float mid = fmaxf(fmaxf(d1 * d1, d2 * d2), fmaxf(d3 * d3, d4 * d4));
+    float dm = weno5(d0, d1, d2, d3, d4, fmaxf(mid, d0 * d0));
+    float dp = weno5(d5, d4, d3, d2, d1, fmaxf(mid, d5 * d5));
+    const float g = fmaxf(fmaxf(-dm, dp), 0.0f);
+    g2 = __fadd_rn(g2, __fmul_rn(g, g));
   }
-  return sqrtf(g2);
+  return __builtin_amdgcn_sqrtf(g2);
 }
 
 // Fraction of the unit cube on the side n.xi < -s of a plane; a >= b >= c >= 0

@@ -1,9 +1,14 @@
 // Signed distance of a triangle mesh on the grid (svierosion/sdf.py:365-415).
 //
-// 1. Exact near field: every (split) triangle visits the nodes of its bounding
-//    box dilated by exact_band cells and atomically min-reduces a packed key
-//    (float bits of the distance << 32 | triangle id) -- order independent, so
-//    the field is deterministic (svierosion/sdf.py:100-145).
+// 1. Exact near field, triangle-bucketed: the grid is cut into 8^3-node
+//    bricks, every (split) triangle is listed in each brick its box (dilated
+//    by exact_band cells) overlaps, and one workgroup per brick takes the
+//    brick's triangles, each thread a triangle visiting the nodes of its box
+//    inside the brick, min-reducing a packed key (float bits of the distance
+//    << 32 | triangle id) in the brick's 512 LDS keys; the brick then writes
+//    its keys with plain stores.  The minimum is order independent, so the
+//    field is deterministic and equals a global atomic reduction over the
+//    same (node, triangle) pairs (svierosion/sdf.py:100-145).
 // 2. Ring propagation: rings of nodes around the known set test the closest
 //    triangles of their 26 neighbours, Jacobi per ring (svierosion/sdf.py:163-202).
 // 3. Vertex-star refinement of nodes with exact_band < d <= band + 1 (:211-265).
@@ -50,18 +55,74 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
-// One thread per triangle, looping over its (clipped, host-computed) box.
-__global__ void k_near_field(TriSoup T, const int32_t* __restrict__ box, int64_t ntri, Dims d, float eb,
-                             unsigned long long* __restrict__ keys) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// ---------------------------------------------------------------- near field
+constexpr int kSdfBrick = 8;  // nodes per brick edge (512 keys of 8 B in LDS)
+
+struct BrickGrid {
+  int bx, by, bz;  // bricks per axis
+  __host__ __device__ int64_t count() const { return (int64_t)bx * by * bz; }
+};
+
+// Bricks overlapped by each triangle's box: counts[brick] += 1.
+__global__ void k_brick_count(const int32_t* __restrict__ box, int64_t ntri, BrickGrid B,
+                              unsigned long long* __restrict__ counts) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntri) return;
   const int32_t* b = box + 6 * t;
-  for (int i = b[0]; i <= b[3]; ++i)
-    for (int j = b[1]; j <= b[4]; ++j)
-      for (int k = b[2]; k <= b[5]; ++k) {
-        float dist = tri_distance(T, t, i, j, k);
-        if (dist <= eb) atomicMin(&keys[((int64_t)i * d.ny + j) * d.nz + k], (unsigned long long)pack_key(dist, t));
-      }
+  for (int x = b[0] / kSdfBrick; x <= b[3] / kSdfBrick; ++x)
+    for (int y = b[1] / kSdfBrick; y <= b[4] / kSdfBrick; ++y)
+      for (int z = b[2] / kSdfBrick; z <= b[5] / kSdfBrick; ++z)
+        atomicAdd(&counts[((int64_t)x * B.by + y) * B.bz + z], 1ull);
+}
+
+// The triangle lists of the bricks, at the scanned offsets (cursor = a copy
+// of the offsets, advanced atomically; the order within a brick is free).
+__global__ void k_brick_fill(const int32_t* __restrict__ box, int64_t ntri, BrickGrid B,
+                             unsigned long long* __restrict__ cursor, int32_t* __restrict__ list) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntri) return;
+  const int32_t* b = box + 6 * t;
+  for (int x = b[0] / kSdfBrick; x <= b[3] / kSdfBrick; ++x)
+    for (int y = b[1] / kSdfBrick; y <= b[4] / kSdfBrick; ++y)
+      for (int z = b[2] / kSdfBrick; z <= b[5] / kSdfBrick; ++z)
+        list[atomicAdd(&cursor[((int64_t)x * B.by + y) * B.bz + z], 1ull)] = (int32_t)t;
+}
+
+// One workgroup per brick: its triangles' (node, triangle) pairs inside the
+// brick, min-reduced in LDS, then one store per node of the brick.
+__global__ void __launch_bounds__(256) k_near_bricks(TriSoup T, const int32_t* __restrict__ box,
+                                                     const int64_t* __restrict__ offs,
+                                                     const int32_t* __restrict__ list, BrickGrid B, Dims d,
+                                                     float eb, unsigned long long* __restrict__ keys) {
+  constexpr int N3 = kSdfBrick * kSdfBrick * kSdfBrick;
+  __shared__ unsigned long long lk[N3];
+  const int64_t brick = blockIdx.x;
+  const int64_t b0 = offs[brick], b1 = offs[brick + 1];
+  if (b0 == b1) return;  // block-uniform: no triangle reaches this brick
+  const int bz = (int)(brick % B.bz), by = (int)((brick / B.bz) % B.by), bx = (int)(brick / ((int64_t)B.bz * B.by));
+  const int x0 = bx * kSdfBrick, y0 = by * kSdfBrick, z0 = bz * kSdfBrick;
+  for (int e = threadIdx.x; e < N3; e += blockDim.x) lk[e] = 0x7FFFFFFFFFFFFFFFull;
+  __syncthreads();
+  for (int64_t q = b0 + threadIdx.x; q < b1; q += blockDim.x) {
+    const int32_t t = list[q];
+    const int32_t* bb = box + 6 * (int64_t)t;
+    const int ilo = max(bb[0], x0), ihi = min(bb[3], x0 + kSdfBrick - 1);
+    const int jlo = max(bb[1], y0), jhi = min(bb[4], y0 + kSdfBrick - 1);
+    const int klo = max(bb[2], z0), khi = min(bb[5], z0 + kSdfBrick - 1);
+    for (int i = ilo; i <= ihi; ++i)
+      for (int j = jlo; j <= jhi; ++j)
+        for (int k = klo; k <= khi; ++k) {
+          const float dist = tri_distance(T, t, i, j, k);
+          if (dist <= eb)
+            atomicMin(&lk[((i - x0) * kSdfBrick + (j - y0)) * kSdfBrick + (k - z0)],
+                      (unsigned long long)pack_key(dist, t));
+        }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < N3; e += blockDim.x) {
+    const int i = x0 + e / (kSdfBrick * kSdfBrick), j = y0 + (e / kSdfBrick) % kSdfBrick, k = z0 + e % kSdfBrick;
+    if (i < d.nx && j < d.ny && k < d.nz) keys[((int64_t)i * d.ny + j) * d.nz + k] = lk[e];
+  }
 }
 
 __global__ void k_unpack(const uint64_t* __restrict__ keys, int64_t n, float* __restrict__ dist,
@@ -326,10 +387,35 @@ int svimx_sdf_build(void* stream, const float* tri_local, const int32_t* tri_org
   SVIMX_CHECK_LAUNCH();
   const float eb = (float)(exact_band * (1.0 + 1e-6));
   if (n_split > 0) {
-    hipLaunchKernelGGL(k_near_field, dim3(blocks_for(n_split, 64)), dim3(64), 0, st, T, tri_box, n_split, d, eb,
+    // triangles binned per brick (count, scan, fill), then one workgroup per brick
+    const BrickGrid B{(nx + kSdfBrick - 1) / kSdfBrick, (ny + kSdfBrick - 1) / kSdfBrick,
+                      (nz + kSdfBrick - 1) / kSdfBrick};
+    const int64_t nbr = B.count();
+    SVIMX_REQUIRE(nbr < (int64_t)INT32_MAX, SVIMX_EBADARG, "too many SDF bricks");
+    DevBuf<int64_t> boffs, cursor;
+    SVIMX_CHECK_HIP(boffs.alloc(nbr + 1, st));
+    SVIMX_CHECK_HIP(cursor.alloc(nbr + 1, st));
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_for(nbr + 1, 256)), tb, 0, st,
+                       reinterpret_cast<uint64_t*>(boffs.p), nbr + 1, (uint64_t)0);
+    SVIMX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_brick_count, dim3(blocks_for(n_split, 256)), tb, 0, st, tri_box, n_split, B,
+                       reinterpret_cast<unsigned long long*>(boffs.p));
+    SVIMX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, boffs.p, nbr);
+    SVIMX_CHECK_LAUNCH();
+    int64_t nlist = 0;
+    SVIMX_CHECK_HIP(hipMemcpyAsync(&nlist, boffs.p + nbr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SVIMX_CHECK_HIP(hipMemcpyAsync(cursor.p, boffs.p, sizeof(int64_t) * (nbr + 1), hipMemcpyDeviceToDevice, st));
+    SVIMX_CHECK_HIP(hipStreamSynchronize(st));
+    DevBuf<int32_t> blist;
+    SVIMX_CHECK_HIP(blist.alloc(std::max<int64_t>(nlist, 1), st));
+    hipLaunchKernelGGL(k_brick_fill, dim3(blocks_for(n_split, 256)), tb, 0, st, tri_box, n_split, B,
+                       reinterpret_cast<unsigned long long*>(cursor.p), blist.p);
+    SVIMX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_near_bricks, dim3((unsigned)nbr), tb, 0, st, T, tri_box, boffs.p, blist.p, B, d, eb,
                        (unsigned long long*)keys.p);
     SVIMX_CHECK_LAUNCH();
-  }
+  }  // the brick buffers are freed stream-ordered at scope exit
   hipLaunchKernelGGL(k_unpack, gn, tb, 0, st, keys.p, n, dist, tid, known.p);
   SVIMX_CHECK_LAUNCH();
   // 2. ring propagation (Jacobi per ring over the new ring plus the previous one)

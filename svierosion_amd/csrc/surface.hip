@@ -22,6 +22,7 @@
 // slot masks by atomicOr), two scans, vertices (+ one capped Newton step on
 // the tricubic interpolant), emit (faces at the scanned offsets).
 #include "common.h"
+#include "compact.h"
 
 namespace svimx {
 
@@ -145,32 +146,6 @@ __global__ void k_mt_count(MtArgs A, int32_t* ntri, uint32_t* masks) {
 __global__ void k_mt_popcount(const uint32_t* masks, int64_t nb, int32_t* cnt) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < nb) cnt[q] = __popc(masks[q]);
-}
-
-// Exclusive scan of n int32 counts into int64 offsets, offs[n] = total.  One
-// block: thread t owns a contiguous run of the counts.
-__global__ void __launch_bounds__(1024) k_mt_scan(const int32_t* cnt, int64_t n, int64_t* offs) {
-  __shared__ int64_t sums[1024];
-  const int t = threadIdx.x;
-  const int64_t per = (n + 1023) / 1024;
-  const int64_t lo = t * per < n ? t * per : n;
-  const int64_t hi = lo + per < n ? lo + per : n;
-  int64_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += cnt[i];
-  sums[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const int64_t v = t >= o ? sums[t - o] : 0;
-    __syncthreads();
-    sums[t] += v;
-    __syncthreads();
-  }
-  int64_t run = sums[t] - s;
-  for (int64_t i = lo; i < hi; ++i) {
-    offs[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) offs[n] = sums[1023];
 }
 
 // Index-space position of every vertex owned by band position q.
@@ -303,10 +278,10 @@ int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, i
   SVIMX_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_mt_popcount, dim3(gb), dim3(256), 0, st, masks.p, nb, vcnt.p);
   SVIMX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mt_scan, dim3(1), dim3(1024), 0, st, ntri.p, nb, toffs.p);
-  SVIMX_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_mt_scan, dim3(1), dim3(1024), 0, st, vcnt.p, nb, voffs.p);
-  SVIMX_CHECK_LAUNCH();
+  int rc = scan_to_offsets(st, ntri.p, nb, toffs.p);
+  if (rc) return rc;
+  rc = scan_to_offsets(st, vcnt.p, nb, voffs.p);
+  if (rc) return rc;
   int64_t nt = 0, nv = 0;
   SVIMX_CHECK_HIP(hipMemcpyAsync(&nt, toffs.p + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SVIMX_CHECK_HIP(hipMemcpyAsync(&nv, voffs.p + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));

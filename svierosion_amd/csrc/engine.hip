@@ -35,7 +35,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "band_dev.h"
@@ -260,7 +262,11 @@ __global__ void __launch_bounds__(kBandThreads) k_eng_velocity_rk(const JobDev* 
     rk.L0 = j.L0;
     rk.inv_h = j.inv_h;
     rk.dt = MODE >= 2 ? (float)dts[jslot] : 0.0f;
-    velocity_body<SVI, MODE>(jbuf(j, rel_in), j.band, j.nb, j.d, j.h, j.law, j.field, j.k12, j.cut_lo, j.cut_hi,
+    // the law by value: velocity_body takes it by reference, and a reference
+    // into the job table had its fields reloaded by scalar loads at every node
+    // of the inner loop (35% slower than the single-job kernel)
+    const LawF law = j.law;
+    velocity_body<SVI, MODE>(jbuf(j, rel_in), j.band, j.nb, j.d, j.h, law, j.field, j.k12, j.cut_lo, j.cut_hi,
                              j.hi, stats, j.V, j.partials, lb, G, rk);
   }
 }
@@ -563,11 +569,36 @@ static unsigned grid_of(const Engine& E, int t) {
   return (unsigned)((g + kXcds - 1) / kXcds * kXcds);
 }
 
+// Blocks of `kern` (with `threads` per block) that the device holds at once,
+// rounded down to a multiple of the XCD count; cached per kernel.  A persistent
+// grid above this leaves its last blocks to start after whole rows of virtual
+// blocks of the first ones, so every job-table kernel runs with at most this
+// many blocks (the virtual blocks, and so the results, do not change): with
+// 6-wave occupancy 1536 blocks are resident, and 2048 blocks of four jobs'
+// virtual blocks took 35% longer than the single-job kernels on the same jobs.
+template <typename K>
+static int64_t resident_blocks(K kern, int threads) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, int64_t> cache;
+  const void* key = reinterpret_cast<const void*>(kern);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int dev = 0, per_cu = 0, cus = 0;
+  int64_t r = kEngGrid;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) == hipSuccess && per_cu > 0 && cus > 0)
+    r = std::max<int64_t>((int64_t)per_cu * cus / kXcds * kXcds, kXcds);
+  cache.emplace(key, r);
+  return r;
+}
+
 #define ENG_LAUNCH(kern, t, threads, ...)                                                                  \
   do {                                                                                                    \
     if (E.h_tab->seg[t].njobs > 0 && E.h_tab->seg[t].total > 0) {                                         \
-      hipLaunchKernelGGL(kern, dim3(grid_of(E, t)), dim3(threads), 0, E.st, E.d_jobs, &E.d_tab->seg[t], \
-                         ##__VA_ARGS__);                                                                  \
+      const unsigned _g = (unsigned)std::min<int64_t>(grid_of(E, t), resident_blocks(kern, threads));     \
+      hipLaunchKernelGGL(kern, dim3(_g), dim3(threads), 0, E.st, E.d_jobs, &E.d_tab->seg[t], ##__VA_ARGS__); \
       SVIMX_CHECK_LAUNCH();                                                                               \
     }                                                                                                     \
   } while (0)

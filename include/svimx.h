@@ -58,6 +58,7 @@ enum {
   SVIMX_R_NALL = 16,
   SVIMX_R_DMAX_ALL = 17,
   SVIMX_R_TILES = 18,     /* N_r tiles evaluated by the last svimx_nr_exec */
+  SVIMX_R_DT = 31,        /* stable dt: svimx_stable_dt, or the last tick of an engine job */
   SVIMX_R_COUNT = 32
 };
 
@@ -129,7 +130,11 @@ int svimx_velocity(void* stream, const float* phi, const int32_t* band, int64_t 
  * the update u0 + dt L0 is then svimx_rk_stage stage 3.  Stages 1 and 2 write
  * phi_out = 3/4 u0 + 1/4 (phi_in + dt L) and 1/3 u0 + 2/3 (phi_in + dt L) at
  * the band nodes, V never stored.  The same floats as svimx_velocity followed
- * by svimx_rk_stage (bit-identical fields and statistics). */
+ * by svimx_rk_stage (bit-identical fields and statistics).  A negative dt
+ * (SVIMX_DT_DEVICE) makes stages 1 and 2 read the step svimx_stable_dt left in
+ * the workspace's SVIMX_R_DT slot, so no host synchronisation sits between
+ * stage 0 and the rest of the step. */
+#define SVIMX_DT_DEVICE (-1.0)
 int svimx_velocity_rk(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
                       const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const svimx_law_t* law,
                       const float* field, const float* k12, double cut_lo, double cut_hi, double dt, float* L0,
@@ -174,6 +179,22 @@ int svimx_velocity_static(void* stream, const float* phi, const int32_t* band, i
 int svimx_rk_stage(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
                    const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, double dt,
                    const float* V, float* L0);
+/* svimx_rk_stage with dt read on the device from *dt_dev (FP64, rounded to
+ * float as the host argument is; a step that is not finite and positive
+ * applies 0).  No counterpart in the reference: with svimx_stable_dt it lets
+ * a step's stages follow stage 0 without a host round trip. */
+int svimx_rk_stage_dt(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                      const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const double* dt_dev,
+                      const float* V, float* L0);
+/* The stable step of ErosionSimulation.stable_dt (svierosion/evolve.py:162-170,
+ * with the max_time clamp of :199-200 when clamp != 0: dt = min(dt, max_dt))
+ * from the statistics a stage-0 velocity pass left in the workspace, written on
+ * the device into the workspace's SVIMX_R_DT slot: FP64 in the host formula's
+ * operation order, bit-identical to it.  radius is the SVI ball radius (unused
+ * by curvature laws).  The stages then take SVIMX_DT_DEVICE (svimx_velocity_rk)
+ * or the slot's address (svimx_rk_stage_dt). */
+int svimx_stable_dt(void* stream, const svimx_law_t* law, double h, double cfl, double feedback_cfl,
+                    double parabolic_cfl, double radius, int clamp, double max_dt, void* ws, int64_t ws_bytes);
 
 /* ---------------------------------------------------------------- N_r (ball convolution)
  * A plan holds the FP32 ball weights (svierosion/svi.py:24-55, computed on the
@@ -312,6 +333,15 @@ int svimx_extract_surface(void* stream, const float* phi, const int32_t* band, i
 int svimx_free(void* stream, void* device_ptr);
 /* Copy `bytes` of device memory to host memory, ordered on `stream` (synchronises it). */
 int svimx_copy_to_host(void* stream, void* dst, const void* device_src, int64_t bytes);
+/* Timing events (no counterpart in the reference: measurement only).  Created
+ * without the system-scope fence a default event performs when it is recorded
+ * (hipEventDisableSystemFence), so bracketing a kernel with them does not
+ * flush the caches between it and its neighbours.  _elapsed waits for `end`
+ * and returns the milliseconds between the two records. */
+int svimx_timing_event_create(void** event);
+int svimx_timing_event_record(void* event, void* stream);
+int svimx_timing_event_elapsed(void* start, void* end, float* ms);
+int svimx_timing_event_destroy(void* event);
 
 /* Per-point output fields of an eroded surface: p holds n index-coordinate
  * points (clamped to [1, dim - 3.001]).  SVI laws: q1 = N_r by tricubic
@@ -476,8 +506,6 @@ typedef struct svimx_job_status {
   double time, volume, displacement;
   char message[256];
 } svimx_job_status_t;
-
-#define SVIMX_R_DT 31 /* result slot: dt of the last tick (engine jobs) */
 
 /* stream NULL: the engine creates its own (needed for graph capture).  The
  * caller synchronises the streams that prepared a job before adding it;

@@ -24,6 +24,8 @@ R_VOLUME_SUM, R_HAS_NEG, R_HAS_POS, R_VMAX = 0, 1, 2, 3
 R_QMIN, R_QMAX, R_DVMAX, R_CAPPED, R_VSUM, R_NNEAR, R_DMAX = 4, 5, 6, 7, 8, 9, 10
 R_QMIN_ALL, R_QMAX_ALL, R_DVMAX_ALL, R_CAPPED_ALL, R_VSUM_ALL, R_NALL, R_DMAX_ALL = 11, 12, 13, 14, 15, 16, 17
 R_TILES = 18
+R_DT = 31
+DT_DEVICE = -1.0
 R_COUNT = 32
 
 
@@ -85,6 +87,8 @@ _SIGNATURES = {
     "svimx_velocity_static": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(Law), _P, _P, _D, _D, _I, _P,
                                    _P]),
     "svimx_rk_stage": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _D, _P, _P]),
+    "svimx_rk_stage_dt": (_I, [_P, _I, _P, _P, _P, _P, _I64, _I, _I, _I, _D, _P, _P, _P]),
+    "svimx_stable_dt": (_I, [_P, C.POINTER(Law), _D, _D, _D, _D, _D, _I, _D, _P, _I64]),
     "svimx_nr_plan_create": (_I, [_P, _I, _D, _D, _I, _I, _I, C.POINTER(C.c_void_p)]),
     "svimx_nr_plan_destroy": (_I, [_P]),
     "svimx_nr_reset": (_I, [_P, _P]),
@@ -117,6 +121,10 @@ _SIGNATURES = {
     "svimx_point_mesh_distance": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, _P, _I64, _P]),
     "svimx_free": (_I, [_P, _P]),
     "svimx_copy_to_host": (_I, [_P, _P, _P, _I64]),
+    "svimx_timing_event_create": (_I, [C.POINTER(C.c_void_p)]),
+    "svimx_timing_event_record": (_I, [_P, _P]),
+    "svimx_timing_event_elapsed": (_I, [_P, _P, C.POINTER(C.c_float)]),
+    "svimx_timing_event_destroy": (_I, [_P]),
     "svimx_engine_create": (_I, [_P, _I, C.POINTER(_P)]),
     "svimx_engine_destroy": (_I, [_P]),
     "svimx_engine_add": (_I, [_P, C.POINTER(JobDesc), C.POINTER(C.c_int)]),
@@ -195,6 +203,35 @@ def check(rc: int, what: str = "") -> None:
 def ptr(t):
     """Raw pointer of a tensor (None -> NULL)."""
     return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class TimingEvent:
+    """A HIP timing event recorded on torch's current stream, without the
+    system-scope fence of a default event (svimx_timing_event_*); the
+    record / elapsed_time interface of torch.cuda.Event."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib.svimx_timing_event_create(C.byref(h)), "timing event")
+        self._h = h
+
+    def record(self) -> None:
+        check(lib.svimx_timing_event_record(self._h, stream()), "timing event record")
+
+    def elapsed_time(self, end: "TimingEvent") -> float:
+        ms = C.c_float()
+        check(lib.svimx_timing_event_elapsed(self._h, end._h, C.byref(ms)), "timing event elapsed")
+        return float(ms.value)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and lib is not None:
+            lib.svimx_timing_event_destroy(h)
+
+
+def ptr_at(t, offset_bytes: int):
+    """Raw pointer ``offset_bytes`` into a tensor's storage."""
+    return C.c_void_p(t.data_ptr() + int(offset_bytes))
 
 
 def stream():

@@ -213,6 +213,43 @@ struct RkFuse {
   const float* cp = nullptr;
 };
 
+// The stable step of svierosion/evolve.py:162-170 from the finalised velocity
+// statistics r[SVIMX_R_VMAX ..], with the max_time clamp of :199-200 when
+// `clamp` is set: FP64 in the operation order of the host formula, without
+// contraction, so the host's and the device's dt agree bit for bit.
+__device__ __forceinline__ double stable_dt_of(const double* __restrict__ r, bool svi, bool curv, double cfl,
+                                               double h, double feedback, double radius, double parabolic,
+                                               bool clamp, double max_dt) {
+#pragma clang fp contract(off)
+  const double vmax = r[SVIMX_R_VMAX];
+  const bool near = r[SVIMX_R_NNEAR] > 0;
+  double dt = vmax > 0 ? (cfl * h) / vmax : INFINITY;
+  if (svi && (r[SVIMX_R_NALL] > 0 || near)) {
+    const double dv = near ? r[SVIMX_R_DVMAX] : r[SVIMX_R_DVMAX_ALL];
+    if (dv > 0) {
+      const double x = (feedback * ((4.0 * radius) / 3.0)) / dv;
+      dt = x < dt ? x : dt;
+    }
+  }
+  if (curv) {
+    const double dm = near ? r[SVIMX_R_DMAX] : r[SVIMX_R_DMAX_ALL];
+    if (dm > 0) {
+      const double x = ((parabolic * h) * h) / dm;
+      dt = x < dt ? x : dt;
+    }
+  }
+  if (clamp) dt = max_dt < dt ? max_dt : dt;
+  return dt;
+}
+
+// The float step a stage applies when it reads dt from device memory.  A step
+// that is not finite and positive is discarded by the caller ("stalled"), so
+// its stages apply 0 and keep the fields finite.
+__device__ __forceinline__ float device_step(const double* __restrict__ dtp) {
+  const double v = *dtp;
+  return (v > 0.0 && v < INFINITY) ? (float)v : 0.0f;
+}
+
 template <bool SVI, int MODE = 0>
 __device__ __forceinline__ void velocity_body(const float* __restrict__ phi, const int32_t* __restrict__ band,
                                               int64_t nb, Dims d, float h, const LawF& L,

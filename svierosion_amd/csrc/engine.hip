@@ -285,28 +285,8 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_eng_stats_dt(const JobDev*
     for (int s = 0; s < 15; ++s) spec.op[s] = ops[s], spec.dst[s] = (int8_t)(SVIMX_R_VMAX + s);
     finalize_body(j.partials, reduce_grid(j.nb), j.results, spec);
     if (threadIdx.x == 0) {
-      const double* r = j.results;
-      const double vmax = r[SVIMX_R_VMAX];
-      const bool near = r[SVIMX_R_NNEAR] > 0;
-      double dt = vmax > 0 ? (j.cfl * j.hd) / vmax : INFINITY;
-      if (j.requires_svi && (r[SVIMX_R_NALL] > 0 || near)) {
-        const double dv = near ? r[SVIMX_R_DVMAX] : r[SVIMX_R_DVMAX_ALL];
-        if (dv > 0) {
-          const double x = (j.feedback * ((4.0 * j.radius) / 3.0)) / dv;
-          dt = x < dt ? x : dt;
-        }
-      }
-      if (j.requires_curv) {
-        const double dm = near ? r[SVIMX_R_DMAX] : r[SVIMX_R_DMAX_ALL];
-        if (dm > 0) {
-          const double x = ((j.parabolic * j.hd) * j.hd) / dm;
-          dt = x < dt ? x : dt;
-        }
-      }
-      if (j.has_max_time) {
-        const double x = j.max_time - j.time;
-        dt = x < dt ? x : dt;
-      }
+      const double dt = stable_dt_of(j.results, j.requires_svi, j.requires_curv, j.cfl, j.hd, j.feedback, j.radius,
+                                     j.parabolic, j.has_max_time, j.has_max_time ? j.max_time - j.time : 0.0);
       dts[jslot] = dt;
       j.results[SVIMX_R_DT] = dt;
     }

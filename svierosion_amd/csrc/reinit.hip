@@ -387,8 +387,27 @@ struct WorkBandPred {
   __device__ bool operator()(int64_t i) const { return fabsf(phi[idx[i]]) < thr; }
 };
 
-// Result scratch slots 19..27 (outside the public SVIMX_R_* range).
-constexpr int kRsOldNeg = 19, kRsWorkNeg = 20, kRsOutNeg = 21, kRsLo = 22, kRsHi = 25;
+// Result scratch slots 19..29 (outside the public SVIMX_R_* range): the
+// negative counts and the new band's extent, then the work-set and band sizes,
+// so one copy brings all of them to the host.
+constexpr int kRsOldNeg = 19, kRsWorkNeg = 20, kRsOutNeg = 21, kRsLo = 22, kRsHi = 25, kRsWork = 28,
+              kRsBand = 29, kRsEnd = 30;
+
+__global__ void k_reinit_readout(double* __restrict__ results, const int64_t* __restrict__ work_ctr,
+                                 const int64_t* __restrict__ band_count) {
+  if (threadIdx.x == 0) {
+    results[kRsWork] = (double)*work_ctr;
+    results[kRsBand] = (double)*band_count;
+  }
+}
+
+// Host-visible (pinned) landing area of the readback, one per host thread.
+static double* pinned_readout() {
+  thread_local double* buf = nullptr;
+  if (!buf && hipHostMalloc(reinterpret_cast<void**>(&buf), 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    buf = nullptr;
+  return buf;
+}
 
 }  // namespace svimx
 
@@ -498,13 +517,14 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   rc = compact_async(st, WorkBandPred{phi, work_idx, thr}, work_ctr, work_cap, work_ctr, work_cap, band, n, w,
                      &band_count, ListMap{work_idx});
   if (rc) return rc;
-  double r[9];
-  int64_t counts[2];
-  SVIMX_CHECK_HIP(hipMemcpyAsync(r, w.results + kRsOldNeg, sizeof(r), hipMemcpyDeviceToHost, st));
-  SVIMX_CHECK_HIP(hipMemcpyAsync(&counts[0], work_ctr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  SVIMX_CHECK_HIP(hipMemcpyAsync(&counts[1], band_count, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  hipLaunchKernelGGL(k_reinit_readout, dim3(1), dim3(64), 0, st, w.results, work_ctr, band_count);
+  SVIMX_CHECK_LAUNCH();
+  double* r = pinned_readout();
+  SVIMX_REQUIRE(r, SVIMX_EDEVICE, "reinitialisation: no pinned host memory for the readback");
+  SVIMX_CHECK_HIP(hipMemcpyAsync(r, w.results + kRsOldNeg, sizeof(double) * (kRsEnd - kRsOldNeg),
+                                 hipMemcpyDeviceToHost, st));
   SVIMX_CHECK_HIP(hipStreamSynchronize(st));
-  const int64_t nw = counts[0], nbn = counts[1];
+  const int64_t nw = (int64_t)r[kRsWork - kRsOldNeg], nbn = (int64_t)r[kRsBand - kRsOldNeg];
   info[9] = nw;
   SVIMX_REQUIRE(nw <= work_cap, SVIMX_ECAPACITY, "reinitialisation work set needs %lld entries, capacity %lld",
                 (long long)nw, (long long)work_cap);

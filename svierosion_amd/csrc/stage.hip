@@ -46,17 +46,29 @@ template <bool SVI, int MODE>
 __global__ void __launch_bounds__(kBandThreads)
     k_velocity_rk(const float* __restrict__ phi, const int32_t* __restrict__ band, int64_t nb, Dims d, float h,
                   LawF L, const float* __restrict__ field, const float* __restrict__ k12, float cut_lo, float cut_hi,
-                  Hi3 hi, int stats, double* __restrict__ partials, RkFuse rk) {
+                  Hi3 hi, int stats, double* __restrict__ partials, RkFuse rk, const double* __restrict__ dtp) {
+  if constexpr (MODE >= 2) {
+    if (dtp) rk.dt = device_step(dtp);
+  }
   velocity_body<SVI, MODE>(phi, band, nb, d, h, L, field, k12, cut_lo, cut_hi, hi, stats, nullptr, partials,
                            blockIdx.x, gridDim.x, rk);
 }
 
+// dtp, when set, replaces dt by the device step it points to
 __global__ void __launch_bounds__(kBandThreads)
     k_rk_stage(int stage, const float* __restrict__ in, const float* __restrict__ u0, float* __restrict__ out,
                const int32_t* __restrict__ band, int64_t nb, Dims d, float inv_h, float dt,
-               const float* __restrict__ V, float* __restrict__ L0) {
+               const float* __restrict__ V, float* __restrict__ L0, const double* __restrict__ dtp) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (dtp) dt = device_step(dtp);
   if (q < nb) rk_node(stage, in, u0, out, band, q, d, inv_h, dt, V, L0);
+}
+
+__global__ void k_stable_dt(double* __restrict__ results, int svi, int curv, double cfl, double h, double feedback,
+                            double radius, double parabolic, int clamp, double max_dt) {
+  if (threadIdx.x == 0)
+    results[SVIMX_R_DT] = stable_dt_of(results, svi != 0, curv != 0, cfl, h, feedback, radius, parabolic,
+                                       clamp != 0, max_dt);
 }
 
 // ---------------------------------------------------------------- operators
@@ -351,9 +363,11 @@ static int velocity_rk(void* stream, int stage, const float* phi_in, const float
     rk.cp = cp;
     const LawF lf = law_to_f(*law);
     const float clo = (float)(cut_lo * h), chi = (float)(cut_hi * h);
+    // a negative dt selects the step svimx_stable_dt left in the workspace
+    const double* dtp = dt < 0 ? w.results + SVIMX_R_DT : nullptr;
 #define SVIMX_VRK(SVI_, MODE_)                                                                                   \
   hipLaunchKernelGGL((k_velocity_rk<SVI_, MODE_>), dim3(nblk), dim3(kBandThreads), 0, st, phi_in, band, nb, d,   \
-                     (float)h, lf, field, k12, clo, chi, hi_of(d), stats, w.partials, rk)
+                     (float)h, lf, field, k12, clo, chi, hi_of(d), stats, w.partials, rk, dtp)
     if (law->kind == 0) {
       if (stage == 0) SVIMX_VRK(true, 1); else if (stage == 1) SVIMX_VRK(true, 2); else SVIMX_VRK(true, 3);
     } else {
@@ -403,16 +417,43 @@ int svimx_velocity_static(void* stream, const float* phi, const int32_t* band, i
   SVIMX_TRY_END
 }
 
-int svimx_rk_stage(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
-                   const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, double dt, const float* V,
-                   float* L0) {
+static int rk_stage(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                    const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, double dt,
+                    const double* dt_dev, const float* V, float* L0) {
   SVIMX_TRY_BEGIN
   SVIMX_REQUIRE(stage >= 0 && stage <= 3, SVIMX_EBADARG, "stage must be 0..3");
   SVIMX_REQUIRE(phi_in != phi_out || stage == 3, SVIMX_EBADARG, "stage output may not alias its input");
   SVIMX_REQUIRE(stage != 3 || L0, SVIMX_EBADARG, "stage 3 replays L0");
   if (nb <= 0) return SVIMX_OK;
   hipLaunchKernelGGL(k_rk_stage, GRID1(nb), stage, phi_in, u0, phi_out, band, nb, Dims{nx, ny, nz},
-                     (float)(1.0 / h), (float)dt, V, L0);
+                     (float)(1.0 / h), (float)dt, V, L0, dt_dev);
+  SVIMX_CHECK_LAUNCH();
+  return SVIMX_OK;
+  SVIMX_TRY_END
+}
+
+int svimx_rk_stage(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                   const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, double dt, const float* V,
+                   float* L0) {
+  return rk_stage(stream, stage, phi_in, u0, phi_out, band, nb, nx, ny, nz, h, dt, nullptr, V, L0);
+}
+
+int svimx_rk_stage_dt(void* stream, int stage, const float* phi_in, const float* u0, float* phi_out,
+                      const int32_t* band, int64_t nb, int nx, int ny, int nz, double h, const double* dt_dev,
+                      const float* V, float* L0) {
+  SVIMX_REQUIRE(dt_dev, SVIMX_EBADARG, "svimx_rk_stage_dt: null dt pointer");
+  return rk_stage(stream, stage, phi_in, u0, phi_out, band, nb, nx, ny, nz, h, 0.0, dt_dev, V, L0);
+}
+
+int svimx_stable_dt(void* stream, const svimx_law_t* law, double h, double cfl, double feedback_cfl,
+                    double parabolic_cfl, double radius, int clamp, double max_dt, void* ws, int64_t ws_bytes) {
+  SVIMX_TRY_BEGIN
+  SVIMX_REQUIRE(law && law->kind >= 0 && law->kind <= 2, SVIMX_EBADARG, "bad law");
+  SVIMX_REQUIRE(ws && ws_bytes >= (int64_t)(sizeof(double) * SVIMX_R_COUNT), SVIMX_EBADARG,
+                "svimx_stable_dt: the workspace must hold the result slots");
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_stable_dt, dim3(1), dim3(64), 0, st, static_cast<double*>(ws), law->kind == 0 ? 1 : 0,
+                     law->kind != 0 ? 1 : 0, cfl, h, feedback_cfl, radius, parabolic_cfl, clamp ? 1 : 0, max_dt);
   SVIMX_CHECK_LAUNCH();
   return SVIMX_OK;
   SVIMX_TRY_END

@@ -326,4 +326,33 @@ int svimx_copy_to_host(void* stream, void* dst, const void* src, int64_t bytes) 
   return SVIMX_OK;
 }
 
+// Timing events: no system-scope fence when they are recorded, so bracketing a
+// launch with them does not write back and invalidate the caches between that
+// launch and its neighbours (what a default event does on this device).
+int svimx_timing_event_create(void** event) {
+  SVIMX_REQUIRE(event, SVIMX_EBADARG, "null argument");
+  hipEvent_t e = nullptr;
+  SVIMX_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+  *event = e;
+  return SVIMX_OK;
+}
+
+int svimx_timing_event_record(void* event, void* stream) {
+  SVIMX_REQUIRE(event, SVIMX_EBADARG, "null event");
+  SVIMX_CHECK_HIP(hipEventRecord(static_cast<hipEvent_t>(event), as_stream(stream)));
+  return SVIMX_OK;
+}
+
+int svimx_timing_event_elapsed(void* start, void* end, float* ms) {
+  SVIMX_REQUIRE(start && end && ms, SVIMX_EBADARG, "null argument");
+  SVIMX_CHECK_HIP(hipEventSynchronize(static_cast<hipEvent_t>(end)));
+  SVIMX_CHECK_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)));
+  return SVIMX_OK;
+}
+
+int svimx_timing_event_destroy(void* event) {
+  if (event) SVIMX_CHECK_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return SVIMX_OK;
+}
+
 }  // extern "C"

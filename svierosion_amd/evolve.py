@@ -14,10 +14,11 @@ floats, bit-identical fields.)
 
 Buffers: the step-start field u0 stays in one dense copy (A); stage outputs go
 to B, C, B, so a secant retry restarts from A without a copy and replays
-stage 0 from the saved L0 (svierosion/evolve.py:215-231).  The host reads the
-results twice per accepted step: the dt statistics after stage 0 and the
-volume after stage 2; the fractions of the final state are reused by the next
-step's first stage.
+stage 0 from the saved L0 (svierosion/evolve.py:215-231).  dt is computed on
+the device from stage 0's statistics (``svimx_stable_dt``, the host formula
+bit for bit) and the later stages read it there, so the host reads the
+results once per accepted step, after the volume of the new state; the
+fractions of that state are reused by the next step's first stage.
 """
 
 from __future__ import annotations
@@ -94,11 +95,18 @@ class ErosionSimulation:
         else:
             self.k12 = torch.zeros(2 * n, dtype=torch.float32, device=dev)
         self._alloc_band_arrays()
-        self._frac_valid = None     # buffer index whose fractions/marks are current
+        self._fv = None             # buffer index whose fractions/marks are current
+        self._nr_valid = None       # buffer whose N_r is in self.field (None, or _frac_valid)
+        self._res_host = None       # pinned landing area of the result slots
+        self._res_event = None
         # run(): the velocity pass fused with the RK stage that consumes it
         # (svimx_velocity_rk); False runs svimx_velocity + svimx_rk_stage, the
         # same floats in two passes
         self.fused = True
+        # run() with fused: dt computed on the device from stage 0's statistics
+        # (svimx_stable_dt, bit-identical to stable_dt); False reads the
+        # statistics on the host first (two host round trips per step)
+        self.device_dt = True
         self.tuning = {}
         if self.conv is not None and autotune and levelset.nb > 0:
             # pick the N_r z-run length on this job's field (results do not depend on it)
@@ -109,10 +117,20 @@ class ErosionSimulation:
         self.history: list[StepRecord] = []
         self._displacement = 0.0
         self.timers = {"stats_sync": 0.0, "volume_sync": 0.0}
-        # name -> [(start, end) torch.cuda.Event pairs]; None disables event recording
+        # name -> [(start, end) timing-event pairs]; None disables event recording
         self.kernel_events = None
         # stage names to record (None: every stage); each span costs two event records
         self.kernel_event_names = None
+
+    @property
+    def _frac_valid(self):
+        return self._fv
+
+    @_frac_valid.setter
+    def _frac_valid(self, buf) -> None:
+        # new fractions/marks (or none) leave no N_r current
+        self._fv = buf
+        self._nr_valid = None
 
     @contextlib.contextmanager
     def _span(self, name: str):
@@ -120,8 +138,8 @@ class ErosionSimulation:
         if self.kernel_events is None or (self.kernel_event_names is not None and name not in self.kernel_event_names):
             yield
             return
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
+        a = _abi.TimingEvent()
+        b = _abi.TimingEvent()
         a.record()
         yield
         b.record()
@@ -146,8 +164,16 @@ class ErosionSimulation:
                 raise ValueError("grid padding is smaller than the SVI ball radius; use a larger padding")
 
     def _alloc_band_arrays(self) -> None:
-        cap = max(self.ls.nb, 1)
+        """Band-indexed V and L0 (and the closest points) for at least nb entries;
+        kept while large enough and grown with 25% headroom, so the band's drift
+        at reinitialisations rarely reallocates them."""
+        need = max(self.ls.nb, 1)
+        if getattr(self, "V", None) is not None and self.V.numel() >= need and \
+                (self.newton_steps == 1 or (self.cp is not None and self.cp.numel() >= 3 * need)):
+            return
+        cap = need + need // 4
         dev = self.ls.device
+        self.V = self.L0 = self.cp = None
         self.V = torch.empty(cap, dtype=torch.float32, device=dev)
         self.L0 = torch.empty(cap, dtype=torch.float32, device=dev)
         if self.newton_steps > 1:
@@ -181,17 +207,44 @@ class ErosionSimulation:
                     _abi.check(_abi.lib.svimx_nr_mark_points(st, self.conv.plan, _abi.ptr(self.cp), ls.nb),
                                "N_r marks")
             if self.conv is not None:
-                with self._span("nr_select"):
-                    _abi.check(_abi.lib.svimx_nr_select(st, self.conv.plan), "N_r select")
-                with self._span("nr_sum"):
-                    _abi.check(_abi.lib.svimx_nr_sum(st, self.conv.plan, _abi.ptr(ls.occupancy), _abi.ptr(self.field),
-                                                     _abi.ptr(ls.ws), ls.ws.numel()), "N_r")
+                if self._nr_valid != buf:
+                    self._nr_inputs()
             else:
                 with self._span("curvatures"):
                     _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(ls.flat), _abi.ptr(ls._band), ls.nb, *ls.dims,
                                                          ls.h, _abi.ptr(self.k12)))
         finally:
             ls.cur = saved
+
+    def _nr_inputs(self) -> None:
+        """N_r at the nodes the next velocity pass reads: selection and summation
+        over the marks and occupancy of the last fractions pass."""
+        ls = self.ls
+        st = _abi.stream()
+        with self._span("nr_select"):
+            _abi.check(_abi.lib.svimx_nr_select(st, self.conv.plan), "N_r select")
+        with self._span("nr_sum"):
+            _abi.check(_abi.lib.svimx_nr_sum(st, self.conv.plan, _abi.ptr(ls.occupancy), _abi.ptr(self.field),
+                                             _abi.ptr(ls.ws), ls.ws.numel()), "N_r")
+
+    def _read_results(self, speculate: int | None = None) -> np.ndarray:
+        """Host copy of the result slots.  With ``speculate`` = the buffer whose
+        fractions were just computed, that buffer's N_r (what the next step's
+        stage 0 reads) is launched between the copy and the wait, so the device
+        keeps working while the host reads the results."""
+        ls = self.ls
+        if self._res_host is None:
+            self._res_host = torch.empty(_abi.R_COUNT, dtype=torch.float64, pin_memory=True)
+            self._res_event = torch.cuda.Event()
+        self._res_host.copy_(_abi.results_view(ls.ws), non_blocking=True)
+        self._res_event.record()
+        if speculate is not None:
+            self._nr_inputs()
+            self._nr_valid = speculate
+        t0 = _time.perf_counter()
+        self._res_event.synchronize()
+        self.timers["volume_sync"] += _time.perf_counter() - t0
+        return self._res_host.numpy().copy()
 
     def _prepare(self, buf: int, stats: bool) -> None:
         """Fractions, N_r / curvatures and velocity (into self.V) for the field in bufs[buf]."""
@@ -243,9 +296,25 @@ class ErosionSimulation:
 
     def _stage_call(self, stage: int, src: int, u0: int, dst: int, dt: float) -> None:
         ls = self.ls
-        _abi.check(_abi.lib.svimx_rk_stage(_abi.stream(), stage, _abi.ptr(ls.bufs[src]), _abi.ptr(ls.bufs[u0]),
-                                           _abi.ptr(ls.bufs[dst]), _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h,
-                                           float(dt), _abi.ptr(self.V), _abi.ptr(self.L0)), "rk stage")
+        args = (_abi.stream(), stage, _abi.ptr(ls.bufs[src]), _abi.ptr(ls.bufs[u0]), _abi.ptr(ls.bufs[dst]),
+                _abi.ptr(ls._band), ls.nb, *ls.dims, ls.h)
+        if dt == _abi.DT_DEVICE:
+            # the step svimx_stable_dt left in the workspace's dt slot
+            rc = _abi.lib.svimx_rk_stage_dt(*args, _abi.ptr_at(ls.ws, 8 * _abi.R_DT), _abi.ptr(self.V),
+                                            _abi.ptr(self.L0))
+        else:
+            rc = _abi.lib.svimx_rk_stage(*args, float(dt), _abi.ptr(self.V), _abi.ptr(self.L0))
+        _abi.check(rc, "rk stage")
+
+    def _device_dt(self, max_dt: float | None) -> None:
+        """stable_dt (and the max_time clamp) on the device from the statistics of
+        the stage-0 pass, into the workspace's dt slot; bit-identical to the host
+        formula."""
+        ls = self.ls
+        _abi.check(_abi.lib.svimx_stable_dt(_abi.stream(), self._law_abi, ls.h, self.cfl, self.feedback_cfl,
+                                            self.parabolic_cfl, float(getattr(self.law, "radius", 0.0)),
+                                            int(max_dt is not None), float(max_dt or 0.0), _abi.ptr(ls.ws),
+                                            ls.ws.numel()), "stable dt")
 
     def _rk3_tail(self, a: int, b: int, c: int, dt: float) -> None:
         """Stages 1 and 2 after stage 0 wrote bufs[b]; the result lands in bufs[b]."""
@@ -260,7 +329,10 @@ class ErosionSimulation:
         self._prepare(c, False)
         self._stage(2, c, a, b, dt)
 
-    def _volume_of(self, buf: int) -> tuple[float, np.ndarray]:
+    def _volume_of(self, buf: int, speculate: bool = False) -> tuple[float, np.ndarray]:
+        """Volume of bufs[buf] (its fractions stay current).  ``speculate``: the
+        next step will most likely start from this buffer, so its N_r is launched
+        before the host waits for the volume."""
         ls = self.ls
         saved = ls.cur
         ls.cur = buf
@@ -268,22 +340,32 @@ class ErosionSimulation:
             with self._span("fractions"):
                 ls.fractions(self._plan(), self.cutoff[1])
             self._frac_valid = buf
-            t0 = _time.perf_counter()
-            r = ls.results()
-            self.timers["volume_sync"] += _time.perf_counter() - t0
+            spec = buf if speculate and self.conv is not None and self.newton_steps == 1 else None
+            r = self._read_results(spec)
             return ls.volume_from_results(r), r
         finally:
             ls.cur = saved
+
+    def _no_reinit_after_step(self) -> bool:
+        """True when the step in flight cannot trigger a reinitialisation: it moves
+        the front by dt max V <= cfl h (dt <= cfl h / max V), so the displacement
+        stays below the reinitialisation distance."""
+        h = self.ls.h
+        return self._displacement + self.cfl * h * (1.0 + 1e-9) < self.reinit_distance * h
 
     def _stats(self) -> tuple[float, dict]:
         t0 = _time.perf_counter()
         r = self.ls.results()
         self.timers["stats_sync"] += _time.perf_counter() - t0
+        return self._stats_of(r)
+
+    @staticmethod
+    def _stats_of_law(law, r) -> tuple[float, dict]:
         info = {}
         # statistics over |phi| < h, or over every evaluated node when that set is
         # empty (svierosion/evolve.py:131, :147)
         near = r[_abi.R_NNEAR] > 0
-        if self.law.requires_svi:
+        if law.requires_svi:
             if r[_abi.R_NALL] > 0 or near:
                 info["qmin"] = float(r[_abi.R_QMIN if near else _abi.R_QMIN_ALL])
                 info["qmax"] = float(r[_abi.R_QMAX if near else _abi.R_QMAX_ALL])
@@ -298,6 +380,10 @@ class ErosionSimulation:
         elif r[_abi.R_NALL] > 0:
             info["vmean"] = float(r[_abi.R_VSUM_ALL] / r[_abi.R_NALL])
         return float(r[_abi.R_VMAX]), info
+
+    def _stats_of(self, r) -> tuple[float, dict]:
+        """max V and the record's statistics from the result slots of stage 0."""
+        return self._stats_of_law(self.law, r)
 
     def stable_dt(self, vmax: float, info: dict) -> float:
         """min(cfl h/max V, feedback_cfl (4r/3)/max dV/dN, parabolic_cfl h^2/max D) (svierosion/evolve.py:162-170)."""
@@ -328,7 +414,8 @@ class ErosionSimulation:
         ls = self.ls
         V0 = self.initial_volume
         target = None if target_fraction is None else target_fraction * V0
-        vol, res = self._volume_of(ls.cur)
+        ls.reserve_reinit()
+        vol, res = self._volume_of(ls.cur, speculate=True)
         reason = "max_steps"
         n_start = self.steps
         while self.steps - n_start < max_steps:
@@ -345,16 +432,29 @@ class ErosionSimulation:
             a = ls.cur
             b, c = (a + 1) % 3, (a + 2) % 3
             self._stage0(a)
-            vmax, info = self._stats()
-            dt = self.stable_dt(vmax, info)
-            if max_time is not None:
-                dt = min(dt, max_time - self.time)
-            if not math.isfinite(dt) or dt <= 0:
-                reason = "stalled"
-                break
-            self._stage0_update(a, b, dt)
-            self._rk3_tail(a, b, c, dt)
-            vnew, res = self._volume_of(b)
+            if self.fused and self.device_dt:
+                # dt on the device: the whole step runs without a host round trip
+                # and the host reads the statistics, dt and the volume once
+                self._device_dt(None if max_time is None else max_time - self.time)
+                self._stage0_update(a, b, _abi.DT_DEVICE)
+                self._rk3_tail(a, b, c, _abi.DT_DEVICE)
+                vnew, res = self._volume_of(b, speculate=self._no_reinit_after_step())
+                vmax, info = self._stats_of(res)
+                dt = float(res[_abi.R_DT])
+                if not math.isfinite(dt) or dt <= 0:
+                    reason = "stalled"  # the step's stages applied 0; u0 stays current
+                    break
+            else:
+                vmax, info = self._stats()
+                dt = self.stable_dt(vmax, info)
+                if max_time is not None:
+                    dt = min(dt, max_time - self.time)
+                if not math.isfinite(dt) or dt <= 0:
+                    reason = "stalled"
+                    break
+                self._stage0_update(a, b, dt)
+                self._rk3_tail(a, b, c, dt)
+                vnew, res = self._volume_of(b, speculate=self._no_reinit_after_step())
             if target is not None and vnew < target:
                 lo_t, lo_v, hi_t, hi_v = 0.0, vol, dt, vnew
                 for _ in range(8):
@@ -388,7 +488,7 @@ class ErosionSimulation:
                 self._frac_valid = None
                 self._displacement = 0.0
                 did_reinit = True
-                vnew, res = self._volume_of(ls.cur)
+                vnew, res = self._volume_of(ls.cur, speculate=True)
             rec = StepRecord(self.steps, self.time, dt, vnew, vnew / V0, vmax, info.get("vmean", vmax),
                              info.get("qmin", float("nan")), info.get("qmax", float("nan")),
                              info.get("capped", 0), did_reinit, _time.time() - t_wall)

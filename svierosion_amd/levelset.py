@@ -57,6 +57,7 @@ class LevelSet:
         self.nb = 0
         self.inside_outside = 0
         self._reinit_bufs = None
+        self._reinit_work = None
         self.rebuild_band()
 
     # ------------------------------------------------------------ state
@@ -187,9 +188,7 @@ class LevelSet:
         info = (C.c_int64 * 10)()
         cap = min(n, 2 * self.nb + 4096)
         for _ in range(2):
-            widx = torch.empty(cap, dtype=torch.int32, device=self.device)
-            winfo = torch.empty(cap, dtype=torch.int32, device=self.device)
-            geo = torch.empty(8 * cap, dtype=torch.float32, device=self.device)
+            widx, winfo, geo = self._work_buffers(cap)
             rc = _abi.lib.svimx_reinit(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(others[0]), _abi.ptr(others[1]),
                                        _abi.ptr(self.occupancy), _abi.ptr(self._band), self.nb,
                                        self.inside_outside, *self.dims, self.h, self.width, int(iterations),
@@ -204,6 +203,29 @@ class LevelSet:
             self.inside_outside = int(info[1])
         self.last_work_nodes = int(info[9])
         _abi.check(rc, "reinitialisation")
+
+    def reserve_reinit(self) -> None:
+        """Allocate the reinitialisation's masks and work buffers now (for the
+        current band, with the usual headroom) rather than at the first
+        reinitialisation."""
+        n = self.grid.size
+        if self._reinit_bufs is None:
+            self._reinit_bufs = (torch.empty(n, dtype=torch.uint8, device=self.device),
+                                 torch.empty(n, dtype=torch.uint8, device=self.device))
+        self._work_buffers(min(n, 2 * self.nb + 4096))
+
+    def _work_buffers(self, cap: int):
+        """Work list, per-node flags and geometry for a work set of ``cap`` nodes,
+        kept between reinitialisations and grown with 25% headroom, so the band's
+        drift does not reallocate them every time."""
+        have = self._reinit_work[0].numel() if self._reinit_work is not None else -1
+        if have < cap:
+            size = min(self.grid.size, cap + cap // 4)
+            self._reinit_work = None  # release the old buffers before the new ones
+            self._reinit_work = (torch.empty(size, dtype=torch.int32, device=self.device),
+                                 torch.empty(size, dtype=torch.int32, device=self.device),
+                                 torch.empty(8 * size, dtype=torch.float32, device=self.device))
+        return self._reinit_work
 
 
 # ------------------------------------------------------------- interpolation

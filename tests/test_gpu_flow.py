@@ -208,6 +208,38 @@ def test_fused_velocity_rk_is_bit_identical(device, law_name):
     print(f"{law_name}: {len(a.history)} steps identical fused and unfused", flush=True)
 
 
+@pytest.mark.parametrize("law_name", ["svi", "mean"])
+def test_device_dt_matches_host_dt(device, law_name):
+    """dt computed on the device (svimx_stable_dt, stages reading it there) against
+    the host's stable_dt (svierosion/evolve.py:162-170) on the same fused path: a
+    run to a target through reinitialisations and the secant stop, then a run that
+    ends on max_time (the clamp of :199-200), give identical histories and fields."""
+    from svierosion_amd.evolve import ErosionSimulation
+    from svierosion_amd.flows import MeanCurvatureLaw, SVIPowerLaw
+    out = []
+    for on_device in (True, False):
+        if law_name == "svi":
+            ls, g, _ = _gpu_sphere(1.0, 0.05, 0.3 + 10 * 0.05)
+            sim = ErosionSimulation(ls, SVIPowerLaw(0.3, 2, 1, 0.93))
+        else:
+            ls, g, _ = _gpu_sphere(1.0, 0.05, 0.6)
+            sim = ErosionSimulation(ls, MeanCurvatureLaw(1.0))
+        sim.device_dt = on_device
+        first = sim.run(target_fraction=0.75)
+        t_end = sim.time + 2.5 * sim.history[-1].dt   # ends inside the third step
+        second = sim.run(target_fraction=None, max_time=t_end)
+        out.append((first, second, sim.history, ls.flat.clone(), ls.band.clone()))
+    (a1, a2, ha, fa, ba), (b1, b2, hb, fb, bb) = out
+    assert a1.stop_reason == b1.stop_reason == "target_volume" and any(r.reinit for r in ha)
+    assert a2.stop_reason == b2.stop_reason == "max_time" and a2.steps == b2.steps
+    key = lambda h: [(r.step, r.time, r.dt, r.volume, r.vmax, r.vmean, r.qmin, r.qmax, r.capped, r.reinit)
+                     for r in h]
+    assert str(key(ha)) == str(key(hb))       # str: NaN statistics compare equal
+    assert torch.equal(ba, bb) and torch.equal(fa, fb)
+    assert ha[-1].dt < ha[-2].dt              # the last step was clamped to max_time
+    print(f"{law_name}: {len(ha)} steps identical with dt on the device and on the host", flush=True)
+
+
 def test_closest_points_newton_steps_match_oracle(device):
     """LevelSet.closest_points with extra Newton steps (svierosion/levelset.py:129-151)
     on a field that is not a distance function (the distorted sphere): the device

@@ -281,6 +281,31 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
                  int iterations, double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b,
                  int32_t* work_idx, int32_t* work_info, float* geo, int64_t work_cap, void* ws, int64_t ws_bytes,
                  int64_t* info);
+/* svimx_reinit that runs only when the step whose results are in the
+ * workspace (stage-0 statistics, the SVIMX_R_DT of svimx_stable_dt, the
+ * volume of svimx_fractions) triggers it, decided on the device by the rule of
+ * ErosionSimulation.run (svierosion/evolve.py:232-245) in the host's FP64
+ * operation order: dt finite and positive, volume >= target (no secant retry),
+ * volume > target_stop (not reached) and displacement + dt * max V >=
+ * threshold.  So a caller can queue it before reading the step's results.
+ * info holds 11 int64: those of svimx_reinit and info[10] = 1 when it ran; when
+ * it did not, nothing changed (info[0] = nb, info[1] = inside_outside).  No
+ * counterpart in the reference. */
+typedef struct svimx_reinit_gate {
+  double displacement;   /* front displacement accumulated before the step */
+  double threshold;      /* reinit_distance * h */
+  double inside_outside; /* inside nodes outside the band (volume = (this + sum) * h3) */
+  double h3;             /* h ** 3 as the host computes it */
+  double target;         /* target volume, when has_target */
+  double target_stop;    /* target * (1 + tol) */
+  int32_t has_target;
+  int32_t reserved;
+} svimx_reinit_gate_t;
+int svimx_reinit_gated(void* stream, float* phi, float* scratch, float* phi_copy, float* occupancy, int32_t* band,
+                       int64_t nb, int64_t inside_outside, int nx, int ny, int nz, double h, double width,
+                       int iterations, double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b,
+                       int32_t* work_idx, int32_t* work_info, float* geo, int64_t work_cap, void* ws,
+                       int64_t ws_bytes, const svimx_reinit_gate_t* gate, int64_t* info);
 
 /* ---------------------------------------------------------------- mesh -> signed distance
  * Dense phi = +-min(d, band)*h, negative inside, for a triangle mesh already in

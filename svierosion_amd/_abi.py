@@ -53,6 +53,13 @@ class JobDesc(C.Structure):
                 ("displacement", C.c_double), ("steps", C.c_int64)]
 
 
+class ReinitGate(C.Structure):
+    """svimx_reinit_gate_t."""
+    _fields_ = [("displacement", C.c_double), ("threshold", C.c_double), ("inside_outside", C.c_double),
+                ("h3", C.c_double), ("target", C.c_double), ("target_stop", C.c_double),
+                ("has_target", C.c_int32), ("reserved", C.c_int32)]
+
+
 class StepRec(C.Structure):
     """svimx_step_record_t."""
     _fields_ = [("step", C.c_int64), ("time", C.c_double), ("dt", C.c_double), ("volume", C.c_double),
@@ -114,6 +121,8 @@ _SIGNATURES = {
     "svimx_nr_plan_method": (_I, [_P]),
     "svimx_reinit": (_I, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I, _I, _I, _D, _D, _I, _D, _I, _I, _P, _P, _P, _P,
                           _P, _I64, _P, _I64, _P]),
+    "svimx_reinit_gated": (_I, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I, _I, _I, _D, _D, _I, _D, _I, _I, _P, _P, _P,
+                                _P, _P, _I64, _P, _I64, C.POINTER(ReinitGate), _P]),
     "svimx_sdf_build": (_I, [_P, _P, _P, _P, _I64, _P, _P, _P, _P, _I64, _P, _I64, _I, _I, _I, _I, _I, _D, _D, _D,
                              _P, _P, _P, _P, _I64]),
     "svimx_extract_surface": (_I, [_P, _P, _P, _I64, _I, _I, _I, _D, C.POINTER(C.c_void_p),

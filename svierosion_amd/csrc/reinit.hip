@@ -219,12 +219,34 @@ static int compact_bits(hipStream_t st, const uint32_t* m, int64_t nw, int wpl, 
 // header): the work-set size.  A work set larger than the caller's capacity leaves every kernel
 // after the compaction idle (the field is not touched) and is reported at
 // the end, so the caller can retry with the exact size.
-enum { kCtrWork = 0 };
+enum { kCtrWork = 0, kCtrGate = 1 };
 
-__global__ void k_reinit_counters(long long* ctr, const int64_t* work_total) {
+// The work count the later kernels read; a closed gate leaves it 0, so every
+// kernel after this one (geometry, sweeps, rebuild, band compaction) changes
+// nothing.
+__global__ void k_reinit_counters(long long* ctr, const int64_t* work_total, int gated) {
   if (threadIdx.x == 0) {
-    ctr[kCtrWork] = (long long)*work_total;
+    ctr[kCtrWork] = (gated && ctr[kCtrGate] == 0) ? 0 : (long long)*work_total;
   }
+}
+
+// The reinitialisation trigger of ErosionSimulation.run (svierosion/evolve.py:
+// 232-245) for the step whose results (stage-0 statistics, device dt, volume)
+// sit in the workspace: the step is accepted (finite positive dt), needs no
+// secant retry (volume >= target), does not reach the target and moves the
+// displacement to the threshold.  FP64 in the host's operation order, without
+// contraction, so the device opens the gate exactly when the host decides to
+// reinitialise.
+__global__ void k_reinit_gate(const double* __restrict__ results, long long* ctr, svimx_reinit_gate_t g) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  const double dt = results[SVIMX_R_DT], vmax = results[SVIMX_R_VMAX];
+  const double vnew = (g.inside_outside + results[SVIMX_R_VOLUME_SUM]) * g.h3;
+  bool open = dt > 0.0 && dt < INFINITY;
+  if (g.has_target && vnew < g.target) open = false;
+  const double disp = g.displacement + dt * vmax;
+  const bool reached = g.has_target && vnew <= g.target_stop;
+  ctr[kCtrGate] = (open && disp >= g.threshold && !reached) ? 1 : 0;
 }
 
 __device__ __forceinline__ int64_t work_count(const long long* ctr, int64_t cap) {
@@ -387,17 +409,19 @@ struct WorkBandPred {
   __device__ bool operator()(int64_t i) const { return fabsf(phi[idx[i]]) < thr; }
 };
 
-// Result scratch slots 19..29 (outside the public SVIMX_R_* range): the
-// negative counts and the new band's extent, then the work-set and band sizes,
-// so one copy brings all of them to the host.
+// Result scratch slots 19..30 (outside the public SVIMX_R_* range): the
+// negative counts and the new band's extent, then the work-set and band sizes
+// and the gate, so one copy brings all of them to the host.
 constexpr int kRsOldNeg = 19, kRsWorkNeg = 20, kRsOutNeg = 21, kRsLo = 22, kRsHi = 25, kRsWork = 28,
-              kRsBand = 29, kRsEnd = 30;
+              kRsBand = 29, kRsGate = 30, kRsEnd = 31;
 
 __global__ void k_reinit_readout(double* __restrict__ results, const int64_t* __restrict__ work_ctr,
-                                 const int64_t* __restrict__ band_count) {
+                                 const int64_t* __restrict__ band_count, const long long* __restrict__ ctr,
+                                 int gated) {
   if (threadIdx.x == 0) {
     results[kRsWork] = (double)*work_ctr;
     results[kRsBand] = (double)*band_count;
+    results[kRsGate] = gated ? (double)ctr[kCtrGate] : 1.0;
   }
 }
 
@@ -421,10 +445,11 @@ extern "C" {
 // plain / crossing split -> 2 x iterations sweeps -> rebuild over the work set
 // -> new band (order-preserving compaction of the work list) -> one readback of
 // the counts and extents.
-int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, float* occupancy, int32_t* band,
-                 int64_t nb, int64_t inside_outside, int nx, int ny, int nz, double h, double width, int iterations,
-                 double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b, int32_t* work_idx,
-                 int32_t* work_info, float* geo, int64_t work_cap, void* ws, int64_t ws_bytes, int64_t* info) {
+static int reinit(void* stream, float* phi, float* scratch, float* phi_copy, float* occupancy, int32_t* band,
+                  int64_t nb, int64_t inside_outside, int nx, int ny, int nz, double h, double width, int iterations,
+                  double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b, int32_t* work_idx,
+                  int32_t* work_info, float* geo, int64_t work_cap, void* ws, int64_t ws_bytes,
+                  const svimx_reinit_gate_t* gate, int64_t* info) {
   SVIMX_TRY_BEGIN
   Dims d{nx, ny, nz};
   const int64_t n = d.size();
@@ -456,6 +481,11 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   uint32_t* ba = reinterpret_cast<uint32_t*>(mask_a);
   uint32_t* bb = reinterpret_cast<uint32_t*>(mask_b);
   const dim3 gwd(blocks_for(nwords, 256));
+  const int gated = gate ? 1 : 0;
+  if (gate) {
+    hipLaunchKernelGGL(k_reinit_gate, dim3(1), dim3(64), 0, st, w.results, w.counters, *gate);
+    SVIMX_CHECK_LAUNCH();
+  }
   hipLaunchKernelGGL(k_bits_fill, gwd, tb, 0, st, ba, nwords);
   SVIMX_CHECK_LAUNCH();
   int nblk = reduce_grid(nb);
@@ -479,7 +509,7 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   rc = compact_bits(st, bb, nwords, wpl, nz, work_idx, work_cap, w, &work_total);
   if (rc) return rc;
   long long* ctr = w.counters;
-  hipLaunchKernelGGL(k_reinit_counters, dim3(1), dim3(64), 0, st, ctr, work_total);
+  hipLaunchKernelGGL(k_reinit_counters, dim3(1), dim3(64), 0, st, ctr, work_total, gated);
   SVIMX_CHECK_LAUNCH();
   const float capf = (float)(width * h);
   const float thr = (float)(width * h * (1.0 - 1e-5));
@@ -517,7 +547,7 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   rc = compact_async(st, WorkBandPred{phi, work_idx, thr}, work_ctr, work_cap, work_ctr, work_cap, band, n, w,
                      &band_count, ListMap{work_idx});
   if (rc) return rc;
-  hipLaunchKernelGGL(k_reinit_readout, dim3(1), dim3(64), 0, st, w.results, work_ctr, band_count);
+  hipLaunchKernelGGL(k_reinit_readout, dim3(1), dim3(64), 0, st, w.results, work_ctr, band_count, ctr, gated);
   SVIMX_CHECK_LAUNCH();
   double* r = pinned_readout();
   SVIMX_REQUIRE(r, SVIMX_EDEVICE, "reinitialisation: no pinned host memory for the readback");
@@ -525,6 +555,17 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
                                  hipMemcpyDeviceToHost, st));
   SVIMX_CHECK_HIP(hipStreamSynchronize(st));
   const int64_t nw = (int64_t)r[kRsWork - kRsOldNeg], nbn = (int64_t)r[kRsBand - kRsOldNeg];
+  if (gate) {
+    info[10] = r[kRsGate - kRsOldNeg] != 0.0 ? 1 : 0;
+    if (!info[10]) {  // closed: nothing changed; the band is the one passed in
+      info[9] = 0;
+      info[0] = nb;
+      info[1] = inside_outside;
+      info[2] = 1;
+      for (int a = 0; a < 6; ++a) info[3 + a] = 0;
+      return SVIMX_OK;
+    }
+  }
   info[9] = nw;
   SVIMX_REQUIRE(nw <= work_cap, SVIMX_ECAPACITY, "reinitialisation work set needs %lld entries, capacity %lld",
                 (long long)nw, (long long)work_cap);
@@ -545,6 +586,26 @@ int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, floa
   }
   return SVIMX_OK;
   SVIMX_TRY_END
+}
+
+int svimx_reinit(void* stream, float* phi, float* scratch, float* phi_copy, float* occupancy, int32_t* band,
+                 int64_t nb, int64_t inside_outside, int nx, int ny, int nz, double h, double width, int iterations,
+                 double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b, int32_t* work_idx,
+                 int32_t* work_info, float* geo, int64_t work_cap, void* ws, int64_t ws_bytes, int64_t* info) {
+  return reinit(stream, phi, scratch, phi_copy, occupancy, band, nb, inside_outside, nx, ny, nz, h, width,
+                iterations, cfl, dilation, margin, mask_a, mask_b, work_idx, work_info, geo, work_cap, ws, ws_bytes,
+                nullptr, info);
+}
+
+int svimx_reinit_gated(void* stream, float* phi, float* scratch, float* phi_copy, float* occupancy, int32_t* band,
+                       int64_t nb, int64_t inside_outside, int nx, int ny, int nz, double h, double width,
+                       int iterations, double cfl, int dilation, int margin, uint8_t* mask_a, uint8_t* mask_b,
+                       int32_t* work_idx, int32_t* work_info, float* geo, int64_t work_cap, void* ws,
+                       int64_t ws_bytes, const svimx_reinit_gate_t* gate, int64_t* info) {
+  SVIMX_REQUIRE(gate, SVIMX_EBADARG, "svimx_reinit_gated: null gate");
+  return reinit(stream, phi, scratch, phi_copy, occupancy, band, nb, inside_outside, nx, ny, nz, h, width,
+                iterations, cfl, dilation, margin, mask_a, mask_b, work_idx, work_info, geo, work_cap, ws, ws_bytes,
+                gate, info);
 }
 
 }  // extern "C"

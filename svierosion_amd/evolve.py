@@ -116,6 +116,8 @@ class ErosionSimulation:
         self.steps = 0
         self.history: list[StepRecord] = []
         self._displacement = 0.0
+        self._last_inc = 0.0        # front displacement dt max V of the last step
+        self._reinit_error = None   # raised by a queued reinitialisation, re-raised in order
         self.timers = {"stats_sync": 0.0, "volume_sync": 0.0}
         # name -> [(start, end) timing-event pairs]; None disables event recording
         self.kernel_events = None
@@ -227,24 +229,29 @@ class ErosionSimulation:
             _abi.check(_abi.lib.svimx_nr_sum(st, self.conv.plan, _abi.ptr(ls.occupancy), _abi.ptr(self.field),
                                              _abi.ptr(ls.ws), ls.ws.numel()), "N_r")
 
-    def _read_results(self, speculate: int | None = None) -> np.ndarray:
+    def _read_results(self, speculate: int | None = None, then=None):
         """Host copy of the result slots.  With ``speculate`` = the buffer whose
         fractions were just computed, that buffer's N_r (what the next step's
         stage 0 reads) is launched between the copy and the wait, so the device
-        keeps working while the host reads the results."""
+        keeps working while the host reads the results.  ``then``: work queued
+        between the copy and the wait instead; returns (results, its value)."""
         ls = self.ls
         if self._res_host is None:
             self._res_host = torch.empty(_abi.R_COUNT, dtype=torch.float64, pin_memory=True)
             self._res_event = torch.cuda.Event()
         self._res_host.copy_(_abi.results_view(ls.ws), non_blocking=True)
         self._res_event.record()
-        if speculate is not None:
+        out = None
+        if then is not None:
+            out = then()
+        elif speculate is not None:
             self._nr_inputs()
             self._nr_valid = speculate
         t0 = _time.perf_counter()
         self._res_event.synchronize()
         self.timers["volume_sync"] += _time.perf_counter() - t0
-        return self._res_host.numpy().copy()
+        r = self._res_host.numpy().copy()
+        return r if then is None else (r, out)
 
     def _prepare(self, buf: int, stats: bool) -> None:
         """Fractions, N_r / curvatures and velocity (into self.V) for the field in bufs[buf]."""
@@ -346,6 +353,61 @@ class ErosionSimulation:
         finally:
             ls.cur = saved
 
+    def _reinit_likely(self) -> bool:
+        """Predict that the step in flight ends in a reinitialisation: its front
+        displacement taken as 1.05 times the last step's reaches the threshold."""
+        h = self.ls.h
+        return self._displacement + 1.05 * self._last_inc >= self.reinit_distance * h
+
+    def _zero_curvatures_on_band(self) -> None:
+        # the dense curvature fields must read 0 off the band after a
+        # reinitialisation, like the reference's freshly zeroed array
+        # (svierosion/evolve.py:138); the next curvature pass rewrites the band
+        if self.k12 is not None:
+            self.k12.view(-1, 2).index_fill_(0, self.ls.band.long(), 0.0)
+
+    def _volume_and_gated_reinit(self, buf: int, target: float | None, tol: float):
+        """Volume of bufs[buf] with the reinitialisation of that field queued
+        behind it (svimx_reinit_gated): it runs iff the step triggers it, decided
+        on the device with the host's rule and operation order, so the host does
+        not wait for the results before starting it.  Returns (volume, results,
+        whether it ran)."""
+        ls = self.ls
+        io0 = ls.inside_outside
+        gate = _abi.ReinitGate(displacement=self._displacement, threshold=self.reinit_distance * ls.h,
+                               inside_outside=float(io0), h3=ls.h ** 3,
+                               target=0.0 if target is None else target,
+                               target_stop=0.0 if target is None else target * (1 + tol),
+                               has_target=0 if target is None else 1)
+        saved = ls.cur
+        ls.cur = buf
+        try:
+            with self._span("fractions"):
+                ls.fractions(self._plan(), self.cutoff[1])
+            self._frac_valid = buf
+
+            def reinit_behind():
+                self._zero_curvatures_on_band()
+                with self._span("reinit"):
+                    try:
+                        return ls.reinitialize(self.reinit_iterations, gate=gate)
+                    except Exception as e:  # raised where the host's rule runs it
+                        self._reinit_error = e
+                        return True
+
+            r, ran = self._read_results(then=reinit_behind)
+            # the step's volume, over the band it was computed on
+            return (io0 + float(r[_abi.R_VOLUME_SUM])) * ls.h ** 3, r, bool(ran)
+        finally:
+            ls.cur = saved
+
+    def _gate_agrees(self, ran: bool, decided: bool) -> None:
+        if ran and not decided:
+            if self._reinit_error is not None:
+                e, self._reinit_error = self._reinit_error, None
+                raise e
+            raise RuntimeError("the device-side reinitialisation trigger ran where the host's rule does not")
+
     def _no_reinit_after_step(self) -> bool:
         """True when the step in flight cannot trigger a reinitialisation: it moves
         the front by dt max V <= cfl h (dt <= cfl h / max V), so the displacement
@@ -431,6 +493,7 @@ class ErosionSimulation:
             t_wall = _time.time()
             a = ls.cur
             b, c = (a + 1) % 3, (a + 2) % 3
+            pre_reinit = False
             self._stage0(a)
             if self.fused and self.device_dt:
                 # dt on the device: the whole step runs without a host round trip
@@ -438,11 +501,17 @@ class ErosionSimulation:
                 self._device_dt(None if max_time is None else max_time - self.time)
                 self._stage0_update(a, b, _abi.DT_DEVICE)
                 self._rk3_tail(a, b, c, _abi.DT_DEVICE)
-                vnew, res = self._volume_of(b, speculate=self._no_reinit_after_step())
+                if self._reinit_likely():
+                    # queue the reinitialisation behind the step; the device runs
+                    # it only if the step triggers it (the host's rule, below)
+                    vnew, res, pre_reinit = self._volume_and_gated_reinit(b, target, tol)
+                else:
+                    vnew, res = self._volume_of(b, speculate=self._no_reinit_after_step())
                 vmax, info = self._stats_of(res)
                 dt = float(res[_abi.R_DT])
                 if not math.isfinite(dt) or dt <= 0:
                     reason = "stalled"  # the step's stages applied 0; u0 stays current
+                    self._gate_agrees(pre_reinit, False)
                     break
             else:
                 vmax, info = self._stats()
@@ -456,6 +525,7 @@ class ErosionSimulation:
                 self._rk3_tail(a, b, c, dt)
                 vnew, res = self._volume_of(b, speculate=self._no_reinit_after_step())
             if target is not None and vnew < target:
+                self._gate_agrees(pre_reinit, False)
                 lo_t, lo_v, hi_t, hi_v = 0.0, vol, dt, vnew
                 for _ in range(8):
                     if hi_v >= lo_v:
@@ -475,20 +545,25 @@ class ErosionSimulation:
             self.time += dt
             self.steps += 1
             self._displacement += dt * vmax
+            self._last_inc = dt * vmax
             did_reinit = False
             reached = target is not None and vnew <= target * (1 + tol)
             if self._displacement >= self.reinit_distance * ls.h and not reached:
-                if self.k12 is not None:
-                    # the dense curvature fields must read 0 off the band, like the
-                    # reference's freshly zeroed array (svierosion/evolve.py:138)
-                    self.k12.view(-1, 2).index_fill_(0, ls.band.long(), 0.0)
-                with self._span("reinit"):
-                    ls.reinitialize(self.reinit_iterations)
+                self._gate_agrees(pre_reinit, True)
+                if self._reinit_error is not None:
+                    e, self._reinit_error = self._reinit_error, None
+                    raise e
+                if not pre_reinit:
+                    self._zero_curvatures_on_band()
+                    with self._span("reinit"):
+                        ls.reinitialize(self.reinit_iterations)
                 self._alloc_band_arrays()
                 self._frac_valid = None
                 self._displacement = 0.0
                 did_reinit = True
                 vnew, res = self._volume_of(ls.cur, speculate=True)
+            else:
+                self._gate_agrees(pre_reinit, False)
             rec = StepRecord(self.steps, self.time, dt, vnew, vnew / V0, vmax, info.get("vmean", vmax),
                              info.get("qmin", float("nan")), info.get("qmax", float("nan")),
                              info.get("capped", 0), did_reinit, _time.time() - t_wall)

@@ -176,33 +176,37 @@ class LevelSet:
         return torch.minimum(p.clamp_min(1.0), hi)
 
     # --------------------------------------------------- reinitialisation
-    def reinitialize(self, iterations: int = 12, cfl: float = 0.45, dilation: int = 3) -> None:
+    def reinitialize(self, iterations: int = 12, cfl: float = 0.45, dilation: int = 3, gate=None) -> bool:
         """Min (2010) PDE reinitialisation with subcell fix, then band rebuild
-        (svierosion/levelset.py:154-183)."""
+        (svierosion/levelset.py:154-183).  ``gate`` (an ``_abi.ReinitGate``): run
+        only if the device-side trigger over the workspace's step results opens
+        (svimx_reinit_gated); returns whether it ran."""
         n = self.grid.size
-        if self._reinit_bufs is None:
-            self._reinit_bufs = (torch.empty(n, dtype=torch.uint8, device=self.device),
-                                 torch.empty(n, dtype=torch.uint8, device=self.device))
+        self.reserve_reinit()
         ma, mb = self._reinit_bufs
         others = [b for i, b in enumerate(self.bufs) if i != self.cur]
-        info = (C.c_int64 * 10)()
+        info = (C.c_int64 * 11)()
         cap = min(n, 2 * self.nb + 4096)
         for _ in range(2):
             widx, winfo, geo = self._work_buffers(cap)
-            rc = _abi.lib.svimx_reinit(_abi.stream(), _abi.ptr(self.flat), _abi.ptr(others[0]), _abi.ptr(others[1]),
-                                       _abi.ptr(self.occupancy), _abi.ptr(self._band), self.nb,
-                                       self.inside_outside, *self.dims, self.h, self.width, int(iterations),
-                                       float(cfl), int(dilation), MARGIN, _abi.ptr(ma), _abi.ptr(mb),
-                                       _abi.ptr(widx), _abi.ptr(winfo), _abi.ptr(geo), cap, _abi.ptr(self.ws),
-                                       self.ws.numel(), info)
+            args = (_abi.stream(), _abi.ptr(self.flat), _abi.ptr(others[0]), _abi.ptr(others[1]),
+                    _abi.ptr(self.occupancy), _abi.ptr(self._band), self.nb, self.inside_outside, *self.dims, self.h,
+                    self.width, int(iterations), float(cfl), int(dilation), MARGIN, _abi.ptr(ma), _abi.ptr(mb),
+                    _abi.ptr(widx), _abi.ptr(winfo), _abi.ptr(geo), cap, _abi.ptr(self.ws), self.ws.numel())
+            if gate is None:
+                rc = _abi.lib.svimx_reinit(*args, info)
+            else:
+                rc = _abi.lib.svimx_reinit_gated(*args, C.byref(gate), info)
             if rc != _abi.ECAPACITY:
                 break
             cap = int(info[9])  # the work set did not fit: retry with its exact size
-        if rc in (_abi.OK, _abi.EMARGIN):
+        ran = gate is None or int(info[10]) == 1
+        if ran and rc in (_abi.OK, _abi.EMARGIN):
             self.nb = int(info[0])
             self.inside_outside = int(info[1])
-        self.last_work_nodes = int(info[9])
+            self.last_work_nodes = int(info[9])
         _abi.check(rc, "reinitialisation")
+        return ran
 
     def reserve_reinit(self) -> None:
         """Allocate the reinitialisation's masks and work buffers now (for the

@@ -68,7 +68,7 @@ def test_ctypes_structs_match_the_header_layout(tmp_path):
         import pytest
         pytest.skip("no host C compiler")
     structs = {"svimx_law_t": _abi.Law, "svimx_job_desc_t": _abi.JobDesc, "svimx_step_record_t": _abi.StepRec,
-               "svimx_job_status_t": _abi.JobStatus}
+               "svimx_job_status_t": _abi.JobStatus, "svimx_reinit_gate_t": _abi.ReinitGate}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "svimx.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -213,31 +213,39 @@ def test_device_dt_matches_host_dt(device, law_name):
     """dt computed on the device (svimx_stable_dt, stages reading it there) against
     the host's stable_dt (svierosion/evolve.py:162-170) on the same fused path: a
     run to a target through reinitialisations and the secant stop, then a run that
-    ends on max_time (the clamp of :199-200), give identical histories and fields."""
+    ends on max_time (the clamp of :199-200), give identical histories and fields.
+    The device path queues a gated reinitialisation (svimx_reinit_gated) behind the
+    steps it predicts to need one; a third run queues it behind every step, so the
+    device-side trigger must stay closed on every step the host does not reinitialise."""
     from svierosion_amd.evolve import ErosionSimulation
     from svierosion_amd.flows import MeanCurvatureLaw, SVIPowerLaw
     out = []
-    for on_device in (True, False):
+    for mode in ("device", "host", "gate_every_step"):
         if law_name == "svi":
             ls, g, _ = _gpu_sphere(1.0, 0.05, 0.3 + 10 * 0.05)
             sim = ErosionSimulation(ls, SVIPowerLaw(0.3, 2, 1, 0.93))
         else:
             ls, g, _ = _gpu_sphere(1.0, 0.05, 0.6)
             sim = ErosionSimulation(ls, MeanCurvatureLaw(1.0))
-        sim.device_dt = on_device
+        sim.device_dt = mode != "host"
+        if mode == "gate_every_step":
+            sim._reinit_likely = lambda: True
         first = sim.run(target_fraction=0.75)
         t_end = sim.time + 2.5 * sim.history[-1].dt   # ends inside the third step
         second = sim.run(target_fraction=None, max_time=t_end)
         out.append((first, second, sim.history, ls.flat.clone(), ls.band.clone()))
-    (a1, a2, ha, fa, ba), (b1, b2, hb, fb, bb) = out
-    assert a1.stop_reason == b1.stop_reason == "target_volume" and any(r.reinit for r in ha)
-    assert a2.stop_reason == b2.stop_reason == "max_time" and a2.steps == b2.steps
     key = lambda h: [(r.step, r.time, r.dt, r.volume, r.vmax, r.vmean, r.qmin, r.qmax, r.capped, r.reinit)
                      for r in h]
-    assert str(key(ha)) == str(key(hb))       # str: NaN statistics compare equal
-    assert torch.equal(ba, bb) and torch.equal(fa, fb)
+    a1, a2, ha, fa, ba = out[0]
+    assert a1.stop_reason == "target_volume" and any(r.reinit for r in ha)
+    assert a2.stop_reason == "max_time"
+    for b1, b2, hb, fb, bb in out[1:]:
+        assert b1.stop_reason == a1.stop_reason and b2.stop_reason == a2.stop_reason and a2.steps == b2.steps
+        assert str(key(ha)) == str(key(hb))       # str: NaN statistics compare equal
+        assert torch.equal(ba, bb) and torch.equal(fa, fb)
     assert ha[-1].dt < ha[-2].dt              # the last step was clamped to max_time
-    print(f"{law_name}: {len(ha)} steps identical with dt on the device and on the host", flush=True)
+    print(f"{law_name}: {len(ha)} steps ({sum(r.reinit for r in ha)} reinitialisations) identical with dt "
+          f"on the device, on the host, and with the gated reinitialisation behind every step", flush=True)
 
 
 def test_closest_points_newton_steps_match_oracle(device):

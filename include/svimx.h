@@ -574,6 +574,12 @@ int svimx_engine_remove(void* engine, int job_id);
 /* Capture the main tick as a HIP graph and replay it (1) or launch eagerly (0).
  * Graphs are kept for the last few job sets a tick stepped. */
 int svimx_engine_use_graphs(void* engine, int on);
+/* Launch groups: a tick runs its jobs in groups whose band nodes add up to at
+ * most `nodes` (a larger job alone; 0: one group), each group through every
+ * stage before the next, so a large job's fields stay in the device's last-level
+ * cache from one kernel to the next.  Results do not depend on it.  Default 2M
+ * nodes.  No counterpart in the reference. */
+int svimx_engine_group_nodes(void* engine, int64_t nodes);
 /* Ids (up to cap) of jobs that stopped since the last call, oldest first. */
 int svimx_engine_stopped(void* engine, int* ids, int cap, int* n);
 /* Ticks whose graph capture was invalidated (by a device-wide operation in

@@ -50,6 +50,14 @@ namespace svimx {
 constexpr int kMaxEngineJobs = 256;
 constexpr int kEngGrid = 2048;  // persistent grid (multiple of the XCD count)
 constexpr int kEngGridWaves = 2560;  // one-wave blocks of the LDS N_r family: 10 per CU
+// A tick runs its jobs in groups whose band nodes add up to at most
+// group_nodes (a job above it runs alone), each group through every stage
+// before the next: the stages of a large job then reuse its fields in the
+// 256 MB Infinity Cache from one kernel to the next, as on the single-job
+// path, instead of evicting them with the other jobs' fields.  Small jobs
+// still share launches.
+constexpr int kMaxGroups = 8;
+constexpr int64_t kDefaultGroupNodes = 2000000;
 
 struct JobDev {
   float* buf[3];
@@ -388,7 +396,8 @@ struct Engine {
   double* h_res = nullptr;
   bool graphs = false;
   struct Captured {
-    std::vector<int> set;  // job slots the graph was captured for
+    std::vector<int> set;     // job slots the graph was captured for
+    std::vector<int> shape;   // their group sizes
     hipGraphExec_t exec;
     uint64_t last_use;
   };
@@ -403,6 +412,10 @@ struct Engine {
   double ms_tick = 0, ms_secant = 0, ms_reinit = 0;
   std::vector<int> set;      // job slots of the current phase
   std::vector<int> stopped;  // jobs that stopped since the last svimx_engine_stopped
+  int64_t group_nodes = kDefaultGroupNodes;
+  std::vector<int32_t> tab_sig;          // what the device tables hold (upload)
+  std::vector<std::vector<int>> groups;  // E.set split into launch groups (upload)
+  int cur_group = 0;                     // the group whose tables the launches use
 };
 
 static double now_ms() {
@@ -503,12 +516,26 @@ static void seg_add(Seg& s, int slot, int64_t blocks) {
   s.start[s.njobs] = s.total;
 }
 
-// Tables for the job slots in E.set; uploads the job records and tables.
+// Tables for the job slots in E.set, one set per launch group; uploads the job
+// records and tables.
 static int upload(Engine& E) {
-  Tables& T = *E.h_tab;
-  for (int t = 0; t < T_COUNT; ++t) seg_reset(T.seg[t]);
-  int top = 0;
+  E.groups.clear();
+  int64_t acc = 0;
   for (int slot : E.set) {
+    const int64_t nb = std::max<int64_t>(E.jobs[slot].nb, 1);
+    if (E.groups.empty() ||
+        (E.group_nodes > 0 && acc + nb > E.group_nodes && (int)E.groups.size() < kMaxGroups)) {
+      E.groups.emplace_back();
+      acc = 0;
+    }
+    E.groups.back().push_back(slot);
+    acc += nb;
+  }
+  int top = 0;
+  for (size_t g = 0; g < E.groups.size(); ++g) {
+  Tables& T = E.h_tab[g];
+  for (int t = 0; t < T_COUNT; ++t) seg_reset(T.seg[t]);
+  for (int slot : E.groups[g]) {
     EngJob& e = E.jobs[slot];
     fill_jobdev(e, E.h_jobs[slot]);
     top = std::max(top, slot + 1);
@@ -527,14 +554,44 @@ static int upload(Engine& E) {
       seg_add(T.seg[t], slot, nr_grid(e));
     }
   }
+  }
+  E.cur_group = 0;
   if (top > 0)
     SVIMX_CHECK_HIP(hipMemcpyAsync(E.d_jobs, E.h_jobs, sizeof(JobDev) * top, hipMemcpyHostToDevice, E.st));
-  SVIMX_CHECK_HIP(hipMemcpyAsync(E.d_tab, E.h_tab, sizeof(Tables), hipMemcpyHostToDevice, E.st));
+  // the tables change only with the set, the groups or a band size: upload
+  // them (ngroups x ~115 KB) only when their contents differ from the last
+  std::vector<int32_t> sig;
+  sig.push_back((int32_t)E.groups.size());
+  for (size_t g = 0; g < E.groups.size(); ++g)
+    for (int t = 0; t < T_COUNT; ++t) {
+      const Seg& sg = E.h_tab[g].seg[t];
+      sig.push_back(sg.njobs);
+      sig.push_back(sg.total);
+      sig.insert(sig.end(), sg.start + 1, sg.start + 1 + sg.njobs);
+      sig.insert(sig.end(), sg.slot, sg.slot + sg.njobs);
+    }
+  if (sig != E.tab_sig) {
+    SVIMX_CHECK_HIP(hipMemcpyAsync(E.d_tab, E.h_tab, sizeof(Tables) * std::max<size_t>(E.groups.size(), 1),
+                                   hipMemcpyHostToDevice, E.st));
+    E.tab_sig.swap(sig);
+  }
   return SVIMX_OK;
 }
 
+// Run f once per launch group, with that group's tables current.
+template <typename F>
+static int each_group(Engine& E, F f) {
+  int rc = SVIMX_OK;
+  for (int g = 0; g < (int)E.groups.size() && !rc; ++g) {
+    E.cur_group = g;
+    rc = f();
+  }
+  E.cur_group = 0;
+  return rc;
+}
+
 static unsigned grid_of(const Engine& E, int t) {
-  const Seg& sg = E.h_tab->seg[t];
+  const Seg& sg = E.h_tab[E.cur_group].seg[t];
   int64_t g;
   if (t == T_JOBS || t == T_RUNJOBS) {
     g = E.graphs ? E.max_jobs : sg.total;  // one block per job
@@ -576,9 +633,10 @@ static int64_t resident_blocks(K kern, int threads) {
 
 #define ENG_LAUNCH(kern, t, threads, ...)                                                                  \
   do {                                                                                                    \
-    if (E.h_tab->seg[t].njobs > 0 && E.h_tab->seg[t].total > 0) {                                         \
+    if (E.h_tab[E.cur_group].seg[t].njobs > 0 && E.h_tab[E.cur_group].seg[t].total > 0) {                 \
       const unsigned _g = (unsigned)std::min<int64_t>(grid_of(E, t), resident_blocks(kern, threads));     \
-      hipLaunchKernelGGL(kern, dim3(_g), dim3(threads), 0, E.st, E.d_jobs, &E.d_tab->seg[t], ##__VA_ARGS__); \
+      hipLaunchKernelGGL(kern, dim3(_g), dim3(threads), 0, E.st, E.d_jobs, &E.d_tab[E.cur_group].seg[t],   \
+                         ##__VA_ARGS__);                                                                  \
       SVIMX_CHECK_LAUNCH();                                                                               \
     }                                                                                                     \
   } while (0)
@@ -616,7 +674,7 @@ static int inputs(Engine& E, int rel, int only_needed) {
   ENG_NRY(16, 4, 4, 9); ENG_NRY(16, 8, 4, 10);
 #undef ENG_NRY
   // tile-mode plans (balls wider than 16 cells) run their own launches
-  for (int slot : E.set) {
+  for (int slot : E.groups[E.cur_group]) {
     EngJob& e = E.jobs[slot];
     if (!e.plan || e.run_plan()) continue;
     int rc = svimx_nr_select(E.st, e.plan);
@@ -672,11 +730,19 @@ static int tail_and_volume(Engine& E) {
 // stage 0 as velocity + L0 and statistics, then its update b = u0 + dt L0
 // (the stage-3 replay of rk_node)
 static int main_sequence(Engine& E) {
-  int rc = inputs(E, 0, 1);
-  if (!rc) rc = velocity_rk(E, 0, 0, 0);
-  if (!rc) rc = rk(E, 3, 0, 0, 1);
-  if (!rc) rc = tail_and_volume(E);
-  return rc;
+  return each_group(E, [&] {
+    int rc = inputs(E, 0, 1);
+    if (!rc) rc = velocity_rk(E, 0, 0, 0);
+    if (!rc) rc = rk(E, 3, 0, 0, 1);
+    if (!rc) rc = tail_and_volume(E);
+    return rc;
+  });
+}
+
+static std::vector<int> group_shape(const Engine& E) {
+  std::vector<int> shape;
+  for (const auto& g : E.groups) shape.push_back((int)g.size());
+  return shape;
 }
 
 static int readback(Engine& E) {
@@ -714,8 +780,9 @@ static void drop_graph(Engine& E, int slot = -1) {
 static int launch_main(Engine& E) {
   if (!E.graphs || tile_jobs_in(E)) return main_sequence(E);
   Engine::Captured* hit = nullptr;
+  const std::vector<int> shape = group_shape(E);
   for (auto& c : E.captured)
-    if (c.set == E.set) hit = &c;
+    if (c.set == E.set && c.shape == shape) hit = &c;
   if (!hit) {
     if ((int)E.captured.size() >= kGraphCache) {  // evict the least recently used
       auto lru = std::min_element(E.captured.begin(), E.captured.end(),
@@ -758,7 +825,7 @@ static int launch_main(Engine& E) {
     e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     SVIMX_CHECK_HIP(e);
-    E.captured.push_back(Engine::Captured{E.set, exec, 0});
+    E.captured.push_back(Engine::Captured{E.set, shape, exec, 0});
     hit = &E.captured.back();
   }
   hit->last_use = ++E.graph_clock;
@@ -1000,8 +1067,11 @@ static int tick(Engine& E, int* n_running) {
     }
     const bool saved = E.graphs;
     E.graphs = false;  // eager launches sized for the retry set
-    if (!rc) rc = rk(E, 3, 0, 0, 1);
-    if (!rc) rc = tail_and_volume(E);
+    if (!rc) rc = each_group(E, [&] {
+      int r = rk(E, 3, 0, 0, 1);
+      if (!r) r = tail_and_volume(E);
+      return r;
+    });
     E.graphs = saved;
     if (!rc) rc = readback(E);
     if (rc) {
@@ -1061,7 +1131,7 @@ static int tick(Engine& E, int* n_running) {
     rc = upload(E);
     const bool saved = E.graphs;
     E.graphs = false;
-    if (!rc) rc = volume(E, 0);
+    if (!rc) rc = each_group(E, [&] { return volume(E, 0); });
     E.graphs = saved;
     if (!rc) rc = readback(E);
     if (rc) {
@@ -1118,11 +1188,11 @@ int svimx_engine_create(void* stream, int max_jobs, void** engine) {
     E->own_stream = e == hipSuccess;
   }
   if (e == hipSuccess) e = hipMalloc(&E->d_jobs, sizeof(JobDev) * max_jobs);
-  if (e == hipSuccess) e = hipMalloc(&E->d_tab, sizeof(Tables));
+  if (e == hipSuccess) e = hipMalloc(&E->d_tab, sizeof(Tables) * kMaxGroups);
   if (e == hipSuccess) e = hipMalloc(&E->d_dts, sizeof(double) * max_jobs);
   if (e == hipSuccess) e = hipMalloc(&E->d_res, sizeof(double) * SVIMX_R_COUNT * max_jobs);
   if (e == hipSuccess) e = hipHostMalloc(&E->h_jobs, sizeof(JobDev) * max_jobs);
-  if (e == hipSuccess) e = hipHostMalloc(&E->h_tab, sizeof(Tables));
+  if (e == hipSuccess) e = hipHostMalloc(&E->h_tab, sizeof(Tables) * kMaxGroups);
   if (e == hipSuccess) e = hipHostMalloc(&E->h_dts, sizeof(double) * max_jobs);
   if (e == hipSuccess) e = hipHostMalloc(&E->h_res, sizeof(double) * SVIMX_R_COUNT * max_jobs);
   if (e != hipSuccess) {
@@ -1130,7 +1200,7 @@ int svimx_engine_create(void* stream, int max_jobs, void** engine) {
     svimx_engine_destroy(E);
     return SVIMX_EDEVICE;
   }
-  std::memset(E->h_tab, 0, sizeof(Tables));
+  std::memset(E->h_tab, 0, sizeof(Tables) * kMaxGroups);
   *engine = E;
   return SVIMX_OK;
   SVIMX_TRY_END
@@ -1326,6 +1396,14 @@ int svimx_engine_schedule(void* engine, int policy, int64_t budget_nodes, double
   E->policy = policy;
   E->budget_nodes = budget_nodes;
   E->slack = slack;
+  return SVIMX_OK;
+}
+
+int svimx_engine_group_nodes(void* engine, int64_t nodes) {
+  Engine* E = static_cast<Engine*>(engine);
+  SVIMX_REQUIRE(E, SVIMX_EBADARG, "null engine");
+  SVIMX_REQUIRE(nodes >= 0, SVIMX_EBADARG, "group_nodes must be >= 0");
+  E->group_nodes = nodes;
   return SVIMX_OK;
 }
 

@@ -103,6 +103,12 @@ class BatchEngine:
     def use_graphs(self, on: bool) -> None:
         _abi.check(_abi.lib.svimx_engine_use_graphs(self._h, int(bool(on))))
 
+    def group_nodes(self, nodes: int) -> None:
+        """Run each tick's jobs in launch groups of at most ``nodes`` band nodes (a larger
+        job alone; 0: one group), each group through every stage before the next, so a
+        large job's fields stay cached between its kernels.  Results do not depend on it."""
+        _abi.check(_abi.lib.svimx_engine_group_nodes(self._h, int(nodes)), "engine groups")
+
     def schedule(self, policy: str = "lockstep", budget_nodes: int = 0, slack: float = 0.0) -> None:
         """Which running jobs a tick steps: ``"lockstep"`` (all), or ``"fifo"`` (the order
         jobs were added) / ``"longest"`` (most estimated steps left first) filled while the

@@ -72,9 +72,11 @@ def _assert_same_run(sa, sb):
     assert sa.time == sb.time and sa.steps == sb.steps
 
 
-def _engine_run(names, graphs):
+def _engine_run(names, graphs, group_nodes=None):
     from svierosion_amd.engine import BatchEngine
     eng = BatchEngine(max_jobs=8, graphs=graphs)
+    if group_nodes is not None:
+        eng.group_nodes(group_nodes)
     jobs = {}
     for nm in names:
         sim, target, max_steps = _sim(nm)
@@ -129,6 +131,22 @@ def test_mix_equals_alone_with_graphs(device):
         alone = _engine_run([nm], graphs=False)[nm]
         assert mix[nm][1] == alone[1]
         _assert_same_run(mix[nm][0], alone[0])
+
+
+@pytest.mark.parametrize("group_nodes", [1, 40000])
+def test_launch_groups_are_result_neutral(device, group_nodes):
+    """A tick split into launch groups (every job alone, or groups of at most 40k
+    band nodes; svimx_engine_group_nodes) gives the same histories and fields as
+    one group, with graphs (captured per set and group shape) and through the
+    secant retries and reinitialisations."""
+    names = ["svi_cube", "mean", "svi_sphere_k2_y2", "gauss", "svi_fft"]
+    one = _engine_run(names, graphs=True, group_nodes=0)
+    split = _engine_run(names, graphs=True, group_nodes=group_nodes)
+    for nm in names:
+        assert split[nm][1] == one[nm][1]
+        _assert_same_run(split[nm][0], one[nm][0])
+    sizes = {nm: one[nm][0].ls.nb for nm in names}
+    print(f"groups of <= {group_nodes} band nodes: identical to one group; band sizes {sizes}", flush=True)
 
 
 def test_engine_rerun_with_lower_target_continues(device):

@@ -410,8 +410,8 @@ def main(argv=None):
     ap.add_argument("--progress", action="store_true", help="print each completed job of the batched legs (stderr)")
     ap.add_argument("--records", default=None, metavar="PATH",
                     help="append one JSON line per job of the batched and served legs to PATH")
-    ap.add_argument("--batch-schedule", choices=["lockstep", "longest", "fifo"], default="lockstep")
-    ap.add_argument("--serve-schedule", choices=["fifo", "longest", "lockstep"], default="fifo")
+    ap.add_argument("--batch-schedule", choices=["lockstep", "longest", "fifo", "short"], default="lockstep")
+    ap.add_argument("--serve-schedule", choices=["fifo", "longest", "lockstep", "short"], default="short")
     ap.add_argument("--serve-part", default=None, metavar="served|fifo:A:B",
                     help="run one part of a long served measurement and exit (records to --records)")
     ap.add_argument("--summarize-records", nargs="+", default=None, metavar="PATH",

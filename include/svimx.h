@@ -555,17 +555,26 @@ int svimx_engine_tick(void* engine, int* n_running);
  *   SVIMX_SCHED_LONGEST:  ... or by most estimated steps left first (from each
  *                         job's volume loss per step since its run started;
  *                         jobs with fewer than 4 steps come first),
- * taken while the selected band nodes stay within max(budget_nodes,
- * (1 + slack) * band nodes of the first job): the first job is always
- * stepped, a job that does not fit is passed over for this tick.  A job's
- * results do not depend on the schedule. */
-enum { SVIMX_SCHED_LOCKSTEP = 0, SVIMX_SCHED_FIFO = 1, SVIMX_SCHED_LONGEST = 2 };
+ *   SVIMX_SCHED_SHORT:    in the order they were added, where behind the first
+ *                         (oldest) job every job with an estimated work left
+ *                         (steps left x band nodes) of at most short_work
+ *                         node-steps, or fewer than 4 steps into its run,
+ *                         joins, whatever its band size, and no other: long
+ *                         jobs run one after another as on a one-at-a-time
+ *                         queue while short ones pass them.
+ * Except under SHORT, jobs are taken while the selected band nodes stay within
+ * max(budget_nodes, (1 + slack) * band nodes of the first job): the first job
+ * is always stepped, a job that does not fit is passed over for this tick.  A
+ * job's results do not depend on the schedule. */
+enum { SVIMX_SCHED_LOCKSTEP = 0, SVIMX_SCHED_FIFO = 1, SVIMX_SCHED_LONGEST = 2, SVIMX_SCHED_SHORT = 3 };
 /* Up to max_ticks ticks in one call, ending early after the tick in which a
  * job stopped or when none is running; *n_ticks = ticks taken.  For request
  * loops that look at arrivals and stopped jobs every few ticks (no
  * counterpart in the reference). */
 int svimx_engine_ticks(void* engine, int max_ticks, int* n_running, int* n_ticks);
 int svimx_engine_schedule(void* engine, int policy, int64_t budget_nodes, double slack);
+/* The work bound of SVIMX_SCHED_SHORT, in node-steps (default 2e10). */
+int svimx_engine_short_work(void* engine, double node_steps);
 int svimx_engine_status(void* engine, int job_id, svimx_job_status_t* out);
 /* Copy step records [first, first + n) of job_id (records accumulate over runs). */
 int svimx_engine_records(void* engine, int job_id, int64_t first, int64_t n, svimx_step_record_t* out);

@@ -145,6 +145,7 @@ _SIGNATURES = {
     "svimx_engine_remove": (_I, [_P, _I]),
     "svimx_engine_use_graphs": (_I, [_P, _I]),
     "svimx_engine_group_nodes": (_I, [_P, _I64]),
+    "svimx_engine_short_work": (_I, [_P, _D]),
     "svimx_engine_schedule": (_I, [_P, _I, _I64, _D]),
     "svimx_engine_stopped": (_I, [_P, C.POINTER(C.c_int), _I, C.POINTER(C.c_int)]),
     "svimx_engine_timers": (_I, [_P, C.POINTER(_D)]),

@@ -13,12 +13,16 @@ the batch engine (svierosion_amd.engine):
   while the main thread keeps ticking the engine;
 * stepping -- a tick advances the running jobs its schedule selects by one
   RK3 step each; jobs join and leave between ticks, so new requests start
-  while others are running.  ``schedule="fifo"`` (the default of
-  :func:`serve`) takes jobs in arrival order and fills each tick with later
-  ones while the selected band nodes stay within ``budget_nodes`` (or
-  ``1 + slack`` times the first job's band): the oldest request runs at close
-  to its own speed and small requests run beside it instead of waiting.
-  ``"longest"`` puts the jobs with the most estimated steps left first;
+  while others are running.  ``schedule="short"`` (the default of
+  :func:`serve`) steps the oldest request in every tick and lets every
+  request with little estimated work left (steps left x band nodes at most
+  2e10 node-steps, or fewer than 4 steps in) run beside it: short requests
+  no longer queue behind long ones, and long ones still run one after another
+  in arrival order, as on a one-at-a-time queue, instead of sharing the GPU
+  (which made early long requests finish last).  ``"fifo"`` takes jobs in
+  arrival order and fills each tick with later ones while the selected band
+  nodes stay within ``budget_nodes`` (or ``1 + slack`` times the first job's
+  band).  ``"longest"`` puts the jobs with the most estimated steps left first;
   ``"lockstep"`` (the default of :func:`erode_batch`) steps every job in
   every tick, which finishes a batch soonest when one job needs far more
   steps than the rest: the others run beside it at full concurrency and the
@@ -169,7 +173,7 @@ DEFAULT_SLACK = 0.5
 
 def serve(requests: list, max_jobs: int = 64, hbm_budget: float | None = None, graphs: bool = True,
           keep_state: bool = False, verbose: bool = False, workers: int = 2, solo_tail: bool = True,
-          schedule: str = "fifo", budget_nodes: int = DEFAULT_BUDGET_NODES, slack: float = DEFAULT_SLACK,
+          schedule: str = "short", budget_nodes: int = DEFAULT_BUDGET_NODES, slack: float = DEFAULT_SLACK,
           ticks_per_visit: int = 8) -> ServeReport:
     """Continuous batching of ``requests`` (arrival times in seconds from now).  With
     ``solo_tail`` the last running job finishes on the single-job path once no

@@ -414,6 +414,7 @@ struct Engine {
   std::vector<int> stopped;  // jobs that stopped since the last svimx_engine_stopped
   int64_t group_nodes = kDefaultGroupNodes;
   std::vector<int32_t> tab_sig;          // what the device tables hold (upload)
+  double short_work = 2e10;              // SVIMX_SCHED_SHORT: riders' estimated node-steps left
   std::vector<std::vector<int>> groups;  // E.set split into launch groups (upload)
   int cur_group = 0;                     // the group whose tables the launches use
 };
@@ -972,15 +973,22 @@ static void select_jobs(Engine& E, const std::vector<int>& cand) {
   order.reserve(cand.size());
   for (int slot : cand) {
     const EngJob& e = E.jobs[slot];
-    order.emplace_back(E.policy == SVIMX_SCHED_FIFO ? (double)e.seq : -steps_left(e), slot);
+    order.emplace_back(E.policy == SVIMX_SCHED_LONGEST ? -steps_left(e) : (double)e.seq, slot);
   }
   std::stable_sort(order.begin(), order.end(),
                    [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first < b.first; });
   const double cap = std::max((double)E.budget_nodes, (1.0 + E.slack) * (double)E.jobs[order[0].second].nb);
   double used = 0;
   for (const auto& o : order) {
-    const double nb = (double)E.jobs[o.second].nb;
-    if (E.set.empty() || used + nb <= cap) {
+    const EngJob& e = E.jobs[o.second];
+    const double nb = (double)e.nb;
+    // SHORT: behind the oldest job only jobs with little work left (or too few
+    // steps in this run to tell) ride along, whatever their band size (they add
+    // a few ticks' work each); long jobs wait for their turn
+    bool joins = used + nb <= cap;
+    if (E.policy == SVIMX_SCHED_SHORT)
+      joins = e.steps - e.n_start < 4 || steps_left(e) * std::max(nb, 1.0) <= E.short_work;
+    if (E.set.empty() || joins) {
       E.set.push_back(o.second);
       used += nb;
     }
@@ -1389,13 +1397,21 @@ int svimx_engine_remove(void* engine, int id) {
 int svimx_engine_schedule(void* engine, int policy, int64_t budget_nodes, double slack) {
   Engine* E = static_cast<Engine*>(engine);
   SVIMX_REQUIRE(E, SVIMX_EBADARG, "null engine");
-  SVIMX_REQUIRE(policy >= SVIMX_SCHED_LOCKSTEP && policy <= SVIMX_SCHED_LONGEST, SVIMX_EBADARG,
+  SVIMX_REQUIRE(policy >= SVIMX_SCHED_LOCKSTEP && policy <= SVIMX_SCHED_SHORT, SVIMX_EBADARG,
                 "unknown schedule %d", policy);
   SVIMX_REQUIRE(budget_nodes >= 0 && slack >= 0 && std::isfinite(slack), SVIMX_EBADARG,
                 "budget_nodes and slack must be >= 0");
   E->policy = policy;
   E->budget_nodes = budget_nodes;
   E->slack = slack;
+  return SVIMX_OK;
+}
+
+int svimx_engine_short_work(void* engine, double node_steps) {
+  Engine* E = static_cast<Engine*>(engine);
+  SVIMX_REQUIRE(E, SVIMX_EBADARG, "null engine");
+  SVIMX_REQUIRE(node_steps >= 0 && std::isfinite(node_steps), SVIMX_EBADARG, "short_work must be finite and >= 0");
+  E->short_work = node_steps;
   return SVIMX_OK;
 }
 

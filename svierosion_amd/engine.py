@@ -29,7 +29,7 @@ from .levelset import MARGIN
 
 STOP_NAMES = {0: "running", 1: "target_volume", 2: "max_time", 3: "vanished", 4: "stalled", 5: "max_steps",
               6: "error"}
-SCHEDULES = {"lockstep": 0, "fifo": 1, "longest": 2}
+SCHEDULES = {"lockstep": 0, "fifo": 1, "longest": 2, "short": 3}
 
 
 JobDesc, StepRec, JobStatus = _abi.JobDesc, _abi.StepRec, _abi.JobStatus
@@ -109,12 +109,18 @@ class BatchEngine:
         large job's fields stay cached between its kernels.  Results do not depend on it."""
         _abi.check(_abi.lib.svimx_engine_group_nodes(self._h, int(nodes)), "engine groups")
 
-    def schedule(self, policy: str = "lockstep", budget_nodes: int = 0, slack: float = 0.0) -> None:
+    def schedule(self, policy: str = "lockstep", budget_nodes: int = 0, slack: float = 0.0,
+                 short_work: float | None = None) -> None:
         """Which running jobs a tick steps: ``"lockstep"`` (all), or ``"fifo"`` (the order
         jobs were added) / ``"longest"`` (most estimated steps left first) filled while the
-        selected band nodes stay within max(budget_nodes, (1 + slack) x the first job's)."""
+        selected band nodes stay within max(budget_nodes, (1 + slack) x the first job's);
+        ``"short"``: the oldest job, joined by every job with at most ``short_work``
+        estimated node-steps left, whatever its band (long jobs take their turn one
+        after another)."""
         _abi.check(_abi.lib.svimx_engine_schedule(self._h, SCHEDULES[policy], int(budget_nodes), float(slack)),
                    "engine schedule")
+        if short_work is not None:
+            _abi.check(_abi.lib.svimx_engine_short_work(self._h, float(short_work)), "engine schedule")
 
     # ----------------------------------------------------------------- jobs
     def add(self, sim: ErosionSimulation) -> int:

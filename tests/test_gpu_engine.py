@@ -72,11 +72,13 @@ def _assert_same_run(sa, sb):
     assert sa.time == sb.time and sa.steps == sb.steps
 
 
-def _engine_run(names, graphs, group_nodes=None):
+def _engine_run(names, graphs, group_nodes=None, schedule=None):
     from svierosion_amd.engine import BatchEngine
     eng = BatchEngine(max_jobs=8, graphs=graphs)
     if group_nodes is not None:
         eng.group_nodes(group_nodes)
+    if schedule is not None:
+        eng.schedule(*schedule)
     jobs = {}
     for nm in names:
         sim, target, max_steps = _sim(nm)
@@ -147,6 +149,19 @@ def test_launch_groups_are_result_neutral(device, group_nodes):
         _assert_same_run(split[nm][0], one[nm][0])
     sizes = {nm: one[nm][0].ls.nb for nm in names}
     print(f"groups of <= {group_nodes} band nodes: identical to one group; band sizes {sizes}", flush=True)
+
+
+def test_short_schedule_serialises_long_jobs(device):
+    """SVIMX_SCHED_SHORT with a zero work bound: behind the oldest job only jobs
+    fewer than 4 steps into their run join a tick, so after their first steps the
+    jobs run one after another in the order they were added; every history and
+    field equals the lockstep run."""
+    names = ["svi_cube", "mean", "svi_sphere_k2_y2", "gauss"]
+    one = _engine_run(names, graphs=True, schedule=("lockstep",))
+    short = _engine_run(names, graphs=True, schedule=("short", 10**9, 0.0, 0.0))
+    for nm in names:
+        assert short[nm][1] == one[nm][1]
+        _assert_same_run(short[nm][0], one[nm][0])
 
 
 def test_engine_rerun_with_lower_target_continues(device):

@@ -47,7 +47,7 @@ def test_served_mix_equals_sequential(device):
     assert mix.latency(0.99) >= mix.latency(0.5) > 0
 
 
-@pytest.mark.parametrize("schedule", ["lockstep", "fifo", "longest"])
+@pytest.mark.parametrize("schedule", ["lockstep", "fifo", "longest", "short"])
 def test_every_schedule_reproduces_sequential_erode(device, schedule):
     """A tight band-node budget (each tick holds the first job and whatever fits
     beside it in 20,000 band nodes) makes the priority schedules pass jobs over

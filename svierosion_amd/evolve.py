@@ -96,7 +96,7 @@ class ErosionSimulation:
             self.k12 = torch.zeros(2 * n, dtype=torch.float32, device=dev)
         self._alloc_band_arrays()
         self._fv = None             # buffer index whose fractions/marks are current
-        self._nr_valid = None       # buffer whose N_r is in self.field (None, or _frac_valid)
+        self._nr_valid = None       # buffer whose N_r (or curvature pairs) are current (None, or _frac_valid)
         self._res_host = None       # pinned landing area of the result slots
         self._res_event = None
         # run(): the velocity pass fused with the RK stage that consumes it
@@ -211,10 +211,9 @@ class ErosionSimulation:
             if self.conv is not None:
                 if self._nr_valid != buf:
                     self._nr_inputs()
-            else:
-                with self._span("curvatures"):
-                    _abi.check(_abi.lib.svimx_curvatures(st, _abi.ptr(ls.flat), _abi.ptr(ls._band), ls.nb, *ls.dims,
-                                                         ls.h, _abi.ptr(self.k12)))
+            elif self._nr_valid != buf:
+                self._curvature_inputs()
+                self._nr_valid = None  # the pairs no longer belong to a speculated buffer
         finally:
             ls.cur = saved
 
@@ -228,6 +227,13 @@ class ErosionSimulation:
         with self._span("nr_sum"):
             _abi.check(_abi.lib.svimx_nr_sum(st, self.conv.plan, _abi.ptr(ls.occupancy), _abi.ptr(self.field),
                                              _abi.ptr(ls.ws), ls.ws.numel()), "N_r")
+
+    def _curvature_inputs(self) -> None:
+        """The dense (k1, k2) pairs of the current field at the band nodes."""
+        ls = self.ls
+        with self._span("curvatures"):
+            _abi.check(_abi.lib.svimx_curvatures(_abi.stream(), _abi.ptr(ls.flat), _abi.ptr(ls._band), ls.nb,
+                                                 *ls.dims, ls.h, _abi.ptr(self.k12)))
 
     def _read_results(self, speculate: int | None = None, then=None):
         """Host copy of the result slots.  With ``speculate`` = the buffer whose
@@ -245,7 +251,15 @@ class ErosionSimulation:
         if then is not None:
             out = then()
         elif speculate is not None:
-            self._nr_inputs()
+            if self.conv is not None:
+                self._nr_inputs()
+            else:
+                saved = ls.cur
+                ls.cur = speculate
+                try:
+                    self._curvature_inputs()
+                finally:
+                    ls.cur = saved
             self._nr_valid = speculate
         t0 = _time.perf_counter()
         self._res_event.synchronize()
@@ -347,7 +361,7 @@ class ErosionSimulation:
             with self._span("fractions"):
                 ls.fractions(self._plan(), self.cutoff[1])
             self._frac_valid = buf
-            spec = buf if speculate and self.conv is not None and self.newton_steps == 1 else None
+            spec = buf if speculate and self.newton_steps == 1 else None
             r = self._read_results(spec)
             return ls.volume_from_results(r), r
         finally:

@@ -394,6 +394,7 @@ struct Engine {
   Tables* h_tab = nullptr;
   double* h_dts = nullptr;
   double* h_res = nullptr;
+  double* res_map = nullptr;  // h_res as the device addresses it (the gather writes there)
   bool graphs = false;
   struct Captured {
     std::vector<int> set;     // job slots the graph was captured for
@@ -517,9 +518,24 @@ static void seg_add(Seg& s, int slot, int64_t blocks) {
   s.start[s.njobs] = s.total;
 }
 
+static int top_slot(const Engine& E) {
+  int top = 0;
+  for (int slot : E.set) top = std::max(top, slot + 1);
+  return top;
+}
+
+// The job records of E.set to the device (inside the captured graph on the
+// tick path, so a replay needs no separate copy).
+static int upload_jobs(Engine& E) {
+  const int top = top_slot(E);
+  if (top > 0)
+    SVIMX_CHECK_HIP(hipMemcpyAsync(E.d_jobs, E.h_jobs, sizeof(JobDev) * top, hipMemcpyHostToDevice, E.st));
+  return SVIMX_OK;
+}
+
 // Tables for the job slots in E.set, one set per launch group; uploads the job
-// records and tables.
-static int upload(Engine& E) {
+// records (unless the caller's launches do) and the tables.
+static int upload(Engine& E, bool jobs = true) {
   E.groups.clear();
   int64_t acc = 0;
   for (int slot : E.set) {
@@ -557,8 +573,11 @@ static int upload(Engine& E) {
   }
   }
   E.cur_group = 0;
-  if (top > 0)
-    SVIMX_CHECK_HIP(hipMemcpyAsync(E.d_jobs, E.h_jobs, sizeof(JobDev) * top, hipMemcpyHostToDevice, E.st));
+  (void)top;
+  if (jobs) {
+    const int rc = upload_jobs(E);
+    if (rc) return rc;
+  }
   // the tables change only with the set, the groups or a band size: upload
   // them (ngroups x ~115 KB) only when their contents differ from the last
   std::vector<int32_t> sig;
@@ -708,7 +727,7 @@ static int volume(Engine& E, int rel) {
   ENG_LAUNCH(k_eng_clear_marks, T_MARKS, 256, 0);
   ENG_LAUNCH(k_eng_fractions, T_ALL, kBandThreads, rel, 0);
   ENG_LAUNCH(k_eng_finalize_volume, T_JOBS, kFinalizeThreads);
-  ENG_LAUNCH(k_eng_gather, T_JOBS, 256, E.d_res);
+  ENG_LAUNCH(k_eng_gather, T_JOBS, 256, E.res_map);
   return SVIMX_OK;
 }
 
@@ -746,12 +765,9 @@ static std::vector<int> group_shape(const Engine& E) {
   return shape;
 }
 
+// The gather kernel wrote the results into host memory (res_map): waiting
+// for the stream is the whole readback.
 static int readback(Engine& E) {
-  int top = 0;
-  for (int slot : E.set) top = std::max(top, slot + 1);
-  if (top > 0)
-    SVIMX_CHECK_HIP(hipMemcpyAsync(E.h_res, E.d_res, sizeof(double) * SVIMX_R_COUNT * top, hipMemcpyDeviceToHost,
-                                   E.st));
   SVIMX_CHECK_HIP(hipStreamSynchronize(E.st));
   return SVIMX_OK;
 }
@@ -778,8 +794,14 @@ static void drop_graph(Engine& E, int slot = -1) {
 
 // The main tick's launches for the current set (captured once per set when
 // graphs are on: the tables it reads are refreshed before every replay).
+static int jobs_and_main_sequence(Engine& E) {
+  int rc = upload_jobs(E);
+  if (!rc) rc = main_sequence(E);
+  return rc;
+}
+
 static int launch_main(Engine& E) {
-  if (!E.graphs || tile_jobs_in(E)) return main_sequence(E);
+  if (!E.graphs || tile_jobs_in(E)) return jobs_and_main_sequence(E);
   Engine::Captured* hit = nullptr;
   const std::vector<int> shape = group_shape(E);
   for (auto& c : E.captured)
@@ -796,7 +818,7 @@ static int launch_main(Engine& E) {
     // Relaxed: other threads of the process (set-up and meshing workers) keep
     // allocating and freeing while this thread captures.
     SVIMX_CHECK_HIP(hipStreamBeginCapture(E.st, hipStreamCaptureModeRelaxed));
-    int rc = main_sequence(E);
+    int rc = jobs_and_main_sequence(E);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(E.st, &g);
     if (rc || e != hipSuccess || !g) {
@@ -820,7 +842,7 @@ static int launch_main(Engine& E) {
       }
       SVIMX_REQUIRE(cs == hipStreamCaptureStatusNone, SVIMX_EDEVICE, "engine stream left in capture state");
       ++E.capture_fallbacks;
-      return main_sequence(E);
+      return jobs_and_main_sequence(E);
     }
     hipGraphExec_t exec = nullptr;
     e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
@@ -1012,8 +1034,8 @@ static int tick(Engine& E, int* n_running) {
     return SVIMX_OK;
   }
   select_jobs(E, cand);
-  // one step of every job in the set
-  int rc = upload(E);
+  // one step of every job in the set (launch_main uploads the job records)
+  int rc = upload(E, false);
   if (!rc) rc = launch_main(E);
   if (!rc) rc = readback(E);
   if (rc) {
@@ -1202,7 +1224,8 @@ int svimx_engine_create(void* stream, int max_jobs, void** engine) {
   if (e == hipSuccess) e = hipHostMalloc(&E->h_jobs, sizeof(JobDev) * max_jobs);
   if (e == hipSuccess) e = hipHostMalloc(&E->h_tab, sizeof(Tables) * kMaxGroups);
   if (e == hipSuccess) e = hipHostMalloc(&E->h_dts, sizeof(double) * max_jobs);
-  if (e == hipSuccess) e = hipHostMalloc(&E->h_res, sizeof(double) * SVIMX_R_COUNT * max_jobs);
+  if (e == hipSuccess) e = hipHostMalloc(&E->h_res, sizeof(double) * SVIMX_R_COUNT * max_jobs, hipHostMallocMapped);
+  if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&E->res_map), E->h_res, 0);
   if (e != hipSuccess) {
     set_error("engine allocation failed: %s", hipGetErrorString(e));
     svimx_engine_destroy(E);

@@ -548,14 +548,12 @@ static int upload(Engine& E, bool jobs = true) {
     E.groups.back().push_back(slot);
     acc += nb;
   }
-  int top = 0;
   for (size_t g = 0; g < E.groups.size(); ++g) {
   Tables& T = E.h_tab[g];
   for (int t = 0; t < T_COUNT; ++t) seg_reset(T.seg[t]);
   for (int slot : E.groups[g]) {
     EngJob& e = E.jobs[slot];
     fill_jobdev(e, E.h_jobs[slot]);
-    top = std::max(top, slot + 1);
     seg_add(T.seg[T_ALL], slot, reduce_grid(e.nb));
     seg_add(T.seg[T_JOBS], slot, 1);
     if (e.svi()) seg_add(T.seg[T_SVI], slot, reduce_grid(e.nb));
@@ -573,7 +571,6 @@ static int upload(Engine& E, bool jobs = true) {
   }
   }
   E.cur_group = 0;
-  (void)top;
   if (jobs) {
     const int rc = upload_jobs(E);
     if (rc) return rc;
@@ -1085,8 +1082,7 @@ static int tick(Engine& E, int* n_running) {
       E.set.push_back(slot);
     }
     if (E.set.empty()) break;
-    int top = 0;
-    for (int slot : E.set) top = std::max(top, slot + 1);
+    const int top = top_slot(E);
     rc = upload(E);
     if (!rc) {
       hipError_t he = hipMemcpyAsync(E.d_dts, E.h_dts, sizeof(double) * top, hipMemcpyHostToDevice, E.st);

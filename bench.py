@@ -462,10 +462,12 @@ def main(argv=None):
     sim.run(target_fraction=None, max_steps=args.warmup)
     torch.cuda.synchronize()
     if not args.no_events:
-        # the timed steps record HIP events around the N_r launches only (the
-        # roofline's kernel); every stage's events cost host time per launch
+        # the timed steps record HIP events around every 4th N_r launch only
+        # (the roofline's kernel, sampled over the whole timed region): each
+        # bracketed launch costs ~5 us of device gap per event record
         sim.kernel_events = {}
         sim.kernel_event_names = {"nr_sum"}
+        sim.kernel_event_every = 4
     h0 = len(sim.history)
     t0 = time.perf_counter()
     sim.run(target_fraction=None, max_steps=args.steps)
@@ -478,6 +480,7 @@ def main(argv=None):
     timed = {k: float(np.sum(v)) for k, v in sim.kernel_times_ms().items()} if not args.no_events else {}
     launches = {k: len(v) for k, v in (sim.kernel_events or {}).items()}
     sim.kernel_events = None
+    sim.kernel_event_every = 1
     # N_r outputs and reads of the field the timed steps ended on
     n_out, n_in, tiles = nr_read_bytes(sim)
     stage_ms = {}

@@ -123,6 +123,8 @@ class ErosionSimulation:
         self.kernel_events = None
         # stage names to record (None: every stage); each span costs two event records
         self.kernel_event_names = None
+        self.kernel_event_every = 1  # record every n-th span of a name
+        self._span_calls = {}
 
     @property
     def _frac_valid(self):
@@ -136,8 +138,14 @@ class ErosionSimulation:
 
     @contextlib.contextmanager
     def _span(self, name: str):
-        """Bracket the launches of one stage with HIP events on the current stream."""
+        """Bracket the launches of one stage with HIP events on the current stream
+        (every kernel_event_every-th span of each name)."""
         if self.kernel_events is None or (self.kernel_event_names is not None and name not in self.kernel_event_names):
+            yield
+            return
+        k = self._span_calls.get(name, 0)
+        self._span_calls[name] = k + 1
+        if k % max(int(self.kernel_event_every), 1):
             yield
             return
         a = _abi.TimingEvent()

@@ -161,7 +161,9 @@ class _Worker:
         self.pool.shutdown(wait=True)
 
 
-# Tick budget of the served schedule, from a sweep over 12 Poisson requests of
+# Tick budget of the `fifo` and `longest` schedules (`short`, the serving
+# default, lets its short jobs join whatever their band), from a sweep of
+# `fifo` over 12 Poisson requests of
 # the served workload at 3.08/s (one-at-a-time FIFO: p50 1.19 s, p99 2.43 s,
 # 8,750 erosions/hour): 2M band nodes / slack 0.5 gave p50 1.23 s, p99 2.01 s,
 # 9,930 erosions/hour; slack 0 or a 1M budget lowered p50 to 1.20-1.24 s and

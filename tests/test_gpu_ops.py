@@ -450,3 +450,69 @@ def test_cube_res64_nr_closed_forms_and_oracle(device):
         print(f"cube N_r at {len(P)} {name} points: discretisation |N - {exact}| <= {disc:.2e} "
               f"(bound {CUBE_NR_BOUNDS[name]:.0e}), GPU vs oracle {arith:.2e}")
         assert disc < CUBE_NR_BOUNDS[name] and arith < 2e-5
+
+
+def test_reinit_capacity_retry_and_gate(device):
+    """svimx_reinit with a work-set capacity below the work set returns
+    SVIMX_ECAPACITY with the needed size and changes neither phi nor the band;
+    the retry at that size equals LevelSet.reinitialize.  svimx_reinit_gated with
+    a closed gate changes nothing (info[10] = 0, info[0] = nb); with an open gate
+    it equals the plain reinitialisation, also through a capacity retry."""
+    import ctypes as C
+    from svierosion_amd import _abi
+    from svierosion_amd.grid import Grid
+    from svierosion_amd.levelset import MARGIN, LevelSet
+    phi, origin, h, shape = sphere_phi(1.0, 0.05, 0.6, distort=True)
+
+    def fresh():
+        return LevelSet(torch.from_numpy(phi.astype(np.float32)).cuda(), Grid(origin, h, shape), width=6.0)
+
+    ref = fresh()
+    ref.reinitialize(20)
+    torch.cuda.synchronize()
+
+    def call(ls, cap, gate=None):
+        ls.reserve_reinit()
+        ma, mb = ls._reinit_bufs
+        widx, winfo, geo = ls._work_buffers(max(cap, 1))
+        others = [b for i, b in enumerate(ls.bufs) if i != ls.cur]
+        info = (C.c_int64 * 11)()
+        args = (_abi.stream(), _abi.ptr(ls.flat), _abi.ptr(others[0]), _abi.ptr(others[1]), _abi.ptr(ls.occupancy),
+                _abi.ptr(ls._band), ls.nb, ls.inside_outside, *ls.dims, ls.h, ls.width, 20, 0.45, 3, MARGIN,
+                _abi.ptr(ma), _abi.ptr(mb), _abi.ptr(widx), _abi.ptr(winfo), _abi.ptr(geo), cap, _abi.ptr(ls.ws),
+                ls.ws.numel())
+        if gate is None:
+            rc = _abi.lib.svimx_reinit(*args, info)
+        else:
+            rc = _abi.lib.svimx_reinit_gated(*args, C.byref(gate), info)
+        torch.cuda.synchronize()
+        return rc, info
+
+    def gate_for(ls, open_):
+        r = _abi.results_view(ls.ws)
+        r[_abi.R_DT], r[_abi.R_VMAX], r[_abi.R_VOLUME_SUM] = 1.0, 1.0, 1e6
+        return _abi.ReinitGate(displacement=0.0, threshold=0.5 if open_ else 1e9, inside_outside=0.0, h3=1.0,
+                               target=0.0, target_stop=0.0, has_target=0)
+
+    for gated in (False, True):
+        ls = fresh()
+        phi0, band0, nb0 = ls.flat.clone(), ls.band.clone(), ls.nb
+        gate = gate_for(ls, True) if gated else None
+        rc, info = call(ls, 1000, gate)
+        assert rc == _abi.ECAPACITY and info[9] > 1000
+        assert torch.equal(ls.flat, phi0) and ls.nb == nb0 and torch.equal(ls.band, band0)
+        need = int(info[9])
+        rc, info = call(ls, need, gate)
+        assert rc == _abi.OK and (not gated or info[10] == 1)
+        ls.nb, ls.inside_outside = int(info[0]), int(info[1])
+        assert ls.nb == ref.nb and ls.inside_outside == ref.inside_outside
+        assert torch.equal(ls.band, ref.band) and torch.equal(ls.flat, ref.flat)
+        print(f"gated={gated}: work set {need} nodes, retry equals LevelSet.reinitialize", flush=True)
+
+    ls = fresh()
+    bufs0 = [b.clone() for b in ls.bufs]
+    band0, nb0, occ0 = ls.band.clone(), ls.nb, ls.occupancy.clone()
+    rc, info = call(ls, min(ls.grid.size, 2 * ls.nb + 4096), gate_for(ls, False))
+    assert rc == _abi.OK and info[10] == 0 and info[0] == nb0
+    assert all(torch.equal(b, b0) for b, b0 in zip(ls.bufs, bufs0))
+    assert torch.equal(ls.band, band0) and torch.equal(ls.occupancy, occ0)
